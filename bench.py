@@ -1,0 +1,175 @@
+"""Throughput of the fused RK4 FDTD stepper on one MI355X.
+
+A step is one RK4 step (soft source + 4 fused stencil passes) over the whole
+grid.  Default workload: Klein-Gordon (psi, pi) plane wave through a
+400-atom random slab on a 256 x 256 x 512 grid (dx = dy = dz = 0.05 A,
+PML 16 planes per z face), complex64 fields, float32 V.  Inputs are resident
+in HBM before the timed region; the timed region is K steps bracketed by
+``torch.cuda.synchronize()``.
+
+Printed (one JSON line): Gvoxel-steps/s; the roofline of the dominant pass
+(algorithmic bytes per launch / its mean HIP-event duration, measured on the
+solver's stream); the CPU baseline (the FP64 NumPy oracle on a bounded slab
+of the same workload, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic HBM bytes per voxel for each pass of a step (4-pass fused schedule)
+PASS_BYTES = {
+    ("klein_gordon", "f32"): (52, 84, 84, 52),
+    ("klein_gordon", "f32_kahan"): (52, 84, 84, 84),
+    ("klein_gordon", "f64"): (104, 168, 168, 104),
+    ("schrodinger", "f32"): (28, 44, 44, 28),
+    ("schrodinger", "f32_kahan"): (28, 44, 44, 52),
+    ("schrodinger", "f64"): (56, 88, 88, 56),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f32_kahan", "f64"])
+    ap.add_argument("--equation", default="klein_gordon", choices=["klein_gordon", "schrodinger"])
+    ap.add_argument("--shape", default="512,256,256", help="nz,ny,nx")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(eq, shape, nsteps=1):
+    """FP64 NumPy oracle on a bounded slab of the workload: the same lateral size, 1/8 of
+    the planes, nsteps steps, single-threaded NumPy."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import fdtd3d
+    from wideanglepropagation_amd.workloads import kg_slab, schrodinger_gaussian_64
+    nz, ny, nx = shape
+    sub = (max(8, nz // 8), ny, nx)
+    if eq == "klein_gordon":
+        p, V = kg_slab(sub, precision="f64", n_atoms=50, device="cpu")
+        V = V.numpy()
+        st = fdtd3d.zero_state(p.oracle_dict())
+    else:
+        p, V, st, _ = schrodinger_gaussian_64("f64")
+        sub = p.shape
+    d = p.oracle_dict()
+    t = time.perf_counter()
+    fdtd3d.run(st, V, nsteps, d)
+    el = time.perf_counter() - t
+    vox = sub[0] * sub[1] * sub[2]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return {"value": vox * nsteps / el / 1e9, "unit": "Gvoxel-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"FP64 NumPy oracle, {eq} {sub[0]}x{sub[1]}x{sub[2]} (nz,ny,nx), "
+                      f"{nsteps} step(s), {el:.1f} s, 1 thread of {cores} host cores"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
+              "sharding is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from wideanglepropagation_amd.engine import FDTD3D
+    from wideanglepropagation_amd.workloads import kg_slab, centred_extent
+    from wideanglepropagation_amd.physics import schrodinger_params, wavelength_from_kinetic_energy
+
+    shape = tuple(int(v) for v in args.shape.split(","))
+    nz, ny, nx = shape
+    print(f"[bench] {args.equation} {args.precision} grid {nx}x{ny}x{nz}", file=sys.stderr,
+          flush=True)
+    if args.equation == "klein_gordon":
+        p, V = kg_slab(shape, precision=args.precision, device="cuda")
+    else:
+        lam = wavelength_from_kinetic_energy(200e3)
+        ext = centred_extent(shape, 0.05)
+        p = schrodinger_params(lam, ext, shape, pml_thick=16 * 0.05, k_source=18,
+                               precision=args.precision)
+        from wideanglepropagation_amd.workloads import atom_potential, random_slab
+        pos, Zs, scr = random_slab(400, ((ext[0], ext[1]), (ext[2], ext[3])), (-10, 10), 0)
+        V = atom_potential(shape, ext, pos, Zs, scr, 0.05, device="cuda")
+    p = p.with_(tile_y=args.tile_y, tile_z=args.tile_z)
+    V = V.to(torch.float64 if args.precision == "f64" else torch.float32).contiguous()
+    sim = FDTD3D(p, V=V)
+    del V
+    ty, tz, groups = sim.launch_shape()
+    print(f"[bench] launch tile_y={ty} tile_z={tz} workgroups/pass={groups} "
+          f"device bytes={sim.device_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+
+    sim.step(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    vox = p.voxels
+    value = vox * args.steps / elapsed / 1e9
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-pass HIP-event timing on the solver's stream
+    pass_ms, tot_ms = sim.step_profiled(args.profile_steps)
+    n_prof = max(1, args.profile_steps)
+    avg = [m / n_prof for m in pass_ms]
+    pb = PASS_BYTES[(p.equation, p.precision)]
+    dom = int(np.argmax([b / 1.0 for b in pb]))  # heaviest pass (P2)
+    dom_bytes = pb[dom] * vox
+    achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
+    step_bytes = sum(pb) * vox
+    step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
+    norm = sim.norm2()
+    sim.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p.equation, shape)
+
+    out = {
+        "metric": "Gvoxel-steps/s",
+        "value": round(value, 4),
+        "unit": "Gvoxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if p.precision == "f64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{p.equation} RK4 FDTD {nx}x{ny}x{nz}, "
+                               f"{'complex128' if p.precision == 'f64' else 'complex64'} fields "
+                               f"({p.precision}), 400-atom slab, PML 16 planes, 200 keV",
+                   "grid": [nz, ny, nx], "precision": p.precision,
+                   "equation": p.equation, "tile_y": ty, "tile_z": tz},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": f"rk4_pass stage {dom + 1}",
+                     "bytes_per_launch": dom_bytes,
+                     "mean_launch_ms": round(avg[dom], 5)},
+        "cpu_baseline": cpu,
+        "detail": {"pass_ms": [round(a, 5) for a in avg],
+                   "algorithmic_bytes_per_voxel_step": sum(pb),
+                   "step_effective_GBs": round(step_gbs, 1),
+                   "norm2_final": norm},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,160 @@
+/*
+ * wap_fdtd.h -- C ABI of the MI355X (gfx950) 3D RK4 FDTD electron-wave solver.
+ *
+ * The reference has no library interface for this path: its Python driver
+ * (wide_angle_propagation/fdtd_solver.py) rewrites #defines into a CUDA
+ * source, compiles it with nvcc, writes potential.bin, runs the program with
+ * argv [n_frames, steps_per_frame, snapshot_count, -|lambda|] and parses
+ * snapshots.bin.  Each entry point below names the part of that contract it
+ * replaces (file:line in the reference).  Everything is in-process, plain
+ * pointers and sizes; no torch types cross this boundary.
+ *
+ * Conventions
+ *   - Arrays are C order (nz, ny, nx), x contiguous (fdtd_3d.cu:81-83).
+ *   - Complex data is interleaved (re, im): c64 = 2 x float, c128 = 2 x double.
+ *     The reference's split phi/psi float arrays are phi = Re, psi = Im.
+ *   - Every call returns WAP_OK (0) or a negative status; the message of the
+ *     last failure on the calling thread is wap_last_error().  Nothing calls
+ *     exit() or abort() (the reference's gpuAssert exits, fdtd_3d.cu:71-78).
+ *   - Device work is ordered on the handle's stream (wap_set_stream, default
+ *     the null stream).  Calls that return data to host memory synchronise it.
+ *   - A handle is not thread-safe; distinct handles are independent.
+ */
+#ifndef WAP_FDTD_H
+#define WAP_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WAP_ABI_VERSION 1
+
+/* status codes */
+#define WAP_OK 0
+#define WAP_ERR_ARG (-1)
+#define WAP_ERR_HIP (-2)
+#define WAP_ERR_STATE (-3)
+#define WAP_ERR_OOM (-4)
+
+/* equations */
+#define WAP_EQ_SCHRODINGER 0  /* i dpsi/dt = (-A Lap + V) psi   (fdtd_3d.cu:242-248) */
+#define WAP_EQ_KLEIN_GORDON 1 /* first-order (psi, pi) system; no reference counterpart */
+
+/* arithmetic of the stepper */
+#define WAP_PREC_F64 0        /* complex128 fields, float64 V */
+#define WAP_PREC_F32 1        /* complex64 fields, float32 V (reference precision) */
+#define WAP_PREC_F32_KAHAN 2  /* complex64 fields + compensated state update */
+
+/* element types of caller buffers */
+#define WAP_DT_F32 0
+#define WAP_DT_F64 1
+#define WAP_DT_C64 2
+#define WAP_DT_C128 3
+
+/* time of step n */
+#define WAP_TIME_EXACT 0  /* t = t0 + (step0 + n) dt in float64 */
+#define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365 */
+
+#define WAP_MAX_SRC_BAND 8
+
+/*
+ * Run description.  Replaces the compile-time #defines of fdtd_3d.cu:17-67
+ * that fdtd_solver.py:285-374 rewrites, and the DT macro (fdtd_3d.cu:30).
+ */
+typedef struct wap_desc {
+  int32_t nx, ny, nz;         /* NX NY NZ */
+  int32_t equation;           /* WAP_EQ_* */
+  int32_t precision;          /* WAP_PREC_* */
+  int32_t time_mode;          /* WAP_TIME_* */
+  int32_t src_enabled;        /* 0: no soft source (the reference always injects) */
+  int32_t src_k;              /* K_SOURCE */
+  int32_t src_band;           /* SRC_BAND (<= WAP_MAX_SRC_BAND) */
+  int32_t tile_y;             /* rows per workgroup (64 x tile_y threads); 0 = auto */
+  int32_t tile_z;             /* planes per thread column; 0 = auto */
+  int32_t reserved0;
+  double xmin, xmax, ymin, ymax, zmin, zmax; /* XMIN..ZMAX; spacing = extent/(n-1) */
+  double dt;                  /* DT */
+  double lap_coeff;           /* LAP_COEFF = A (Schrodinger) */
+  double e_sim;               /* carrier frequency of the source phase (E_SIM) */
+  double k_beam;              /* source wavenumber sqrt(E_sim/A) (fdtd_3d.cu:139) */
+  double hbar_c;              /* KG: hbar c [eV A] */
+  double rest_energy;         /* KG: m c^2 [eV] */
+  double pml_thick, pml_sigma0, pml_theta, pml_power; /* PML_* (fdtd_3d.cu:64-67) */
+  double kg_gamma_max;        /* KG sponge damping rate at the outer face */
+  double src_alpha;           /* SRC_ALPHA */
+} wap_desc;
+
+typedef struct wap_handle wap_handle;
+
+/* Library ABI version (WAP_ABI_VERSION). */
+int wap_abi_version(void);
+
+/* Thread-local message of the last failure (replaces fdtd_3d.cu:71-78 stderr + exit). */
+const char *wap_last_error(void);
+
+/* Allocate device state for one grid (replaces fdtd_3d.cu:277-344: host/device
+ * allocation, zero initial field, launch geometry).  The field starts at 0. */
+int wap_create(const wap_desc *desc, wap_handle **out);
+
+/* Release every device buffer of the handle (fdtd_3d.cu:400 frees only 3 of 7). */
+int wap_destroy(wap_handle *h);
+
+/* Order all later work of the handle on this hipStream_t (NULL = null stream). */
+int wap_set_stream(wap_handle *h, void *stream);
+
+/* Upload V(r) (nz,ny,nx), real, eV (replaces potential.bin: fdtd_solver.py:416-419,
+ * fdtd_3d.cu:284-296).  dtype WAP_DT_F32 or WAP_DT_F64; on_device != 0 means the
+ * pointer is device memory.  Must be called before wap_step. */
+int wap_set_potential(wap_handle *h, const void *v, int dtype, int on_device);
+
+/* Set the state (replaces the zero init of fdtd_3d.cu:322-326).  psi (and pi for
+ * Klein-Gordon; NULL means 0) are complex (nz,ny,nx) of dtype WAP_DT_C64 or C128.
+ * psi == NULL resets the whole state to zero. */
+int wap_set_state(wap_handle *h, const void *psi, const void *pi, int dtype, int on_device);
+
+/* Copy the state out (replaces the per-frame D2H of fdtd_3d.cu:386-388).  pi may be
+ * NULL.  dtype WAP_DT_C64 or C128; the F32_KAHAN state is exported as hi - comp. */
+int wap_get_state(wap_handle *h, void *psi, void *pi, int dtype, int on_device);
+
+/* Lateral profile P(x,y) of the soft source, complex (ny,nx); NULL = plane wave. */
+int wap_set_source_profile(wap_handle *h, const void *prof, int dtype, int on_device);
+
+/* Advance nsteps steps (replaces the hot loop fdtd_3d.cu:363-382: memsets,
+ * inject_source_3d, 4 x rk4_stage_3d, rk4_final_3d).  Step n injects at
+ * t_n (time_mode) and then takes one classical RK4 step.  Asynchronous. */
+int wap_step(wap_handle *h, int64_t nsteps, int64_t step0, double t0);
+
+/* As wap_step, timing every pass with HIP events on the handle's stream.
+ * pass_ms[4] receives the summed device time of passes 1..4, total_ms the whole
+ * region; synchronises. */
+int wap_step_profiled(wap_handle *h, int64_t nsteps, int64_t step0, double t0,
+                      double *pass_ms, double *total_ms);
+
+/* Copy plane k of psi, complex (ny,nx), in dtype (C64/C128). */
+int wap_get_plane(wap_handle *h, int k, void *out, int dtype, int on_device);
+
+/* acc[j,i] += psi[k,j,i] * (cos_p + i sin_p) in float64 on device; acc is a device
+ * complex128 (ny,nx) buffer.  The demodulated running average of the notebooks'
+ * exit-wave recipe (fdtd_3D_baseline_comparison.ipynb:267-290) without dumping volumes. */
+int wap_accumulate_plane(wap_handle *h, int k, double cos_p, double sin_p, void *acc);
+
+/* sum |psi|^2 over the grid, accumulated in float64 on device; synchronises. */
+int wap_norm2(wap_handle *h, double *out);
+
+/* Wait for the handle's stream. */
+int wap_synchronize(wap_handle *h);
+
+/* Device bytes held by the handle. */
+int wap_device_bytes(wap_handle *h, int64_t *out);
+
+/* Effective launch shape chosen for the handle (tile_y, tile_z, workgroups per pass). */
+int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAP_FDTD_H */

@@ -1,0 +1,33 @@
+"""CPU oracle for the 3D RK4 FDTD electron-wave path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
+package.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
+(or, in ``bench.py``, as the timed CPU baseline), never as a code path of the
+solver.
+
+Contents
+--------
+physics   closed forms of the reference's constants
+          (``fdtd_solver.py:11-29, 306-342``) and its macro quantisation.
+fdtd3d    FP64 NumPy restatement of ``fdtd_3d.cu`` (race-free classical RK4,
+          same stencil, PML, soft source and frozen end planes) plus the
+          first-order Klein-Gordon extension.
+extract   the notebooks' exit-wave / far-field recipe
+          (``FDTDmethod/fdtd_3D_baseline_comparison.ipynb:267-290``,
+          ``propagation.py:346-350``).
+
+Pinning (what checks each function, in ``tests/``):
+  physics.*            reference-held golden scalars (notebook outputs and the
+                       checked-in macros of ``fdtd_3d.cu:18-67``).
+  fdtd3d.rk4_step      brute force: one step equals P4(-i dt H) applied to the
+                       injected state, with H assembled independently as a
+                       scipy.sparse matrix from the ``fdtd_3d.cu:192-248``
+                       formulae; closed-form discrete dispersion in a periodic
+                       box; bitwise determinism; the reference's five physics
+                       invariants (``tests/test_fdtd_3d.py:203-323``).
+  fdtd3d (KG)          closed-form KG dispersion in a periodic box
+                       (omega = V +- sqrt(c^2 k_d^2 + m^2)); charge
+                       conservation without sponge.
+  extract.*            synthetic snapshots with a known answer.
+"""

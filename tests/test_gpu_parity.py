@@ -1,0 +1,162 @@
+"""HIP stepper (through the C ABI) against the FP64 CPU oracle on the same inputs.
+
+Tolerances (relative L2 over the whole state):
+  f64        1e-12 after up to 200 steps (roundoff only)
+  f32        2e-6 after 20 steps (float32 arithmetic vs float64)
+  f32_kahan  2e-6 after 20 steps
+"""
+import numpy as np
+import pytest
+
+from oracle import fdtd3d
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.engine import FDTD3D
+    return FDTD3D
+
+
+def rel_l2(a, b):
+    a = np.concatenate([np.ravel(x) for x in (a if isinstance(a, tuple) else (a,))])
+    b = np.concatenate([np.ravel(x) for x in (b if isinstance(b, tuple) else (b,))])
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _params(shape, eq="schrodinger", precision="f64", **kw):
+    from wideanglepropagation_amd.physics import schrodinger_params, klein_gordon_params
+    from wideanglepropagation_amd.workloads import centred_extent
+    lam = 0.05
+    ext = centred_extent(shape, kw.pop("spacing", 0.05))
+    pml = kw.pop("pml_thick", 3 * 0.05)
+    ks = kw.pop("k_source", 4)
+    if eq == "schrodinger":
+        return schrodinger_params(lam, ext, shape, pml_thick=pml, k_source=ks,
+                                  precision=precision, **kw)
+    return klein_gordon_params(lam, ext, shape, pml_thick=pml, k_source=ks,
+                               precision=precision, **kw)
+
+
+def _random_case(p, rng, vscale=2e3):
+    shape = p.shape
+    V = vscale * rng.standard_normal(shape)
+    psi = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.5
+    if p.equation == "klein_gordon":
+        pi = -1j * p.e_sim * psi * (1 + 0.01 * rng.standard_normal(shape))
+        return V, (psi, pi)
+    return V, psi
+
+
+def _run_both(p, V, st0, nsteps, profile=None):
+    FDTD3D = _gpu()
+    if p.precision != "f64":
+        V = V.astype(np.float32)
+    sim = FDTD3D(p, V=V, source_profile=profile)
+    if isinstance(st0, tuple):
+        sim.set_state(st0[0], st0[1])
+    else:
+        sim.set_state(st0)
+    sim.step(nsteps)
+    got = sim.get_state(np.complex128)
+    sim.close()
+    want = fdtd3d.run(st0, V.astype(np.float64), nsteps, p.oracle_dict(src_profile=profile),
+                      f32_time=(p.time_mode == "f32"))
+    return got, want
+
+
+@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+def test_f64_matches_oracle(eq, rng):
+    print(f"case f64 {eq} 16x16x24", flush=True)
+    p = _params((24, 16, 16), eq)
+    V, st0 = _random_case(p, rng)
+    got, want = _run_both(p, V, st0, 20)
+    assert rel_l2(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("shape,ty,tz", [((9, 5, 70), 1, 1), ((9, 5, 70), 2, 3),
+                                          ((11, 13, 130), 8, 100), ((7, 3, 2), 4, 0)])
+def test_f64_ragged_shapes_and_tiles(shape, ty, tz, rng):
+    print(f"case ragged {shape} ty={ty} tz={tz}", flush=True)
+    p = _params(shape, tile_y=ty, tile_z=tz, pml_thick=0.1, k_source=2)
+    V, st0 = _random_case(p, rng)
+    got, want = _run_both(p, V, st0, 6)
+    assert rel_l2(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+@pytest.mark.parametrize("prec", ["f32", "f32_kahan"])
+def test_f32_matches_oracle(eq, prec, rng):
+    print(f"case {prec} {eq}", flush=True)
+    p = _params((24, 16, 16), eq, precision=prec)
+    V, st0 = _random_case(p, rng)
+    got, want = _run_both(p, V, st0, 20)
+    assert rel_l2(got, want) < 2e-6
+
+
+def test_source_profile_and_reference_clock(rng):
+    print("case source profile + f32 clock", flush=True)
+    p = _params((20, 12, 16), reference_quantize=True, k_source=5)
+    assert p.time_mode == "f32"
+    prof = np.exp(-np.linspace(-1, 1, 16)[None, :] ** 2 - np.linspace(-1, 1, 12)[:, None] ** 2)
+    prof = prof * np.exp(0.3j)
+    V, st0 = _random_case(p, rng)
+    got, want = _run_both(p, V, st0, 12, profile=prof.astype(np.complex128))
+    assert rel_l2(got, want) < 1e-12
+
+
+def test_schrodinger_gaussian_64_f64_200_steps():
+    from wideanglepropagation_amd.workloads import schrodinger_gaussian_64
+    print("case 64^3 wavepacket f64 200 steps", flush=True)
+    p, V, psi0, nsteps = schrodinger_gaussian_64("f64")
+    got, want = _run_both(p, V, psi0, nsteps)
+    assert rel_l2(got, want) < 1e-11
+
+
+def test_f32_is_bitwise_deterministic(rng):
+    FDTD3D = _gpu()
+    p = _params((40, 48, 96), precision="f32")
+    V, st0 = _random_case(p, rng)
+    outs = []
+    for _ in range(2):
+        sim = FDTD3D(p, V=V.astype(np.float32), psi0=st0)
+        sim.step(10)
+        outs.append(sim.get_state(np.complex64))
+        sim.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_norm_plane_and_accumulator(rng):
+    FDTD3D = _gpu()
+    p = _params((12, 10, 20), precision="f32_kahan")
+    V, st0 = _random_case(p, rng)
+    sim = FDTD3D(p, V=V.astype(np.float32), psi0=st0)
+    sim.step(3)
+    st = sim.get_state(np.complex128)
+    assert sim.norm2() == pytest.approx(float(np.sum(np.abs(st) ** 2)), rel=1e-12)
+    assert np.array_equal(sim.get_plane(5), st[5])
+    acc = torch.zeros((10, 20), dtype=torch.complex128, device="cuda")
+    sim.accumulate_plane(5, 0.7, acc)
+    sim.accumulate_plane(5, 0.7, acc)
+    assert np.allclose(acc.cpu().numpy(), 2 * st[5] * np.exp(0.7j), rtol=1e-14, atol=1e-14)
+    sim.close()
+
+
+def test_errors_are_reported_not_fatal():
+    FDTD3D = _gpu()
+    from wideanglepropagation_amd._lib import WapError
+    p = _params((12, 8, 8))
+    sim = FDTD3D(p)
+    with pytest.raises(WapError, match="potential not set"):
+        sim.step(1)
+    sim.close()
+    with pytest.raises(WapError, match="grid must be"):
+        FDTD3D(p.with_(nz=2))
+    with pytest.raises(WapError, match="source plane"):
+        FDTD3D(p.with_(src_k=99))
+    with pytest.raises(ValueError):
+        FDTD3D(p, V=np.zeros(7))

@@ -1,0 +1,28 @@
+"""MI355X-native 3D RK4 FDTD electron-wave solver.
+
+Drop-in for the ``fdtd_3d.cu`` / ``fdtd_solver.py`` path of
+DavidLanders95/WideAnglePropagation: the Python API of ``fdtd_solver``
+(``RelativisticSolver3D``, ``calculate_physics_params``) over hand-written
+gfx950 HIP kernels reached through the C ABI of ``libwapfdtd.so``
+(``include/wap_fdtd.h``).
+
+Modules
+  fdtd_solver  the reference's Python API (drop-in)
+  physics      constants, ``FDTDParams`` and their builders
+  engine       ``FDTD3D``: one device-resident grid (C-ABI wrapper)
+  build        hipcc build of ``libwapfdtd.so``
+"""
+from .physics import (SOL_LIGHT, MASS_PART, BOHR_RADIUS_ANGSTROM,  # noqa: F401
+                      calculate_physics_params, FDTDParams, schrodinger_params,
+                      klein_gordon_params, beam_constants)
+from .fdtd_solver import RelativisticSolver3D  # noqa: F401
+
+__version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # device-side classes load torch and the HIP library only when asked for
+    if name == "FDTD3D":
+        from .engine import FDTD3D
+        return FDTD3D
+    raise AttributeError(name)

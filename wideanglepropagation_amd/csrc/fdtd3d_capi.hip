@@ -1,0 +1,723 @@
+// C ABI of the gfx950 3D RK4 FDTD stepper (declared in include/wap_fdtd.h).
+//
+// The handle owns every device buffer of one grid.  Per-plane coefficient
+// tables (PML weights of fdtd_3d.cu:96-123 and :215-234, the KG sponge) and
+// the soft-source phasors (fdtd_3d.cu:139-156) are evaluated on the host in
+// float64 once per handle / once per step, so no transcendental runs per
+// voxel on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/wap_fdtd.h"
+#include "fdtd3d_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define WAP_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? WAP_ERR_OOM : WAP_ERR_HIP,               \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+}  // namespace
+
+struct wap_handle {
+  wap_desc d{};
+  size_t n = 0;       // voxels
+  size_t plane = 0;   // nx*ny
+  size_t cs = 0;      // bytes per complex element
+  size_t rs = 0;      // bytes per real element
+  hipStream_t stream = nullptr;
+  void *psi = nullptr, *pi = nullptr;
+  void *acc_psi = nullptr, *acc_pi = nullptr;
+  void *ya_psi = nullptr, *ya_pi = nullptr;
+  void *yb_psi = nullptr, *yb_pi = nullptr;
+  void *cmp_psi = nullptr, *cmp_pi = nullptr;
+  void *V = nullptr;
+  void *zc = nullptr;
+  void *gam = nullptr;
+  void *prof = nullptr;
+  double *partial = nullptr;
+  double *dsum = nullptr;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  int64_t bytes = 0;
+  bool v_set = false;
+  int tile_y = 4, tile_z = 8;
+  int ntx = 1, nty = 1, ntz = 1;
+  double hx = 0, hy = 0, m2 = 0;
+  int nsrc = 0;
+  int src_k0 = 0;
+  double src_keep[wap::kMaxBand] = {};
+  std::complex<double> src_spatial[wap::kMaxBand];
+  int n_partial = 0;
+};
+
+namespace {
+
+bool is_kg(const wap_handle *h) { return h->d.equation == WAP_EQ_KLEIN_GORDON; }
+bool is_f64(const wap_handle *h) { return h->d.precision == WAP_PREC_F64; }
+bool is_kahan(const wap_handle *h) { return h->d.precision == WAP_PREC_F32_KAHAN; }
+
+int dev_alloc(wap_handle *h, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? WAP_ERR_OOM : WAP_ERR_HIP,
+                std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  h->bytes += (int64_t)bytes;
+  return WAP_OK;
+}
+
+void free_all(wap_handle *h) {
+  void **bufs[] = {&h->psi, &h->pi, &h->acc_psi, &h->acc_pi, &h->ya_psi, &h->ya_pi,
+                   &h->yb_psi, &h->yb_pi, &h->cmp_psi, &h->cmp_pi, &h->V, &h->zc,
+                   &h->gam, &h->prof, (void **)&h->partial, (void **)&h->dsum, &h->staging};
+  for (void **b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+}
+
+int ensure_staging(wap_handle *h, size_t bytes) {
+  if (h->staging_bytes >= bytes) return WAP_OK;
+  if (h->staging) {
+    WAP_HIP(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->staging);
+    h->bytes -= (int64_t)h->staging_bytes;
+    h->staging = nullptr;
+    h->staging_bytes = 0;
+  }
+  int rc = dev_alloc(h, &h->staging, bytes);
+  if (rc) return rc;
+  h->staging_bytes = bytes;
+  return WAP_OK;
+}
+
+int grid_1d(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// Per-plane coefficients, float64 on the host.
+int build_tables(wap_handle *h) {
+  const wap_desc &d = h->d;
+  const double dx = (d.xmax - d.xmin) / (d.nx - 1.0);
+  const double dy = (d.ymax - d.ymin) / (d.ny - 1.0);
+  const double dz = (d.zmax - d.zmin) / (d.nz - 1.0);
+  const int nz = d.nz;
+  std::vector<double> ramp(nz), z(nz);
+  for (int k = 0; k < nz; ++k) {
+    z[k] = d.zmin + k * (d.zmax - d.zmin) / (nz - 1.0);
+    const double rb = std::fmax((d.zmin + d.pml_thick) - z[k], 0.0) / d.pml_thick;
+    const double rt = std::fmax(z[k] - (d.zmax - d.pml_thick), 0.0) / d.pml_thick;
+    ramp[k] = std::fmax(rb, rt);
+  }
+  std::vector<std::complex<double>> a(nz, std::complex<double>(1.0, 0.0));
+  if (!is_kg(h)) {
+    for (int k = 0; k < nz; ++k) {
+      if (ramp[k] > 0.0) {
+        const double sig = d.pml_sigma0 * std::pow(ramp[k], d.pml_power);
+        if (sig > 0.0) a[k] = 1.0 / (1.0 + sig * std::polar(1.0, d.pml_theta));
+      }
+    }
+  }
+  const double scale = is_kg(h) ? d.hbar_c * d.hbar_c : -d.lap_coeff;
+  h->hx = scale / (dx * dx);
+  h->hy = scale / (dy * dy);
+  h->m2 = d.rest_energy * d.rest_energy;
+  std::vector<std::complex<double>> zc(2 * (size_t)nz, std::complex<double>(0, 0));
+  for (int k = 1; k + 1 < nz; ++k) {
+    zc[2 * k] = scale * a[k] * 0.5 * (a[k] + a[k + 1]) / (dz * dz);
+    zc[2 * k + 1] = scale * a[k] * 0.5 * (a[k - 1] + a[k]) / (dz * dz);
+  }
+  std::vector<double> gam(nz, 0.0);
+  if (is_kg(h))
+    for (int k = 0; k < nz; ++k)
+      if (ramp[k] > 0.0) gam[k] = d.kg_gamma_max * std::pow(ramp[k], d.pml_power);
+
+  // upload in the stepper's precision
+  if (is_f64(h)) {
+    WAP_HIP(hipMemcpy(h->zc, zc.data(), zc.size() * 16, hipMemcpyHostToDevice));
+    WAP_HIP(hipMemcpy(h->gam, gam.data(), gam.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> zf(4 * (size_t)nz), gf(nz);
+    for (size_t t = 0; t < zc.size(); ++t) {
+      zf[2 * t] = (float)zc[t].real();
+      zf[2 * t + 1] = (float)zc[t].imag();
+    }
+    for (int k = 0; k < nz; ++k) gf[k] = (float)gam[k];
+    WAP_HIP(hipMemcpy(h->zc, zf.data(), zf.size() * 4, hipMemcpyHostToDevice));
+    WAP_HIP(hipMemcpy(h->gam, gf.data(), gf.size() * 4, hipMemcpyHostToDevice));
+  }
+
+  // soft source: window weights and spatial phasors (fdtd_3d.cu:134-156)
+  h->nsrc = 0;
+  h->src_k0 = d.src_k;
+  if (d.src_enabled) {
+    const double zK = d.zmin + d.src_k * (d.zmax - d.zmin) / (nz - 1.0);
+    for (int b = 0; b < d.src_band; ++b) {
+      const int k = d.src_k + b;
+      const double s = d.src_band > 1 ? (double)b / (d.src_band - 1.0) : 0.0;
+      const double win = 0.5 * (1.0 - std::cos(M_PI * s));
+      const double al = d.src_alpha * win;
+      const double zk = d.zmin + k * (d.zmax - d.zmin) / (nz - 1.0);
+      h->src_keep[b] = 1.0 - al;
+      h->src_spatial[b] = al * std::polar(1.0, d.k_beam * (zk - zK));
+    }
+    h->nsrc = d.src_band;
+  }
+  return WAP_OK;
+}
+
+double step_time(const wap_handle *h, int64_t step0, int64_t n, double t0) {
+  if (h->d.time_mode == WAP_TIME_F32) {
+    const float sf = (float)(step0 + n);
+    return (double)(sf * (float)h->d.dt);
+  }
+  return t0 + (double)(step0 + n) * h->d.dt;
+}
+
+template <typename R>
+void fill_source(const wap_handle *h, double t, wap::PassArgs<R> &a) {
+  a.src_k0 = h->src_k0;
+  a.src_n = h->nsrc;
+  const std::complex<double> ph = std::polar(1.0, -h->d.e_sim * t);
+  const std::complex<double> mi_w(0.0, -h->d.e_sim);
+  for (int b = 0; b < wap::kMaxBand; ++b) {
+    if (b < h->nsrc) {
+      const std::complex<double> c = h->src_spatial[b] * ph;
+      const std::complex<double> cq = mi_w * c;
+      a.src_keep[b] = (R)h->src_keep[b];
+      a.src_coef[b] = wap::cplx<R>{(R)c.real(), (R)c.imag()};
+      a.src_coef_pi[b] = wap::cplx<R>{(R)cq.real(), (R)cq.imag()};
+    } else {
+      a.src_keep[b] = (R)1;
+      a.src_coef[b] = wap::cplx<R>{(R)0, (R)0};
+      a.src_coef_pi[b] = wap::cplx<R>{(R)0, (R)0};
+    }
+  }
+}
+
+template <typename R>
+wap::PassArgs<R> base_args(const wap_handle *h) {
+  using C = wap::cplx<R>;
+  wap::PassArgs<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.gam = (const R *)h->gam;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx;
+  a.ny = h->d.ny;
+  a.nz = h->d.nz;
+  a.tile_y = h->tile_y;
+  a.tile_z = h->tile_z;
+  a.ntx = h->ntx;
+  a.nty = h->nty;
+  a.ntz = h->ntz;
+  a.hx = (R)h->hx;
+  a.hy = (R)h->hy;
+  a.m2 = (R)h->m2;
+  a.st_psi = (const C *)h->psi;
+  a.st_pi = (const C *)h->pi;
+  a.acc_psi = (C *)h->acc_psi;
+  a.acc_pi = (C *)h->acc_pi;
+  a.cmp_psi = (C *)h->cmp_psi;
+  a.cmp_pi = (C *)h->cmp_pi;
+  return a;
+}
+
+template <typename R, int EQ, bool K>
+int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+              double *pass_ms) {
+  using C = wap::cplx<R>;
+  const dim3 block(wap::kWaveX, h->tile_y);
+  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
+  const R dt = (R)h->d.dt;
+  wap::PassArgs<R> a = base_args<R>(h);
+  C *psi = (C *)h->psi, *pi = (C *)h->pi;
+  C *ya = (C *)h->ya_psi, *yaq = (C *)h->ya_pi;
+  C *yb = (C *)h->yb_psi, *ybq = (C *)h->yb_pi;
+  for (int64_t n = 0; n < nsteps; ++n) {
+    fill_source<R>(h, step_time(h, step0, n, t0), a);
+    if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
+    // P1: psi_n -> y2, acc
+    a.yin_psi = psi; a.yin_pi = pi; a.out_psi = ya; a.out_pi = yaq;
+    a.w_acc = dt / (R)6; a.w_y = dt / (R)2;
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 1, K>), grid, block, 0, h->stream, a);
+    if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
+    // P2: y2 -> y3
+    a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = yb; a.out_pi = ybq;
+    a.w_acc = dt / (R)3; a.w_y = dt / (R)2;
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 2, K>), grid, block, 0, h->stream, a);
+    if (ev) WAP_HIP(hipEventRecord(ev[2], h->stream));
+    // P3: y3 -> y4
+    a.yin_psi = yb; a.yin_pi = ybq; a.out_psi = ya; a.out_pi = yaq;
+    a.w_acc = dt / (R)3; a.w_y = dt;
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 3, K>), grid, block, 0, h->stream, a);
+    if (ev) WAP_HIP(hipEventRecord(ev[3], h->stream));
+    // P4: y4 -> psi_{n+1}
+    a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = psi; a.out_pi = pi;
+    a.w_acc = dt / (R)6; a.w_y = (R)0;
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 4, K>), grid, block, 0, h->stream, a);
+    if (ev) {
+      WAP_HIP(hipEventRecord(ev[4], h->stream));
+      WAP_HIP(hipEventSynchronize(ev[4]));
+      for (int s = 0; s < 4; ++s) {
+        float ms = 0.f;
+        WAP_HIP(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+        pass_ms[s] += ms;
+      }
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                   double *pass_ms) {
+  const bool kg = is_kg(h);
+  switch (h->d.precision) {
+    case WAP_PREC_F64:
+      return kg ? run_steps<double, 1, false>(h, nsteps, step0, t0, ev, pass_ms)
+                : run_steps<double, 0, false>(h, nsteps, step0, t0, ev, pass_ms);
+    case WAP_PREC_F32:
+      return kg ? run_steps<float, 1, false>(h, nsteps, step0, t0, ev, pass_ms)
+                : run_steps<float, 0, false>(h, nsteps, step0, t0, ev, pass_ms);
+    default:
+      return kg ? run_steps<float, 1, true>(h, nsteps, step0, t0, ev, pass_ms)
+                : run_steps<float, 0, true>(h, nsteps, step0, t0, ev, pass_ms);
+  }
+}
+
+void choose_shape(wap_handle *h) {
+  const wap_desc &d = h->d;
+  int ty = d.tile_y > 0 ? d.tile_y : 4;
+  if (ty > 8) ty = 8;
+  h->tile_y = ty;
+  h->ntx = (d.nx + wap::kWaveX - 1) / wap::kWaveX;
+  h->nty = (d.ny + ty - 1) / ty;
+  int tz = d.tile_z;
+  if (tz <= 0) {
+    // aim for >= 2048 workgroups (8 per CU) with the longest z run that allows
+    const long cols = (long)h->ntx * h->nty;
+    long t = (long)d.nz * cols / 2048;
+    if (t < 2) t = 2;
+    if (t > 32) t = 32;
+    tz = (int)t;
+  }
+  if (tz > d.nz) tz = d.nz;
+  h->tile_z = tz;
+  h->ntz = (d.nz + tz - 1) / tz;
+}
+
+int validate(const wap_desc *d) {
+  if (!d) return fail(WAP_ERR_ARG, "desc is NULL");
+  if (d->nx < 2 || d->ny < 2 || d->nz < 3)
+    return fail(WAP_ERR_ARG, "grid must be at least 2 x 2 x 3 (nx, ny >= 2; nz >= 3)");
+  if ((int64_t)d->nx * d->ny > (int64_t)1 << 30)
+    return fail(WAP_ERR_ARG, "nx*ny exceeds the 32-bit in-plane index range");
+  if (d->equation != WAP_EQ_SCHRODINGER && d->equation != WAP_EQ_KLEIN_GORDON)
+    return fail(WAP_ERR_ARG, "unknown equation");
+  if (d->precision < WAP_PREC_F64 || d->precision > WAP_PREC_F32_KAHAN)
+    return fail(WAP_ERR_ARG, "unknown precision");
+  if (d->time_mode != WAP_TIME_EXACT && d->time_mode != WAP_TIME_F32)
+    return fail(WAP_ERR_ARG, "unknown time_mode");
+  if (!(d->xmax > d->xmin) || !(d->ymax > d->ymin) || !(d->zmax > d->zmin))
+    return fail(WAP_ERR_ARG, "extent must be increasing on every axis");
+  if (!(d->dt > 0.0)) return fail(WAP_ERR_ARG, "dt must be > 0");
+  if (!(d->pml_thick > 0.0)) return fail(WAP_ERR_ARG, "pml_thick must be > 0");
+  if (d->src_enabled) {
+    if (d->src_band < 1 || d->src_band > WAP_MAX_SRC_BAND)
+      return fail(WAP_ERR_ARG, "src_band must be in [1, 8]");
+    if (d->src_k < 0 || d->src_k >= d->nz)
+      return fail(WAP_ERR_ARG, "source plane k_source must lie inside the grid");
+  }
+  if (d->tile_y < 0 || d->tile_y > 8 || (d->tile_y & (d->tile_y - 1)))
+    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8");
+  if (d->tile_z < 0) return fail(WAP_ERR_ARG, "tile_z must be >= 0");
+  return WAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wap_abi_version(void) { return WAP_ABI_VERSION; }
+
+const char *wap_last_error(void) { return g_err.c_str(); }
+
+int wap_create(const wap_desc *desc, wap_handle **out) {
+  if (!out) return fail(WAP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  wap_handle *h = new (std::nothrow) wap_handle();
+  if (!h) return fail(WAP_ERR_OOM, "host allocation failed");
+  h->d = *desc;
+  h->plane = (size_t)desc->nx * desc->ny;
+  h->n = h->plane * desc->nz;
+  h->cs = is_f64(h) ? 16 : 8;
+  h->rs = is_f64(h) ? 8 : 4;
+  choose_shape(h);
+  const size_t fb = h->n * h->cs;
+  const bool kg = is_kg(h);
+  void **fields[] = {&h->psi, &h->acc_psi, &h->ya_psi, &h->yb_psi};
+  void **fields_pi[] = {&h->pi, &h->acc_pi, &h->ya_pi, &h->yb_pi};
+  for (int s = 0; s < 4 && !rc; ++s) rc = dev_alloc(h, fields[s], fb);
+  if (kg)
+    for (int s = 0; s < 4 && !rc; ++s) rc = dev_alloc(h, fields_pi[s], fb);
+  if (!rc && is_kahan(h)) {
+    rc = dev_alloc(h, &h->cmp_psi, fb);
+    if (!rc && kg) rc = dev_alloc(h, &h->cmp_pi, fb);
+  }
+  if (!rc) rc = dev_alloc(h, &h->V, h->n * h->rs);
+  if (!rc) rc = dev_alloc(h, &h->zc, 2 * (size_t)desc->nz * h->cs);
+  if (!rc) rc = dev_alloc(h, &h->gam, (size_t)desc->nz * h->rs);
+  h->n_partial = grid_1d(h->n) < 1024 ? grid_1d(h->n) : 1024;
+  if (!rc) rc = dev_alloc(h, (void **)&h->partial, (size_t)h->n_partial * 8);
+  if (!rc) rc = dev_alloc(h, (void **)&h->dsum, 8);
+  if (!rc) rc = build_tables(h);
+  if (!rc) {
+    // zero state (fdtd_3d.cu:322-326); V must be set explicitly
+    void *zero_bufs[] = {h->psi, h->pi, h->cmp_psi, h->cmp_pi};
+    for (void *b : zero_bufs) {
+      if (!b) continue;
+      hipError_t e = hipMemset(b, 0, fb);
+      if (e != hipSuccess) {
+        rc = fail(WAP_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+        break;
+      }
+    }
+  }
+  if (rc) {
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return WAP_OK;
+}
+
+int wap_destroy(wap_handle *h) {
+  if (!h) return WAP_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  else (void)hipDeviceSynchronize();
+  free_all(h);
+  delete h;
+  return WAP_OK;
+}
+
+int wap_set_stream(wap_handle *h, void *stream) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  h->stream = (hipStream_t)stream;
+  return WAP_OK;
+}
+
+int wap_set_potential(wap_handle *h, const void *v, int dtype, int on_device) {
+  if (!h || !v) return fail(WAP_ERR_ARG, "handle or V is NULL");
+  if (dtype != WAP_DT_F32 && dtype != WAP_DT_F64)
+    return fail(WAP_ERR_ARG, "V dtype must be WAP_DT_F32 or WAP_DT_F64");
+  const size_t in_b = (dtype == WAP_DT_F64 ? 8 : 4);
+  const bool same = (in_b == h->rs);
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (same) {
+    WAP_HIP(hipMemcpyAsync(h->V, v, h->n * in_b, kind, h->stream));
+  } else {
+    const void *src = v;
+    if (!on_device) {
+      int rc = ensure_staging(h, h->n * in_b);
+      if (rc) return rc;
+      WAP_HIP(hipMemcpyAsync(h->staging, v, h->n * in_b, hipMemcpyHostToDevice, h->stream));
+      src = h->staging;
+    }
+    if (dtype == WAP_DT_F32)
+      hipLaunchKernelGGL((wap::import_real<float, double>), dim3(grid_1d(h->n)), dim3(256), 0,
+                         h->stream, (const float *)src, (double *)h->V, h->n);
+    else
+      hipLaunchKernelGGL((wap::import_real<double, float>), dim3(grid_1d(h->n)), dim3(256), 0,
+                         h->stream, (const double *)src, (float *)h->V, h->n);
+    WAP_HIP(hipGetLastError());
+  }
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  h->v_set = true;
+  return WAP_OK;
+}
+
+namespace {
+int import_one(wap_handle *h, const void *src, int dtype, int on_device, void *dst_hi,
+               void *dst_cmp) {
+  const size_t in_b = (dtype == WAP_DT_C128 ? 16 : 8);
+  if (!src) {
+    WAP_HIP(hipMemsetAsync(dst_hi, 0, h->n * h->cs, h->stream));
+    if (dst_cmp) WAP_HIP(hipMemsetAsync(dst_cmp, 0, h->n * h->cs, h->stream));
+    return WAP_OK;
+  }
+  const void *s = src;
+  if (!on_device) {
+    int rc = ensure_staging(h, h->n * in_b);
+    if (rc) return rc;
+    WAP_HIP(hipMemcpyAsync(h->staging, src, h->n * in_b, hipMemcpyHostToDevice, h->stream));
+    s = h->staging;
+  }
+  const dim3 g(grid_1d(h->n)), b(256);
+  if (is_f64(h)) {
+    if (dtype == WAP_DT_C128)
+      WAP_HIP(hipMemcpyAsync(dst_hi, s, h->n * 16, hipMemcpyDeviceToDevice, h->stream));
+    else
+      hipLaunchKernelGGL((wap::import_state<float, double>), g, b, 0, h->stream,
+                         (const float2 *)s, (double2 *)dst_hi, (double2 *)nullptr, h->n);
+  } else {
+    if (dtype == WAP_DT_C128)
+      hipLaunchKernelGGL((wap::import_state<double, float>), g, b, 0, h->stream,
+                         (const double2 *)s, (float2 *)dst_hi, (float2 *)dst_cmp, h->n);
+    else {
+      WAP_HIP(hipMemcpyAsync(dst_hi, s, h->n * 8, hipMemcpyDeviceToDevice, h->stream));
+      if (dst_cmp) WAP_HIP(hipMemsetAsync(dst_cmp, 0, h->n * 8, h->stream));
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  // the staging buffer is reused by the next import: finish this one first
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  return WAP_OK;
+}
+
+int export_one(wap_handle *h, void *dst, int dtype, int on_device, const void *src_hi,
+               const void *src_cmp) {
+  const size_t out_b = (dtype == WAP_DT_C128 ? 16 : 8);
+  const bool direct = (out_b == h->cs) && !src_cmp;
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (direct) {
+    WAP_HIP(hipMemcpyAsync(dst, src_hi, h->n * out_b, kind, h->stream));
+  } else {
+    void *d = dst;
+    if (!on_device) {
+      int rc = ensure_staging(h, h->n * out_b);
+      if (rc) return rc;
+      d = h->staging;
+    }
+    const dim3 g(grid_1d(h->n)), b(256);
+    if (is_f64(h)) {
+      if (dtype == WAP_DT_C64)
+        hipLaunchKernelGGL((wap::export_state<double, float>), g, b, 0, h->stream,
+                           (const double2 *)src_hi, (const double2 *)nullptr, (float2 *)d, h->n);
+      else
+        hipLaunchKernelGGL((wap::export_state<double, double>), g, b, 0, h->stream,
+                           (const double2 *)src_hi, (const double2 *)nullptr, (double2 *)d, h->n);
+    } else {
+      if (dtype == WAP_DT_C64)
+        hipLaunchKernelGGL((wap::export_state<float, float>), g, b, 0, h->stream,
+                           (const float2 *)src_hi, (const float2 *)src_cmp, (float2 *)d, h->n);
+      else
+        hipLaunchKernelGGL((wap::export_state<float, double>), g, b, 0, h->stream,
+                           (const float2 *)src_hi, (const float2 *)src_cmp, (double2 *)d, h->n);
+    }
+    WAP_HIP(hipGetLastError());
+    if (!on_device)
+      WAP_HIP(hipMemcpyAsync(dst, h->staging, h->n * out_b, hipMemcpyDeviceToHost, h->stream));
+  }
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  return WAP_OK;
+}
+}  // namespace
+
+int wap_set_state(wap_handle *h, const void *psi, const void *pi, int dtype, int on_device) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (dtype != WAP_DT_C64 && dtype != WAP_DT_C128)
+    return fail(WAP_ERR_ARG, "state dtype must be WAP_DT_C64 or WAP_DT_C128");
+  int rc = import_one(h, psi, dtype, on_device, h->psi, h->cmp_psi);
+  if (rc) return rc;
+  if (is_kg(h)) rc = import_one(h, psi ? pi : nullptr, dtype, on_device, h->pi, h->cmp_pi);
+  return rc;
+}
+
+int wap_get_state(wap_handle *h, void *psi, void *pi, int dtype, int on_device) {
+  if (!h || !psi) return fail(WAP_ERR_ARG, "handle or psi is NULL");
+  if (dtype != WAP_DT_C64 && dtype != WAP_DT_C128)
+    return fail(WAP_ERR_ARG, "state dtype must be WAP_DT_C64 or WAP_DT_C128");
+  int rc = export_one(h, psi, dtype, on_device, h->psi, h->cmp_psi);
+  if (rc) return rc;
+  if (pi) {
+    if (!is_kg(h)) return fail(WAP_ERR_ARG, "pi requested from a Schrodinger handle");
+    rc = export_one(h, pi, dtype, on_device, h->pi, h->cmp_pi);
+  }
+  return rc;
+}
+
+int wap_set_source_profile(wap_handle *h, const void *prof, int dtype, int on_device) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (!prof) {
+    if (h->prof) {
+      WAP_HIP(hipStreamSynchronize(h->stream));
+      (void)hipFree(h->prof);
+      h->bytes -= (int64_t)(h->plane * h->cs);
+      h->prof = nullptr;
+    }
+    return WAP_OK;
+  }
+  if (dtype != WAP_DT_C64 && dtype != WAP_DT_C128)
+    return fail(WAP_ERR_ARG, "profile dtype must be WAP_DT_C64 or WAP_DT_C128");
+  if (!h->prof) {
+    int rc = dev_alloc(h, &h->prof, h->plane * h->cs);
+    if (rc) return rc;
+  }
+  // convert on the host side (one plane)
+  const size_t in_b = (dtype == WAP_DT_C128 ? 16 : 8);
+  std::vector<unsigned char> hostbuf(h->plane * in_b);
+  if (on_device)
+    WAP_HIP(hipMemcpy(hostbuf.data(), prof, hostbuf.size(), hipMemcpyDeviceToHost));
+  else
+    std::memcpy(hostbuf.data(), prof, hostbuf.size());
+  std::vector<unsigned char> conv(h->plane * h->cs);
+  for (size_t t = 0; t < 2 * h->plane; ++t) {
+    const double v = (in_b == 16) ? ((const double *)hostbuf.data())[t]
+                                  : (double)((const float *)hostbuf.data())[t];
+    if (h->cs == 16) ((double *)conv.data())[t] = v;
+    else ((float *)conv.data())[t] = (float)v;
+  }
+  WAP_HIP(hipMemcpyAsync(h->prof, conv.data(), conv.size(), hipMemcpyHostToDevice, h->stream));
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  return WAP_OK;
+}
+
+int wap_step(wap_handle *h, int64_t nsteps, int64_t step0, double t0) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  if (nsteps < 0) return fail(WAP_ERR_ARG, "nsteps < 0");
+  return dispatch_steps(h, nsteps, step0, t0, nullptr, nullptr);
+}
+
+int wap_step_profiled(wap_handle *h, int64_t nsteps, int64_t step0, double t0, double *pass_ms,
+                      double *total_ms) {
+  if (!h || !pass_ms || !total_ms) return fail(WAP_ERR_ARG, "NULL argument");
+  if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  hipEvent_t ev[5], t_start, t_stop;
+  for (int s = 0; s < 5; ++s) WAP_HIP(hipEventCreate(&ev[s]));
+  WAP_HIP(hipEventCreate(&t_start));
+  WAP_HIP(hipEventCreate(&t_stop));
+  for (int s = 0; s < 4; ++s) pass_ms[s] = 0.0;
+  WAP_HIP(hipEventRecord(t_start, h->stream));
+  int rc = dispatch_steps(h, nsteps, step0, t0, ev, pass_ms);
+  if (!rc) {
+    WAP_HIP(hipEventRecord(t_stop, h->stream));
+    WAP_HIP(hipEventSynchronize(t_stop));
+    float ms = 0.f;
+    WAP_HIP(hipEventElapsedTime(&ms, t_start, t_stop));
+    *total_ms = ms;
+  }
+  for (int s = 0; s < 5; ++s) (void)hipEventDestroy(ev[s]);
+  (void)hipEventDestroy(t_start);
+  (void)hipEventDestroy(t_stop);
+  return rc;
+}
+
+int wap_get_plane(wap_handle *h, int k, void *out, int dtype, int on_device) {
+  if (!h || !out) return fail(WAP_ERR_ARG, "handle or out is NULL");
+  if (k < 0 || k >= h->d.nz) return fail(WAP_ERR_ARG, "plane index out of range");
+  if (dtype != WAP_DT_C64 && dtype != WAP_DT_C128)
+    return fail(WAP_ERR_ARG, "plane dtype must be WAP_DT_C64 or WAP_DT_C128");
+  int rc = ensure_staging(h, h->plane * 16);
+  if (rc) return rc;
+  WAP_HIP(hipMemsetAsync(h->staging, 0, h->plane * 16, h->stream));
+  const int g = (int)((h->plane + 255) / 256);
+  const size_t off = (size_t)k * h->plane;
+  if (is_f64(h))
+    hipLaunchKernelGGL((wap::accumulate_plane<double>), dim3(g), dim3(256), 0, h->stream,
+                       (const double2 *)h->psi, (const double2 *)nullptr, off, (int)h->plane, 1.0,
+                       0.0, (double2 *)h->staging);
+  else
+    hipLaunchKernelGGL((wap::accumulate_plane<float>), dim3(g), dim3(256), 0, h->stream,
+                       (const float2 *)h->psi, (const float2 *)h->cmp_psi, off, (int)h->plane, 1.0,
+                       0.0, (double2 *)h->staging);
+  WAP_HIP(hipGetLastError());
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (dtype == WAP_DT_C128) {
+    WAP_HIP(hipMemcpyAsync(out, h->staging, h->plane * 16, kind, h->stream));
+    WAP_HIP(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<double> tmp(2 * h->plane);
+    WAP_HIP(hipMemcpyAsync(tmp.data(), h->staging, h->plane * 16, hipMemcpyDeviceToHost,
+                           h->stream));
+    WAP_HIP(hipStreamSynchronize(h->stream));
+    std::vector<float> f(2 * h->plane);
+    for (size_t t = 0; t < f.size(); ++t) f[t] = (float)tmp[t];
+    if (on_device) WAP_HIP(hipMemcpy(out, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    else std::memcpy(out, f.data(), f.size() * 4);
+  }
+  return WAP_OK;
+}
+
+int wap_accumulate_plane(wap_handle *h, int k, double cos_p, double sin_p, void *acc) {
+  if (!h || !acc) return fail(WAP_ERR_ARG, "handle or acc is NULL");
+  if (k < 0 || k >= h->d.nz) return fail(WAP_ERR_ARG, "plane index out of range");
+  const int g = (int)((h->plane + 255) / 256);
+  const size_t off = (size_t)k * h->plane;
+  if (is_f64(h))
+    hipLaunchKernelGGL((wap::accumulate_plane<double>), dim3(g), dim3(256), 0, h->stream,
+                       (const double2 *)h->psi, (const double2 *)nullptr, off, (int)h->plane,
+                       cos_p, sin_p, (double2 *)acc);
+  else
+    hipLaunchKernelGGL((wap::accumulate_plane<float>), dim3(g), dim3(256), 0, h->stream,
+                       (const float2 *)h->psi, (const float2 *)h->cmp_psi, off, (int)h->plane,
+                       cos_p, sin_p, (double2 *)acc);
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+int wap_norm2(wap_handle *h, double *out) {
+  if (!h || !out) return fail(WAP_ERR_ARG, "handle or out is NULL");
+  const int g = h->n_partial;
+  if (is_f64(h))
+    hipLaunchKernelGGL((wap::norm2_partial<double>), dim3(g), dim3(256), 0, h->stream,
+                       (const double2 *)h->psi, (const double2 *)nullptr, h->n, h->partial);
+  else
+    hipLaunchKernelGGL((wap::norm2_partial<float>), dim3(g), dim3(256), 0, h->stream,
+                       (const float2 *)h->psi, (const float2 *)h->cmp_psi, h->n, h->partial);
+  hipLaunchKernelGGL(wap::sum_partials, dim3(1), dim3(256), 0, h->stream, h->partial, g, h->dsum);
+  WAP_HIP(hipGetLastError());
+  WAP_HIP(hipMemcpyAsync(out, h->dsum, 8, hipMemcpyDeviceToHost, h->stream));
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  return WAP_OK;
+}
+
+int wap_synchronize(wap_handle *h) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  WAP_HIP(hipStreamSynchronize(h->stream));
+  return WAP_OK;
+}
+
+int wap_device_bytes(wap_handle *h, int64_t *out) {
+  if (!h || !out) return fail(WAP_ERR_ARG, "NULL argument");
+  *out = h->bytes;
+  return WAP_OK;
+}
+
+int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
+  if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (tile_y) *tile_y = h->tile_y;
+  if (tile_z) *tile_z = h->tile_z;
+  if (groups) *groups = (int64_t)h->ntx * h->nty * h->ntz;
+  return WAP_OK;
+}
+
+}  // extern "C"

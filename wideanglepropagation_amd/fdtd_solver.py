@@ -1,0 +1,343 @@
+"""Drop-in replacement of the reference's ``wide_angle_propagation/fdtd_solver.py`` (3D path).
+
+Same names, signatures, defaults, return dictionary and error conventions as
+the reference (``fdtd_solver.py:7-29, 274-484``); the stepping runs
+in-process on the GPU through ``libwapfdtd.so`` instead of rewriting and
+recompiling a CUDA program and exchanging ``potential.bin`` /
+``snapshots.bin`` with a child process.
+
+What stays the same
+  * ``calculate_physics_params`` and the module constants.
+  * ``RelativisticSolver3D(source_file=None, binary_file=None)``,
+    ``_update_macros(...) -> bool`` (rewrites ``#define`` lines of
+    ``source_file``), ``compile()``, ``run(...) -> dict | None``,
+    ``_load_snapshots(path) -> dict | None``.
+  * ``run`` returns ``{"snapshots": [(t, phi, psi), ...], "dt", "E_sim",
+    "success": True}`` with float32 phi = Re, psi = Im arrays (nz, ny, nx)
+    and float32 scalars; failures print and return None.
+  * Numbers: by default the constants are quantised exactly as the compiled
+    reference sees them (``reference_quantize=True``), the stepper runs in
+    float32 like the reference, and snapshot times are the frame *start*
+    (``fdtd_3d.cu:389``).
+
+What is new (only through keyword arguments)
+  precision ("f32" | "f32_kahan" | "f64"), reference_quantize (bool),
+  dt, pml_theta, pml_power (the reference never forwards these),
+  equation ("schrodinger" | "klein_gordon"), psi0 / pi0 (initial state),
+  source (bool; the reference always injects), source_profile (ny, nx),
+  exit_plane (k: a GPU-accumulated demodulated average of plane k over the
+  saved frames, returned as "exit_wave"), backend ("hip" | "cli": run the
+  ``fdtd_3d_bin`` command through the file contract), return_timing.
+
+Deviations (documented): AU mode computes P_INF from the wavelength instead
+of leaving a stale value in the source (``fdtd_solver.py:344-349``); the
+package file is never modified (the default ``source_file`` is a private
+copy of the parameter header).
+"""
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+import time
+
+import numpy as np
+
+from .physics import (SOL_LIGHT, MASS_PART, BOHR_RADIUS_ANGSTROM,  # noqa: F401
+                      calculate_physics_params, beam_constants, schrodinger_params,
+                      klein_gordon_params, FDTDParams, DT_DEFAULT)
+
+__all__ = ["SOL_LIGHT", "MASS_PART", "BOHR_RADIUS_ANGSTROM", "calculate_physics_params",
+           "RelativisticSolver3D"]
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+_PARAMS_TEMPLATE = os.path.join(_PKG_DIR, "csrc", "fdtd_3d_params.h")
+
+
+def _replace_define(content, name, val_str):
+    """``fdtd_solver.py:298-304``: replace the value of ``#define NAME ...``."""
+    return re.sub(fr"(#define\s+{name}\s+).*?$", fr"\g<1>{val_str}", content, flags=re.MULTILINE)
+
+
+class RelativisticSolver3D:
+    """3D RK4 FDTD solver (``fdtd_solver.py:274-484``) on MI355X."""
+
+    def __init__(self, source_file=None, binary_file=None):
+        if source_file is None:
+            # private copy: the package's own files are never rewritten
+            self._tmpdir = tempfile.mkdtemp(prefix="wap_fdtd3d_")
+            source_file = os.path.join(self._tmpdir, "fdtd_3d_params.h")
+            shutil.copyfile(_PARAMS_TEMPLATE, source_file)
+        self.source_file = os.path.abspath(source_file)
+        if binary_file is None:
+            self.binary_file = os.path.join(os.path.dirname(self.source_file), "fdtd_3d_bin")
+        else:
+            self.binary_file = os.path.abspath(binary_file)
+        self.working_dir = os.path.dirname(self.source_file) or "."
+        self.last_run = None
+
+    # ------------------------------------------------------------ macros
+    def _update_macros(self, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax,
+                       pml_thick=0.5, pml_sigma=80.0, pml_theta=0.785, pml_power=3.0,
+                       k_source=20, wavelength_angstrom=None, use_angstrom_units=False):
+        """Rewrite the ``#define`` values of ``source_file`` (``fdtd_solver.py:285-374``).
+
+        Returns True if the file changed.  Raises FileNotFoundError when the
+        file does not exist.
+        """
+        if not os.path.exists(self.source_file):
+            raise FileNotFoundError(f"Source file not found: {self.source_file}")
+        with open(self.source_file, "r") as f:
+            content = f.read()
+        new = content
+        if use_angstrom_units and wavelength_angstrom:
+            c = beam_constants(wavelength_angstrom)
+            new = _replace_define(new, "LAP_COEFF", f"{c['A']:.6f}f")
+            new = _replace_define(new, "P_INF", f"{c['P_INF']:.6f}f")
+        else:
+            new = _replace_define(new, "LAP_COEFF", "0.5f")
+        new = _replace_define(new, "NX", str(int(nx)))
+        new = _replace_define(new, "NY", str(int(ny)))
+        new = _replace_define(new, "NZ", str(int(nz)))
+        new = _replace_define(new, "XMIN", f"{xmin:.4f}f")
+        new = _replace_define(new, "XMAX", f"{xmax:.4f}f")
+        new = _replace_define(new, "YMIN", f"{ymin:.4f}f")
+        new = _replace_define(new, "YMAX", f"{ymax:.4f}f")
+        new = _replace_define(new, "ZMIN", f"{zmin:.4f}f")
+        new = _replace_define(new, "ZMAX", f"{zmax:.4f}f")
+        new = _replace_define(new, "PML_THICK", f"{pml_thick:.4f}f")
+        new = _replace_define(new, "PML_SIGMA0", f"{pml_sigma:.4f}f")
+        new = _replace_define(new, "PML_THETA", f"{pml_theta:.4f}f")
+        new = _replace_define(new, "PML_POWER", f"{pml_power:.1f}f")
+        new = _replace_define(new, "K_SOURCE", str(int(k_source)))
+        if new != content:
+            print(f"Configuration changed. Updating {os.path.basename(self.source_file)}...")
+            with open(self.source_file, "w") as f:
+                f.write(new)
+            return True
+        return False
+
+    # ----------------------------------------------------------- compile
+    def compile(self):
+        """Make the HIP solver library available (``fdtd_solver.py:101-126``).
+
+        Builds ``libwapfdtd.so`` with hipcc if it is missing and loads it;
+        on failure prints the compiler output and raises
+        ``RuntimeError("Build failed")`` like the reference.
+        """
+        from .build import LIB_PATH, build
+        try:
+            if not os.path.exists(LIB_PATH):
+                print("Compiling HIP solver...")
+                build()
+            from ._lib import lib
+            lib()
+        except Exception as e:  # noqa: BLE001
+            print("Compilation FAILED:")
+            print(e)
+            raise RuntimeError("Build failed")
+
+    # --------------------------------------------------------------- run
+    def _params(self, shape, extent, wavelength_angstrom, pml_thick, pml_sigma, k_source,
+                use_angstrom_units, kw):
+        nz, ny, nx = shape
+        equation = kw.get("equation", "schrodinger")
+        precision = kw.get("precision", "f32")
+        pml_theta = kw.get("pml_theta", 0.785)
+        pml_power = kw.get("pml_power", 3.0)
+        quant = kw.get("reference_quantize", True)
+        extra = {"precision": precision, "src_enabled": bool(kw.get("source", True))}
+        if kw.get("dt") is not None:
+            extra["dt"] = float(kw["dt"])
+        for key in ("tile_y", "tile_z"):
+            if key in kw:
+                extra[key] = int(kw[key])
+        if equation == "klein_gordon":
+            if not use_angstrom_units:
+                raise ValueError("klein_gordon needs use_angstrom_units=True")
+            return klein_gordon_params(wavelength_angstrom, extent, shape, pml_thick=pml_thick,
+                                       k_source=k_source, pml_power=pml_power,
+                                       kg_gamma_max=kw.get("kg_gamma_max"), **extra)
+        if use_angstrom_units:
+            return schrodinger_params(wavelength_angstrom, extent, shape, pml_thick=pml_thick,
+                                      pml_sigma=pml_sigma, pml_theta=pml_theta,
+                                      pml_power=pml_power, k_source=k_source,
+                                      reference_quantize=quant, **extra)
+        # AU mode: LAP_COEFF 0.5 (fdtd_solver.py:344-346); P_INF from the wavelength
+        _, p_inf, _ = calculate_physics_params(abs(wavelength_angstrom))
+        if quant:
+            a32 = np.float32(0.5)
+            p32 = np.float32(float("%.6f" % p_inf))
+            e_sim = float(np.float32(np.float32(a32 * p32) * p32))
+            k_beam = float(np.float32(np.sqrt(np.float32(np.float32(e_sim) / a32))))
+            dt = float(np.float32(extra.pop("dt", DT_DEFAULT)))
+            return FDTDParams(nx=nx, ny=ny, nz=nz,
+                              extent=tuple(float(np.float32(float("%.4f" % v))) for v in extent),
+                              lap_coeff=0.5, e_sim=e_sim, k_beam=k_beam, dt=dt, time_mode="f32",
+                              pml_thick=float(np.float32(float("%.4f" % pml_thick))),
+                              pml_sigma0=float(np.float32(float("%.4f" % pml_sigma))),
+                              pml_theta=float(np.float32(float("%.4f" % pml_theta))),
+                              pml_power=float(np.float32(float("%.1f" % pml_power))),
+                              src_k=int(k_source), **extra)
+        return FDTDParams(nx=nx, ny=ny, nz=nz, extent=tuple(extent), lap_coeff=0.5,
+                          e_sim=0.5 * p_inf * p_inf, k_beam=p_inf, pml_thick=pml_thick,
+                          pml_sigma0=pml_sigma, pml_theta=pml_theta, pml_power=pml_power,
+                          src_k=int(k_source), **extra)
+
+    def run(self, V_map, wavelength_angstrom, extent,
+            n_frames=600, steps_per_frame=100, snapshot_count=None,
+            pml_thick=0.5, pml_sigma=80.0, k_source=20,
+            use_angstrom_units=True, **kwargs):
+        """Run the 3D FDTD simulation (``fdtd_solver.py:376-454``).
+
+        V_map: (NZ, NY, NX) potential energy, eV (Angstrom mode) or Hartree.
+        extent: (xmin, xmax, ymin, ymax, zmin, zmax), Angstrom or Bohr.
+        """
+        nz, ny, nx = V_map.shape
+        xmin, xmax, ymin, ymax, zmin, zmax = extent
+        if snapshot_count is None:
+            snapshot_count = n_frames
+        if "abs_width" in kwargs:
+            pml_thick = kwargs["abs_width"]
+        self._update_macros(nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax,
+                            pml_thick=pml_thick, pml_sigma=pml_sigma,
+                            pml_theta=kwargs.get("pml_theta", 0.785),
+                            pml_power=kwargs.get("pml_power", 3.0), k_source=k_source,
+                            wavelength_angstrom=wavelength_angstrom,
+                            use_angstrom_units=use_angstrom_units)
+        if kwargs.get("backend", "hip") == "cli":
+            return self._run_cli(V_map, wavelength_angstrom, n_frames, steps_per_frame,
+                                 snapshot_count)
+        try:
+            self.compile()
+            params = self._params(V_map.shape, extent, wavelength_angstrom, pml_thick,
+                                  pml_sigma, k_source, use_angstrom_units, kwargs)
+            return self._run_inprocess(params, V_map, n_frames, steps_per_frame,
+                                       snapshot_count, kwargs)
+        except RuntimeError as e:
+            print("Simulation Execution Failed:")
+            print(e)
+            return None
+
+    def _run_inprocess(self, params, V_map, n_frames, steps_per_frame, snapshot_count, kw):
+        import torch
+        from .engine import FDTD3D
+
+        V32 = np.ascontiguousarray(np.asarray(V_map).astype(np.float32))
+        sim = FDTD3D(params, V=V32)
+        try:
+            if kw.get("psi0") is not None:
+                sim.set_state(kw["psi0"], kw.get("pi0"))
+            if kw.get("source_profile") is not None:
+                sim.set_source_profile(kw["source_profile"])
+            M = int(snapshot_count)
+            save_start = max(0, n_frames - M)
+            saved = 0
+            snaps = []
+            exit_k = kw.get("exit_plane")
+            acc = None
+            if exit_k is not None:
+                acc = torch.zeros((params.ny, params.nx), dtype=torch.complex128, device="cuda")
+            quant = params.time_mode == "f32"
+            dt32 = np.float32(params.dt)
+            e_sim32 = np.float32(params.e_sim)
+            t_run = 0.0
+            for frame in range(n_frames):
+                torch.cuda.synchronize()
+                t_a = time.perf_counter()
+                sim.step(steps_per_frame)
+                sim.synchronize()
+                t_run += time.perf_counter() - t_a
+                if frame >= save_start and saved < M:
+                    if quant:
+                        t_snap = np.float32(np.float32(frame * steps_per_frame) * dt32)
+                    else:
+                        t_snap = np.float32((frame + 1) * steps_per_frame * params.dt)
+                    if acc is not None:
+                        sim.accumulate_plane(exit_k, float(e_sim32) * float(t_snap), acc)
+                    if not kw.get("no_snapshots", False):
+                        st = sim.get_state(np.complex64)
+                        if isinstance(st, tuple):
+                            st = st[0]
+                        phi = np.ascontiguousarray(st.real)
+                        psi = np.ascontiguousarray(st.imag)
+                        phi.setflags(write=False)
+                        psi.setflags(write=False)
+                        snaps.append((t_snap, phi, psi))
+                    saved += 1
+            out = {"snapshots": snaps, "dt": dt32, "E_sim": e_sim32, "success": True}
+            vox_steps = params.voxels * n_frames * steps_per_frame
+            out["elapsed_s"] = t_run
+            out["gvoxel_steps_per_s"] = vox_steps / t_run / 1e9 if t_run > 0 else float("nan")
+            if acc is not None:
+                out["exit_wave"] = (acc / max(1, saved)).cpu().numpy()
+            if kw.get("return_state", False):
+                out["state"] = sim.get_state(np.complex128)
+            print(f"Loaded {len(snaps)} snapshots.")
+            self.last_run = out
+            return out
+        finally:
+            sim.close()
+
+    # ------------------------------------------------------------- files
+    def _write_potential(self, V_map):
+        """``potential.bin``: int32 [nx, ny, nz] + float32 V (``fdtd_solver.py:416-419``)."""
+        nz, ny, nx = V_map.shape
+        pot_file = os.path.join(self.working_dir, "potential.bin")
+        with open(pot_file, "wb") as f:
+            f.write(np.array([nx, ny, nz], dtype=np.int32).tobytes())
+            f.write(np.asarray(V_map).astype(np.float32).tobytes())
+        return pot_file
+
+    def _run_cli(self, V_map, wavelength_angstrom, n_frames, steps_per_frame, snapshot_count):
+        """File-contract path (``fdtd_solver.py:405-454``): potential.bin -> fdtd_3d_bin ->
+        snapshots.bin, run in ``working_dir``."""
+        from .cli import ensure_cli_binary
+        source_mtime = os.path.getmtime(self.source_file)
+        binary_mtime = os.path.getmtime(self.binary_file) if os.path.exists(self.binary_file) else 0
+        if not os.path.exists(self.binary_file) or source_mtime > binary_mtime:
+            self.compile()
+            ensure_cli_binary(self.binary_file)
+        self._write_potential(V_map)
+        cmd = [self.binary_file, str(n_frames), str(steps_per_frame), str(snapshot_count),
+               str(-abs(wavelength_angstrom)), self.source_file]
+        if os.path.exists(self.binary_file):
+            os.chmod(self.binary_file, 0o755)
+        print(f"Running 3D: {' '.join([os.path.basename(c) for c in cmd])}")
+        try:
+            subprocess.run(cmd, check=True, capture_output=True, text=True, cwd=self.working_dir)
+        except subprocess.CalledProcessError as e:
+            print("Simulation Execution Failed:")
+            print(e.stdout)
+            print(e.stderr)
+            return None
+        return self._load_snapshots(os.path.join(self.working_dir, "snapshots.bin"))
+
+    def _load_snapshots(self, filepath):
+        """Parse ``snapshots.bin`` (``fdtd_solver.py:456-484``).
+
+        Header int32 nx, ny, nz, M, float32 dt, E_sim; then M records of float32 t,
+        phi[nz,ny,nx], psi[nz,ny,nx].  Returns None if the file is missing; a
+        truncated file raises ValueError.
+        """
+        if not os.path.exists(filepath):
+            print(f"Error: Output file {filepath} not found.")
+            return None
+        snapshots = []
+        with open(filepath, "rb") as f:
+            head = f.read(24)
+            if len(head) != 24:
+                raise ValueError(f"{filepath}: truncated header")
+            nx, ny, nz, M = np.frombuffer(head[:16], dtype=np.int32)
+            dt, e_sim = np.frombuffer(head[16:], dtype=np.float32)
+            vol = int(nx) * int(ny) * int(nz)
+            for _ in range(int(M)):
+                rec = f.read(4 + 8 * vol)
+                if len(rec) != 4 + 8 * vol:
+                    raise ValueError(f"{filepath}: truncated snapshot record")
+                t = np.frombuffer(rec[:4], dtype=np.float32)[0]
+                phi = np.frombuffer(rec[4:4 + 4 * vol], dtype=np.float32).reshape((nz, ny, nx))
+                psi = np.frombuffer(rec[4 + 4 * vol:], dtype=np.float32).reshape((nz, ny, nx))
+                snapshots.append((t, phi, psi))
+        print(f"Loaded {len(snapshots)} snapshots.")
+        return {"snapshots": snapshots, "dt": dt, "E_sim": e_sim, "success": True}
