@@ -22,15 +22,26 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic HBM bytes per voxel for each pass of a step (4-pass fused schedule)
+# algorithmic HBM bytes per voxel for each pass of a step
 PASS_BYTES = {
-    ("klein_gordon", "f32"): (52, 84, 84, 52),
-    ("klein_gordon", "f32_kahan"): (52, 84, 84, 84),
-    ("klein_gordon", "f64"): (104, 168, 168, 104),
-    ("schrodinger", "f32"): (28, 44, 44, 28),
-    ("schrodinger", "f32_kahan"): (28, 44, 44, 52),
-    ("schrodinger", "f64"): (56, 88, 88, 56),
+    # one streaming pass per RK4 stage
+    ("stages", "klein_gordon", "f32"): (52, 84, 84, 52),
+    ("stages", "klein_gordon", "f32_kahan"): (52, 84, 84, 84),
+    ("stages", "klein_gordon", "f64"): (104, 168, 168, 104),
+    ("stages", "schrodinger", "f32"): (28, 44, 44, 28),
+    ("stages", "schrodinger", "f32_kahan"): (28, 44, 44, 52),
+    ("stages", "schrodinger", "f64"): (56, 88, 88, 56),
+    # Klein-Gordon RK4 step in two stencil passes
+    ("kg_2pass", "klein_gordon", "f32"): (60, 68),
+    ("kg_2pass", "klein_gordon", "f32_kahan"): (60, 108),
+    ("kg_2pass", "klein_gordon", "f64"): (120, 136),
 }
+
+
+def schedule_of(p):
+    if p.equation == "klein_gordon" and p.schedule != "stages":
+        return "kg_2pass"
+    return "stages"
 
 
 def parse():
@@ -45,7 +56,26 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--schedule", default="auto", choices=["auto", "stages", "kg_2pass"])
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(gib, reps=3):
+    """A plain copy kernel of known size: reads and writes gib GiB per launch."""
+    import torch
+    n = int(gib * 2 ** 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "copy_GBs": 2 * n * 4 / el / 1e9}), flush=True)
 
 
 def cpu_baseline(eq, shape, nsteps=1):
@@ -84,6 +114,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
               "sharding is not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     from wideanglepropagation_amd.engine import FDTD3D
     from wideanglepropagation_amd.workloads import kg_slab, centred_extent
@@ -103,7 +136,7 @@ def main():
         from wideanglepropagation_amd.workloads import atom_potential, random_slab
         pos, Zs, scr = random_slab(400, ((ext[0], ext[1]), (ext[2], ext[3])), (-10, 10), 0)
         V = atom_potential(shape, ext, pos, Zs, scr, 0.05, device="cuda")
-    p = p.with_(tile_y=args.tile_y, tile_z=args.tile_z)
+    p = p.with_(tile_y=args.tile_y, tile_z=args.tile_z, schedule=args.schedule)
     V = V.to(torch.float64 if args.precision == "f64" else torch.float32).contiguous()
     sim = FDTD3D(p, V=V)
     del V
@@ -124,9 +157,10 @@ def main():
     # per-pass HIP-event timing on the solver's stream
     pass_ms, tot_ms = sim.step_profiled(args.profile_steps)
     n_prof = max(1, args.profile_steps)
-    avg = [m / n_prof for m in pass_ms]
-    pb = PASS_BYTES[(p.equation, p.precision)]
-    dom = int(np.argmax([b / 1.0 for b in pb]))  # heaviest pass (P2)
+    sched = schedule_of(p)
+    pb = PASS_BYTES[(sched, p.equation, p.precision)]
+    avg = [m / n_prof for m in pass_ms][:len(pb)]
+    dom = int(np.argmax(avg))  # longest pass
     dom_bytes = pb[dom] * vox
     achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
     step_bytes = sum(pb) * vox
@@ -155,11 +189,12 @@ def main():
                                f"{'complex128' if p.precision == 'f64' else 'complex64'} fields "
                                f"({p.precision}), 400-atom slab, PML 16 planes, 200 keV",
                    "grid": [nz, ny, nx], "precision": p.precision,
-                   "equation": p.equation, "tile_y": ty, "tile_z": tz},
+                   "equation": p.equation, "schedule": sched, "tile_y": ty, "tile_z": tz},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": f"rk4_pass stage {dom + 1}",
+                     "kernel": (f"kg2_pass pass {dom + 1}" if sched == "kg_2pass"
+                                else f"rk4_pass stage {dom + 1}"),
                      "bytes_per_launch": dom_bytes,
                      "mean_launch_ms": round(avg[dom], 5)},
         "cpu_baseline": cpu,

@@ -69,30 +69,33 @@ def _run_both(p, V, st0, nsteps, profile=None):
     return got, want
 
 
-@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
-def test_f64_matches_oracle(eq, rng):
-    print(f"case f64 {eq} 16x16x24", flush=True)
-    p = _params((24, 16, 16), eq)
+@pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
+                                       ("klein_gordon", "stages")])
+def test_f64_matches_oracle(eq, sched, rng):
+    print(f"case f64 {eq} {sched} 16x16x24", flush=True)
+    p = _params((24, 16, 16), eq, schedule=sched)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 20)
     assert rel_l2(got, want) < 1e-12
 
 
+@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
 @pytest.mark.parametrize("shape,ty,tz", [((9, 5, 70), 1, 1), ((9, 5, 70), 2, 3),
                                           ((11, 13, 130), 8, 100), ((7, 3, 2), 4, 0)])
-def test_f64_ragged_shapes_and_tiles(shape, ty, tz, rng):
-    print(f"case ragged {shape} ty={ty} tz={tz}", flush=True)
-    p = _params(shape, tile_y=ty, tile_z=tz, pml_thick=0.1, k_source=2)
+def test_f64_ragged_shapes_and_tiles(shape, ty, tz, eq, rng):
+    print(f"case ragged {eq} {shape} ty={ty} tz={tz}", flush=True)
+    p = _params(shape, eq, tile_y=ty, tile_z=tz, pml_thick=0.1, k_source=2)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 6)
     assert rel_l2(got, want) < 1e-12
 
 
-@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+@pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
+                                       ("klein_gordon", "stages")])
 @pytest.mark.parametrize("prec", ["f32", "f32_kahan"])
-def test_f32_matches_oracle(eq, prec, rng):
-    print(f"case {prec} {eq}", flush=True)
-    p = _params((24, 16, 16), eq, precision=prec)
+def test_f32_matches_oracle(eq, sched, prec, rng):
+    print(f"case {prec} {eq} {sched}", flush=True)
+    p = _params((24, 16, 16), eq, precision=prec, schedule=sched)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 20)
     assert rel_l2(got, want) < 2e-6
@@ -158,5 +161,7 @@ def test_errors_are_reported_not_fatal():
         FDTD3D(p.with_(nz=2))
     with pytest.raises(WapError, match="source plane"):
         FDTD3D(p.with_(src_k=99))
+    with pytest.raises(WapError, match="2-pass schedule"):
+        FDTD3D(p.with_(schedule="kg_2pass"))
     with pytest.raises(ValueError):
         FDTD3D(p, V=np.zeros(7))

@@ -16,6 +16,7 @@ WAP_OK = 0
 EQ = {"schrodinger": 0, "klein_gordon": 1}
 PREC = {"f64": 0, "f32": 1, "f32_kahan": 2}
 TIME = {"exact": 0, "f32": 1}
+SCHED = {"auto": 0, "stages": 1, "kg_2pass": 2}
 DT_F32, DT_F64, DT_C64, DT_C128 = 0, 1, 2, 3
 
 _c_int = ctypes.c_int
@@ -31,7 +32,7 @@ class WapDesc(ctypes.Structure):
         ("nx", _c_i32), ("ny", _c_i32), ("nz", _c_i32),
         ("equation", _c_i32), ("precision", _c_i32), ("time_mode", _c_i32),
         ("src_enabled", _c_i32), ("src_k", _c_i32), ("src_band", _c_i32),
-        ("tile_y", _c_i32), ("tile_z", _c_i32), ("reserved0", _c_i32),
+        ("tile_y", _c_i32), ("tile_z", _c_i32), ("schedule", _c_i32),
         ("xmin", _c_dbl), ("xmax", _c_dbl), ("ymin", _c_dbl), ("ymax", _c_dbl),
         ("zmin", _c_dbl), ("zmax", _c_dbl),
         ("dt", _c_dbl), ("lap_coeff", _c_dbl), ("e_sim", _c_dbl), ("k_beam", _c_dbl),
@@ -108,6 +109,7 @@ def make_desc(p):
     d.src_band = int(p.src_band)
     d.tile_y = int(p.tile_y)
     d.tile_z = int(p.tile_z)
+    d.schedule = SCHED[p.schedule]
     (d.xmin, d.xmax, d.ymin, d.ymax, d.zmin, d.zmax) = [float(v) for v in p.extent]
     d.dt = float(p.dt)
     d.lap_coeff = float(p.lap_coeff)

@@ -294,9 +294,89 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
   return WAP_OK;
 }
 
+bool kg_two_pass(const wap_handle *h) {
+  return is_kg(h) && h->d.schedule != WAP_SCHED_STAGES;
+}
+
+template <typename R>
+wap::KG2Args<R> kg2_args(const wap_handle *h) {
+  using C = wap::cplx<R>;
+  wap::KG2Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.st_psi = (const C *)h->psi;
+  a.st_pi = (const C *)h->pi;
+  a.y3 = (C *)h->ya_psi;
+  a.y4 = (C *)h->yb_psi;
+  a.z3 = (C *)h->ya_pi;
+  a.pp = (C *)h->acc_psi;
+  a.qq = (C *)h->acc_pi;
+  a.out_psi = (C *)h->psi;
+  a.out_pi = (C *)h->pi;
+  a.cmp_psi = (C *)h->cmp_psi;
+  a.cmp_pi = (C *)h->cmp_pi;
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.gam = (const R *)h->gam;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx;
+  a.ny = h->d.ny;
+  a.nz = h->d.nz;
+  a.tile_y = h->tile_y;
+  a.tile_z = h->tile_z;
+  a.ntx = h->ntx;
+  a.nty = h->nty;
+  a.ntz = h->ntz;
+  a.hx = (R)h->hx;
+  a.hy = (R)h->hy;
+  a.m2 = (R)h->m2;
+  a.h = (R)h->d.dt;
+  return a;
+}
+
+template <typename R, bool K>
+int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                  double *pass_ms) {
+  const dim3 block(wap::kWaveX, h->tile_y);
+  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
+  wap::KG2Args<R> a = kg2_args<R>(h);
+  wap::PassArgs<R> src{};
+  for (int64_t n = 0; n < nsteps; ++n) {
+    fill_source<R>(h, step_time(h, step0, n, t0), src);
+    a.src_k0 = src.src_k0;
+    a.src_n = src.src_n;
+    for (int b = 0; b < wap::kMaxBand; ++b) {
+      a.src_keep[b] = src.src_keep[b];
+      a.src_coef[b] = src.src_coef[b];
+      a.src_coef_pi[b] = src.src_coef_pi[b];
+    }
+    if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
+    hipLaunchKernelGGL((wap::kg2_pass<R, 1, K>), grid, block, 0, h->stream, a);
+    if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
+    hipLaunchKernelGGL((wap::kg2_pass<R, 2, K>), grid, block, 0, h->stream, a);
+    if (ev) {
+      WAP_HIP(hipEventRecord(ev[2], h->stream));
+      WAP_HIP(hipEventSynchronize(ev[2]));
+      for (int s = 0; s < 2; ++s) {
+        float ms = 0.f;
+        WAP_HIP(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+        pass_ms[s] += ms;
+      }
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
+  if (kg_two_pass(h)) {
+    switch (h->d.precision) {
+      case WAP_PREC_F64: return run_steps_kg2<double, false>(h, nsteps, step0, t0, ev, pass_ms);
+      case WAP_PREC_F32: return run_steps_kg2<float, false>(h, nsteps, step0, t0, ev, pass_ms);
+      default: return run_steps_kg2<float, true>(h, nsteps, step0, t0, ev, pass_ms);
+    }
+  }
   switch (h->d.precision) {
     case WAP_PREC_F64:
       return kg ? run_steps<double, 1, false>(h, nsteps, step0, t0, ev, pass_ms)
@@ -356,6 +436,10 @@ int validate(const wap_desc *d) {
   if (d->tile_y < 0 || d->tile_y > 8 || (d->tile_y & (d->tile_y - 1)))
     return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8");
   if (d->tile_z < 0) return fail(WAP_ERR_ARG, "tile_z must be >= 0");
+  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_KG_2PASS)
+    return fail(WAP_ERR_ARG, "unknown schedule");
+  if (d->schedule == WAP_SCHED_KG_2PASS && d->equation != WAP_EQ_KLEIN_GORDON)
+    return fail(WAP_ERR_ARG, "the 2-pass schedule is for the Klein-Gordon equation");
   return WAP_OK;
 }
 
@@ -385,8 +469,11 @@ int wap_create(const wap_desc *desc, wap_handle **out) {
   void **fields[] = {&h->psi, &h->acc_psi, &h->ya_psi, &h->yb_psi};
   void **fields_pi[] = {&h->pi, &h->acc_pi, &h->ya_pi, &h->yb_pi};
   for (int s = 0; s < 4 && !rc; ++s) rc = dev_alloc(h, fields[s], fb);
-  if (kg)
-    for (int s = 0; s < 4 && !rc; ++s) rc = dev_alloc(h, fields_pi[s], fb);
+  if (kg) {
+    // the 2-pass KG schedule uses 7 complex fields (state + Y3 Y4 Z3 P Q), not 8
+    const int npi = kg_two_pass(h) ? 3 : 4;
+    for (int s = 0; s < npi && !rc; ++s) rc = dev_alloc(h, fields_pi[s], fb);
+  }
   if (!rc && is_kahan(h)) {
     rc = dev_alloc(h, &h->cmp_psi, fb);
     if (!rc && kg) rc = dev_alloc(h, &h->cmp_pi, fb);

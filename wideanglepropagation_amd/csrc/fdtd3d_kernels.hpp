@@ -270,6 +270,252 @@ __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
   }
 }
 
+// ------------------------------------------------- Klein-Gordon in 2 passes
+//
+// With y = (psi, pi), k = M y, M = [[0, I], [A, D]], A = c^2 Lap - (m^2 - V^2),
+// D = -2iV - 2 gamma, the RK4 step P4(dt M) contains A at most squared
+// (d psi/dt = pi is pointwise), so one classical RK4 step needs only two
+// stencil sweeps.  With b = (s, q) the injected state and h = dt:
+//
+//  pass 1 (stencils of s and q^, q^ = q with the frozen planes set to 0):
+//    f  = A s + D q                       (k1 pi)
+//    u  = q + h/2 f                       (y2 pi = k2 psi)
+//    g  = A s + h/2 A q^ + D u            (k2 pi)
+//    Y3 = s + h/2 u   Z3 = q + h/2 g      (y3)
+//    Y4 = s + h Z3                        (y4 psi)
+//    P  = s + h/6 (q + 2u + 2 Z3)         Q = q + h/6 (f + 2g)
+//  pass 2 (stencils of Y3, Y4):
+//    k3 = A Y3 + D Z3 ;  w = q + h k3 ;  k4 = A Y4 + D w
+//    psi' = P + h/6 w ;  pi' = Q + h/6 (2 k3 + k4)
+//
+// On the frozen planes every increment is 0 (Y3 = Y4 = P = s, Z3 = Q = q).
+// HBM bytes per voxel (complex64, float32 V): pass 1 reads s, q, V (20) and
+// writes Y3, Y4, Z3, P, Q (40); pass 2 reads Y3, Y4, Z3, P, Q, q, V (52) and
+// writes psi', pi' (16): 128 B per step against 272 B for 4 stage passes.
+// F32_KAHAN stores P, Q without s, q and adds them in pass 2 with a
+// compensated update (pass 2 also reads s and R/W the compensation): 168 B.
+template <typename R>
+struct KG2Args {
+  using C = cplx<R>;
+  const C *st_psi;     // state psi_n (pass 1 stencil; Kahan pass 2 base)
+  const C *st_pi;      // state pi_n  (pass 1 stencil; pass 2 base)
+  C *y3;               // Y3  (pass 1 out, pass 2 stencil)
+  C *y4;               // Y4
+  C *z3;               // Z3
+  C *pp;               // P
+  C *qq;               // Q
+  C *out_psi;          // new state (pass 2)
+  C *out_pi;
+  C *cmp_psi;          // Kahan compensation (pass 2)
+  C *cmp_pi;
+  const R *V;
+  const C *zc;
+  const R *gam;
+  const C *prof;
+  int nx, ny, nz;
+  int tile_y, tile_z;
+  int ntx, nty, ntz;
+  R hx, hy, m2, h;
+  int src_k0, src_n;
+  R src_keep[kMaxBand];
+  C src_coef[kMaxBand];
+  C src_coef_pi[kMaxBand];
+};
+
+template <typename R>
+__device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, int k, int ij, R &keep, cplx<R> &inc,
+                                        cplx<R> &inc_pi) {
+  const int b = k - p.src_k0;
+  if (b < 0 || b >= p.src_n) return false;
+  cplx<R> c = mkc<R>(R(0), R(0)), cq = c;
+  keep = R(1);
+#pragma unroll
+  for (int t = 0; t < kMaxBand; ++t) {
+    if (t == b) {
+      keep = p.src_keep[t];
+      c = p.src_coef[t];
+      cq = p.src_coef_pi[t];
+    }
+  }
+  if (p.prof) {
+    const cplx<R> pr = p.prof[ij];
+    c = cmul<R>(c, pr);
+    cq = cmul<R>(cq, pr);
+  }
+  inc = c;
+  inc_pi = cq;
+  return true;
+}
+
+// injected state (s, q) at voxel (k, ij)
+template <typename R>
+__device__ __forceinline__ void kg2_state(const KG2Args<R> &p, size_t idx, int k, int ij, bool want_s,
+                                          cplx<R> &s, cplx<R> &q) {
+  R keep;
+  cplx<R> inc, incq;
+  const bool band = kg2_src(p, k, ij, keep, inc, incq);
+  if (want_s) {
+    s = p.st_psi[idx];
+    if (band) s = caxpy<R>(cscale<R>(s, keep), R(1), inc);
+  }
+  q = p.st_pi[idx];
+  if (band) q = caxpy<R>(cscale<R>(q, keep), R(1), incq);
+}
+
+// A y without the diagonal term: weighted 7-point Laplacian (c^2 in the weights)
+template <typename R>
+__device__ __forceinline__ cplx<R> kg2_lap(const KG2Args<R> &p, int k, cplx<R> c, cplx<R> xp,
+                                           cplx<R> xm, cplx<R> yp, cplx<R> ym, cplx<R> zp,
+                                           cplx<R> zm) {
+  const cplx<R> wzp = p.zc[2 * k];
+  const cplx<R> wzm = p.zc[2 * k + 1];
+  const cplx<R> lx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
+  const cplx<R> ly = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
+  const cplx<R> lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
+  return cadd<R>(cadd<R>(cscale<R>(lx, p.hx), cscale<R>(ly, p.hy)), lz);
+}
+
+// D z = (-2 i V - 2 g) z
+template <typename R>
+__device__ __forceinline__ cplx<R> kg2_damp(R v, R g, cplx<R> z) {
+  return mkc<R>(-R(2) * g * z.x + R(2) * v * z.y, -R(2) * v * z.x - R(2) * g * z.y);
+}
+
+template <typename R, int PASS, bool KAHAN>
+__global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
+  using C = cplx<R>;
+  const int ntiles = p.ntx * p.nty * p.ntz;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int ty = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int i = tx * kWaveX + threadIdx.x;
+  const int j = ty * p.tile_y + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;  // no workgroup barriers in this kernel
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(p.nz, k0 + p.tile_z);
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const size_t plane = (size_t)nx * ny;
+  const int ij = j * nx + i;
+  const int ip = (i + 1 == nx) ? 0 : i + 1;
+  const int im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1;
+  const int jm = (j == 0) ? ny - 1 : j - 1;
+  const int o_xp = j * nx + ip, o_xm = j * nx + im;
+  const int o_yp = jp * nx + i, o_ym = jm * nx + i;
+  const C zero = mkc<R>(R(0), R(0));
+
+  // stencil field values at (k, off): pass 1 -> (s, q^), pass 2 -> (Y3, Y4)
+  auto fields = [&](int k, int off, C &a, C &b) {
+    const size_t id = (size_t)k * plane + off;
+    if (PASS == 1) {
+      kg2_state<R>(p, id, k, off, true, a, b);
+      if (k == 0 || k == nz - 1) b = zero;
+    } else {
+      a = p.y3[id];
+      b = p.y4[id];
+    }
+  };
+
+  C am = zero, bm = zero, ac, bc, ap, bp;
+  if (k0 > 0) fields(k0 - 1, ij, am, bm);
+  fields(k0, ij, ac, bc);
+  for (int k = k0; k < k1; ++k) {
+    const size_t idx = (size_t)k * plane + ij;
+    if (k + 1 < nz) fields(k + 1, ij, ap, bp);
+    else ap = bp = zero;
+    const bool interior = (k > 0) && (k < nz - 1);
+    const R v = p.V[idx];
+    const R g = p.gam[k];
+    const R w = p.m2 - v * v;
+    if (PASS == 1) {
+      const C s = ac;
+      C q = bc;  // q^ == q on interior planes
+      if (!interior) {
+        C dummy;
+        kg2_state<R>(p, idx, k, ij, false, dummy, q);
+      }
+      if (interior) {
+        C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
+        fields(k, o_xp, ax0, bx0);
+        fields(k, o_xm, ax1, bx1);
+        fields(k, o_yp, ay0, by0);
+        fields(k, o_ym, ay1, by1);
+        const C As = csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am), cscale<R>(ac, w));
+        const C Aq = csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm), cscale<R>(bc, w));
+        const C f = cadd<R>(As, kg2_damp<R>(v, g, q));
+        const C u = caxpy<R>(q, p.h * R(0.5), f);
+        const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
+        const C y3 = caxpy<R>(s, p.h * R(0.5), u);
+        const C z3 = caxpy<R>(q, p.h * R(0.5), gg);
+        const C y4 = caxpy<R>(s, p.h, z3);
+        const C pinc = cscale<R>(cadd<R>(q, cscale<R>(cadd<R>(u, z3), R(2))), p.h / R(6));
+        const C qinc = cscale<R>(caxpy<R>(f, R(2), gg), p.h / R(6));
+        p.y3[idx] = y3;
+        p.y4[idx] = y4;
+        p.z3[idx] = z3;
+        p.pp[idx] = KAHAN ? pinc : cadd<R>(s, pinc);
+        p.qq[idx] = KAHAN ? qinc : cadd<R>(q, qinc);
+      } else {
+        p.y3[idx] = s;
+        p.y4[idx] = s;
+        p.z3[idx] = q;
+        p.pp[idx] = KAHAN ? zero : s;
+        p.qq[idx] = KAHAN ? zero : q;
+      }
+    } else {
+      C s = zero, q;
+      kg2_state<R>(p, idx, k, ij, KAHAN, s, q);
+      C dpsi = zero, dpi = zero;  // increments beyond P, Q
+      if (interior) {
+        C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
+        fields(k, o_xp, ax0, bx0);
+        fields(k, o_xm, ax1, bx1);
+        fields(k, o_yp, ay0, by0);
+        fields(k, o_ym, ay1, by1);
+        const C z3 = p.z3[idx];
+        const C k3 = cadd<R>(csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am),
+                                     cscale<R>(ac, w)),
+                             kg2_damp<R>(v, g, z3));
+        const C wv = caxpy<R>(q, p.h, k3);
+        const C k4 = cadd<R>(csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm),
+                                     cscale<R>(bc, w)),
+                             kg2_damp<R>(v, g, wv));
+        dpsi = cscale<R>(wv, p.h / R(6));
+        dpi = cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6));
+      }
+      const C P = p.pp[idx];
+      const C Q = p.qq[idx];
+      if (!KAHAN) {
+        p.out_psi[idx] = cadd<R>(P, dpsi);
+        p.out_pi[idx] = cadd<R>(Q, dpi);
+      } else {
+        R keep;
+        C inc, incq;
+        const bool band = kg2_src(p, k, ij, keep, inc, incq);
+        C cs = p.cmp_psi[idx], cq = p.cmp_pi[idx];
+        if (band) {
+          cs = cscale<R>(cs, keep);
+          cq = cscale<R>(cq, keep);
+        }
+        const C ys = csub<R>(cadd<R>(P, dpsi), cs);
+        const C ts = cadd<R>(s, ys);
+        p.cmp_psi[idx] = csub<R>(csub<R>(ts, s), ys);
+        p.out_psi[idx] = ts;
+        const C yq = csub<R>(cadd<R>(Q, dpi), cq);
+        const C tq = cadd<R>(q, yq);
+        p.cmp_pi[idx] = csub<R>(csub<R>(tq, q), yq);
+        p.out_pi[idx] = tq;
+      }
+    }
+    am = ac;
+    bm = bc;
+    ac = ap;
+    bc = bp;
+  }
+}
+
 // ---------------------------------------------------------------- utilities
 
 // out = (complex TO) (hi - cmp)   (cmp may be null); n elements

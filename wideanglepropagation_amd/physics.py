@@ -98,12 +98,15 @@ class FDTDParams:
     src_alpha: float = SRC_ALPHA
     tile_y: int = 0
     tile_z: int = 0
+    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass"
 
     def __post_init__(self):
         if self.equation not in EQUATIONS:
             raise ValueError(f"equation must be one of {EQUATIONS}")
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
+        if self.schedule not in ("auto", "stages", "kg_2pass"):
+            raise ValueError("schedule must be 'auto', 'stages' or 'kg_2pass'")
         if self.time_mode not in ("exact", "f32"):
             raise ValueError("time_mode must be 'exact' or 'f32'")
         self.extent = tuple(float(v) for v in self.extent)
