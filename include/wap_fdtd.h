@@ -158,6 +158,32 @@ int wap_device_bytes(wap_handle *h, int64_t *out);
 /* Effective launch shape chosen for the handle (tile_y, tile_z, workgroups per pass). */
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups);
 
+/*
+ * Batches: many independent grids (the reference runs one grid per process)
+ * stepped by one launch per pass.  All handles must share equation, precision
+ * and schedule and have V set; sizes, energies, dt and sources may differ.
+ * The batch keeps pointers to the handles' buffers: destroy it before them.
+ */
+typedef struct wap_batch wap_batch;
+
+/* Group n handles; tile_y = workgroup rows for all (0 = the first handle's). */
+int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **out);
+int wap_batch_destroy(wap_batch *b);
+/* Stream of the batch's launches (default: the first handle's stream). */
+int wap_batch_set_stream(wap_batch *b, void *stream);
+/* Total workgroups per pass and the shared tile_y. */
+int wap_batch_tiles(wap_batch *b, int64_t *tiles, int32_t *tile_y);
+/* Advance every grid nsteps steps (step n of grid i injects at its own t_n). */
+int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0);
+/* Record nsteps steps of every grid into one hipGraph (on a non-null stream;
+ * one is created if the batch has none). */
+int wap_batch_capture(wap_batch *b, int64_t nsteps);
+/* Run the captured graph for steps step0 .. step0+nsteps-1 (the per-step source
+ * phasors are rewritten before the launch); ms != NULL returns its device time
+ * (HIP events on the batch stream) and synchronises. */
+int wap_batch_replay(wap_batch *b, int64_t step0, double *ms);
+int wap_batch_synchronize(wap_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,14 @@ SIGNATURES = {
     "wap_device_bytes": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
     "wap_launch_shape": (_c_int, [_vp, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32),
                                   ctypes.POINTER(_c_i64)]),
+    "wap_batch_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "wap_batch_destroy": (_c_int, [_vp]),
+    "wap_batch_set_stream": (_c_int, [_vp, _vp]),
+    "wap_batch_tiles": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i32)]),
+    "wap_batch_step": (_c_int, [_vp, _c_i64, _c_i64]),
+    "wap_batch_capture": (_c_int, [_vp, _c_i64]),
+    "wap_batch_replay": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_dbl)]),
+    "wap_batch_synchronize": (_c_int, [_vp]),
 }
 
 _LIB = None

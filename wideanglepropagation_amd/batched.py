@@ -1,0 +1,98 @@
+"""Many independent grids stepped by one launch per pass, optionally as one hipGraph.
+
+The reference runs one grid per process (``fdtd_3d.cu:275-403``); a set of
+probe positions, beam energies or wave packets is a set of independent runs.
+``BatchedFDTD`` groups ``FDTD3D`` handles by (equation, precision, schedule)
+and, per group, issues one kernel launch per pass whose workgroups are the
+union of every grid's column tiles (equal-sized work units, so uneven grids
+balance by construction).  ``capture(n)`` records n steps of every group into
+a single hipGraph per group; ``replay(step0)`` rewrites the per-step source
+phasors and launches it.
+"""
+import ctypes
+
+import torch
+
+from ._lib import check, lib
+
+
+class BatchedFDTD:
+    """One batched launch group per (equation, precision, schedule) of the given grids."""
+
+    def __init__(self, sims, tile_y=0):
+        if not sims:
+            raise ValueError("need at least one FDTD3D")
+        self._L = lib()
+        self.sims = list(sims)
+        groups = {}
+        for s in self.sims:
+            p = s.params
+            sched = "kg_2pass" if (p.equation == "klein_gordon" and p.schedule != "stages") \
+                else "stages"
+            groups.setdefault((p.equation, p.precision, sched), []).append(s)
+        self.groups = []
+        self.stream = torch.cuda.current_stream()
+        for key, members in groups.items():
+            arr = (ctypes.c_void_p * len(members))(*[m.handle.value for m in members])
+            b = ctypes.c_void_p()
+            check(self._L.wap_batch_create(arr, len(members), int(tile_y), ctypes.byref(b)),
+                  "wap_batch_create")
+            self.groups.append((key, members, b))
+        self.step_index = 0
+        self.captured_steps = 0
+
+    @property
+    def voxels(self):
+        return sum(s.params.voxels for s in self.sims)
+
+    def tiles(self):
+        out = []
+        for key, _, b in self.groups:
+            t, ty = ctypes.c_int64(), ctypes.c_int32()
+            check(self._L.wap_batch_tiles(b, ctypes.byref(t), ctypes.byref(ty)), "tiles")
+            out.append((key, t.value, ty.value))
+        return out
+
+    def step(self, nsteps=1):
+        for _, members, b in self.groups:
+            check(self._L.wap_batch_step(b, int(nsteps), int(self.step_index)), "wap_batch_step")
+        self.step_index += int(nsteps)
+        for s in self.sims:
+            s.step_index = self.step_index
+
+    def capture(self, nsteps):
+        """Record nsteps steps of every group as hipGraphs (one per group)."""
+        for _, _, b in self.groups:
+            check(self._L.wap_batch_capture(b, int(nsteps)), "wap_batch_capture")
+        self.captured_steps = int(nsteps)
+
+    def replay(self, timed=False):
+        """Run the captured steps from the current step index; returns device ms if timed."""
+        if not self.captured_steps:
+            raise RuntimeError("capture() first")
+        total = 0.0
+        for _, _, b in self.groups:
+            ms = ctypes.c_double()
+            check(self._L.wap_batch_replay(b, int(self.step_index),
+                                           ctypes.byref(ms) if timed else None), "replay")
+            total += ms.value
+        self.step_index += self.captured_steps
+        for s in self.sims:
+            s.step_index = self.step_index
+        return total if timed else None
+
+    def synchronize(self):
+        for _, _, b in self.groups:
+            check(self._L.wap_batch_synchronize(b), "wap_batch_synchronize")
+
+    def close(self):
+        for _, _, b in self.groups:
+            if b:
+                self._L.wap_batch_destroy(b)
+        self.groups = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

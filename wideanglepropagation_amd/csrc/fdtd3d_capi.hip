@@ -247,39 +247,57 @@ wap::PassArgs<R> base_args(const wap_handle *h) {
   return a;
 }
 
-template <typename R, int EQ, bool K>
-int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
-              double *pass_ms) {
+// Arguments of stage pass s (1..4) of the 4-pass schedule.  ty overrides the
+// handle's rows per workgroup (batched launches share one workgroup shape).
+template <typename R>
+wap::PassArgs<R> stage_args(const wap_handle *h, int s, int ty) {
   using C = wap::cplx<R>;
-  const dim3 block(wap::kWaveX, h->tile_y);
-  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
-  const R dt = (R)h->d.dt;
   wap::PassArgs<R> a = base_args<R>(h);
+  a.tile_y = ty;
+  a.nty = (h->d.ny + ty - 1) / ty;
+  const R dt = (R)h->d.dt;
   C *psi = (C *)h->psi, *pi = (C *)h->pi;
   C *ya = (C *)h->ya_psi, *yaq = (C *)h->ya_pi;
   C *yb = (C *)h->yb_psi, *ybq = (C *)h->yb_pi;
+  switch (s) {
+    case 1:  // psi_n -> y2, acc
+      a.yin_psi = psi; a.yin_pi = pi; a.out_psi = ya; a.out_pi = yaq;
+      a.w_acc = dt / (R)6; a.w_y = dt / (R)2;
+      break;
+    case 2:  // y2 -> y3
+      a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = yb; a.out_pi = ybq;
+      a.w_acc = dt / (R)3; a.w_y = dt / (R)2;
+      break;
+    case 3:  // y3 -> y4
+      a.yin_psi = yb; a.yin_pi = ybq; a.out_psi = ya; a.out_pi = yaq;
+      a.w_acc = dt / (R)3; a.w_y = dt;
+      break;
+    default:  // y4 -> psi_{n+1}
+      a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = psi; a.out_pi = pi;
+      a.w_acc = dt / (R)6; a.w_y = (R)0;
+      break;
+  }
+  return a;
+}
+
+template <typename R, int EQ, bool K>
+int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+              double *pass_ms) {
+  const dim3 block(wap::kWaveX, h->tile_y);
+  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
+  wap::PassArgs<R> a[4];
+  for (int s = 0; s < 4; ++s) a[s] = stage_args<R>(h, s + 1, h->tile_y);
   for (int64_t n = 0; n < nsteps; ++n) {
-    fill_source<R>(h, step_time(h, step0, n, t0), a);
+    const double t = step_time(h, step0, n, t0);
+    for (int s = 0; s < 4; ++s) fill_source<R>(h, t, a[s]);
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    // P1: psi_n -> y2, acc
-    a.yin_psi = psi; a.yin_pi = pi; a.out_psi = ya; a.out_pi = yaq;
-    a.w_acc = dt / (R)6; a.w_y = dt / (R)2;
-    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 1, K>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 1, K>), grid, block, 0, h->stream, a[0]);
     if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
-    // P2: y2 -> y3
-    a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = yb; a.out_pi = ybq;
-    a.w_acc = dt / (R)3; a.w_y = dt / (R)2;
-    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 2, K>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 2, K>), grid, block, 0, h->stream, a[1]);
     if (ev) WAP_HIP(hipEventRecord(ev[2], h->stream));
-    // P3: y3 -> y4
-    a.yin_psi = yb; a.yin_pi = ybq; a.out_psi = ya; a.out_pi = yaq;
-    a.w_acc = dt / (R)3; a.w_y = dt;
-    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 3, K>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 3, K>), grid, block, 0, h->stream, a[2]);
     if (ev) WAP_HIP(hipEventRecord(ev[3], h->stream));
-    // P4: y4 -> psi_{n+1}
-    a.yin_psi = ya; a.yin_pi = yaq; a.out_psi = psi; a.out_pi = pi;
-    a.w_acc = dt / (R)6; a.w_y = (R)0;
-    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 4, K>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 4, K>), grid, block, 0, h->stream, a[3]);
     if (ev) {
       WAP_HIP(hipEventRecord(ev[4], h->stream));
       WAP_HIP(hipEventSynchronize(ev[4]));
@@ -299,7 +317,7 @@ bool kg_two_pass(const wap_handle *h) {
 }
 
 template <typename R>
-wap::KG2Args<R> kg2_args(const wap_handle *h) {
+wap::KG2Args<R> kg2_args(const wap_handle *h, int ty = 0) {
   using C = wap::cplx<R>;
   wap::KG2Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
@@ -330,6 +348,10 @@ wap::KG2Args<R> kg2_args(const wap_handle *h) {
   a.hy = (R)h->hy;
   a.m2 = (R)h->m2;
   a.h = (R)h->d.dt;
+  if (ty > 0) {
+    a.tile_y = ty;
+    a.nty = (h->d.ny + ty - 1) / ty;
+  }
   return a;
 }
 
@@ -804,6 +826,301 @@ int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *g
   if (tile_y) *tile_y = h->tile_y;
   if (tile_z) *tile_z = h->tile_z;
   if (groups) *groups = (int64_t)h->ntx * h->nty * h->ntz;
+  return WAP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+struct wap_batch {
+  std::vector<wap_handle *> hs;
+  bool kg2 = false;
+  int precision = 0;
+  int equation = 0;
+  int tile_y = 4;
+  int total_tiles = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void *d_args[4] = {nullptr, nullptr, nullptr, nullptr};
+  int *d_tile_begin = nullptr;
+  void *d_phase = nullptr;  // [steps][ninst] complex, e^{-i w_i t_n}
+  int64_t phase_steps = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_steps = 0;
+};
+
+namespace {
+
+void batch_free(wap_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *&p : b->d_args)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  if (b->d_tile_begin) (void)hipFree(b->d_tile_begin);
+  if (b->d_phase) (void)hipFree(b->d_phase);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->d_tile_begin = nullptr;
+  b->d_phase = nullptr;
+}
+
+// time-free source coefficients: the kernel multiplies by the step's phasor
+template <typename A>
+void timeless_source(const wap_handle *h, A &a) {
+  using R = decltype(a.hx);
+  a.src_k0 = h->src_k0;
+  a.src_n = h->nsrc;
+  const std::complex<double> mi_w(0.0, -h->d.e_sim);
+  for (int q = 0; q < wap::kMaxBand; ++q) {
+    if (q < h->nsrc) {
+      const std::complex<double> c = h->src_spatial[q];
+      const std::complex<double> cq = mi_w * c;
+      a.src_keep[q] = (R)h->src_keep[q];
+      a.src_coef[q] = wap::cplx<R>{(R)c.real(), (R)c.imag()};
+      a.src_coef_pi[q] = wap::cplx<R>{(R)cq.real(), (R)cq.imag()};
+    } else {
+      a.src_keep[q] = (R)1;
+      a.src_coef[q] = wap::cplx<R>{(R)0, (R)0};
+      a.src_coef_pi[q] = wap::cplx<R>{(R)0, (R)0};
+    }
+  }
+}
+
+template <typename R>
+int batch_build(wap_batch *b) {
+  const int n = (int)b->hs.size();
+  std::vector<int> begin(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const wap_handle *h = b->hs[i];
+    const int nty = (h->d.ny + b->tile_y - 1) / b->tile_y;
+    begin[i + 1] = begin[i] + h->ntx * nty * h->ntz;
+  }
+  b->total_tiles = begin[n];
+  WAP_HIP(hipMalloc((void **)&b->d_tile_begin, (n + 1) * sizeof(int)));
+  WAP_HIP(hipMemcpy(b->d_tile_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (b->kg2) {
+    std::vector<wap::KG2Args<R>> a(n);
+    for (int i = 0; i < n; ++i) {
+      a[i] = kg2_args<R>(b->hs[i], b->tile_y);
+      timeless_source(b->hs[i], a[i]);
+    }
+    WAP_HIP(hipMalloc(&b->d_args[0], n * sizeof(wap::KG2Args<R>)));
+    WAP_HIP(hipMemcpy(b->d_args[0], a.data(), n * sizeof(wap::KG2Args<R>), hipMemcpyHostToDevice));
+  } else {
+    for (int s = 0; s < 4; ++s) {
+      std::vector<wap::PassArgs<R>> a(n);
+      for (int i = 0; i < n; ++i) {
+        a[i] = stage_args<R>(b->hs[i], s + 1, b->tile_y);
+        timeless_source(b->hs[i], a[i]);
+      }
+      WAP_HIP(hipMalloc(&b->d_args[s], n * sizeof(wap::PassArgs<R>)));
+      WAP_HIP(hipMemcpy(b->d_args[s], a.data(), n * sizeof(wap::PassArgs<R>),
+                        hipMemcpyHostToDevice));
+    }
+  }
+  return WAP_OK;
+}
+
+// e^{-i w_i t_n} for n in [0, nsteps) into the device table (synchronous upload)
+template <typename R>
+int batch_phases(wap_batch *b, int64_t nsteps, int64_t step0) {
+  const int n = (int)b->hs.size();
+  if (nsteps > b->phase_steps) {
+    if (b->d_phase) {
+      WAP_HIP(hipStreamSynchronize(b->stream));
+      (void)hipFree(b->d_phase);
+      b->d_phase = nullptr;
+    }
+    WAP_HIP(hipMalloc(&b->d_phase, (size_t)nsteps * n * sizeof(wap::cplx<R>)));
+    b->phase_steps = nsteps;
+  }
+  std::vector<wap::cplx<R>> ph((size_t)nsteps * n);
+  for (int64_t t = 0; t < nsteps; ++t)
+    for (int i = 0; i < n; ++i) {
+      const wap_handle *h = b->hs[i];
+      const std::complex<double> z = std::polar(1.0, -h->d.e_sim * step_time(h, step0, t, 0.0));
+      ph[(size_t)t * n + i] = wap::cplx<R>{(R)z.real(), (R)z.imag()};
+    }
+  WAP_HIP(hipStreamSynchronize(b->stream));
+  WAP_HIP(hipMemcpy(b->d_phase, ph.data(), ph.size() * sizeof(wap::cplx<R>),
+                    hipMemcpyHostToDevice));
+  return WAP_OK;
+}
+
+template <typename R, int EQ, bool K>
+int batch_launch(wap_batch *b, int64_t nsteps) {
+  using C = wap::cplx<R>;
+  const int n = (int)b->hs.size();
+  const dim3 block(wap::kWaveX, b->tile_y);
+  const dim3 grid((unsigned)b->total_tiles);
+  for (int64_t t = 0; t < nsteps; ++t) {
+    const C *ph = (const C *)b->d_phase + (size_t)t * n;
+    if (b->kg2) {
+      const wap::KG2Args<R> *a = (const wap::KG2Args<R> *)b->d_args[0];
+      hipLaunchKernelGGL((wap::kg2_pass_batched<R, 1, K>), grid, block, 0, b->stream, a,
+                         b->d_tile_begin, n, ph);
+      hipLaunchKernelGGL((wap::kg2_pass_batched<R, 2, K>), grid, block, 0, b->stream, a,
+                         b->d_tile_begin, n, ph);
+    } else {
+      const wap::PassArgs<R> *const *a = (const wap::PassArgs<R> *const *)b->d_args;
+      hipLaunchKernelGGL((wap::rk4_pass_batched<R, EQ, 1, K>), grid, block, 0, b->stream, a[0],
+                         b->d_tile_begin, n, ph);
+      hipLaunchKernelGGL((wap::rk4_pass_batched<R, EQ, 2, K>), grid, block, 0, b->stream, a[1],
+                         b->d_tile_begin, n, ph);
+      hipLaunchKernelGGL((wap::rk4_pass_batched<R, EQ, 3, K>), grid, block, 0, b->stream, a[2],
+                         b->d_tile_begin, n, ph);
+      hipLaunchKernelGGL((wap::rk4_pass_batched<R, EQ, 4, K>), grid, block, 0, b->stream, a[3],
+                         b->d_tile_begin, n, ph);
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+int batch_dispatch(wap_batch *b, int64_t nsteps) {
+  const bool kg = b->equation == WAP_EQ_KLEIN_GORDON;
+  switch (b->precision) {
+    case WAP_PREC_F64:
+      return kg ? batch_launch<double, 1, false>(b, nsteps) : batch_launch<double, 0, false>(b, nsteps);
+    case WAP_PREC_F32:
+      return kg ? batch_launch<float, 1, false>(b, nsteps) : batch_launch<float, 0, false>(b, nsteps);
+    default:
+      return kg ? batch_launch<float, 1, true>(b, nsteps) : batch_launch<float, 0, true>(b, nsteps);
+  }
+}
+
+int batch_phase_dispatch(wap_batch *b, int64_t nsteps, int64_t step0) {
+  return b->precision == WAP_PREC_F64 ? batch_phases<double>(b, nsteps, step0)
+                                      : batch_phases<float>(b, nsteps, step0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **out) {
+  if (!out || !handles || n < 1) return fail(WAP_ERR_ARG, "need at least one handle");
+  *out = nullptr;
+  if (tile_y < 0 || tile_y > 8 || (tile_y & (tile_y - 1)))
+    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8");
+  const wap_handle *h0 = handles[0];
+  if (!h0) return fail(WAP_ERR_ARG, "NULL handle");
+  for (int i = 0; i < n; ++i) {
+    const wap_handle *h = handles[i];
+    if (!h) return fail(WAP_ERR_ARG, "NULL handle");
+    if (h->d.equation != h0->d.equation || h->d.precision != h0->d.precision ||
+        kg_two_pass(h) != kg_two_pass(h0))
+      return fail(WAP_ERR_ARG, "batched handles must share equation, precision and schedule");
+    if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set on a batched handle");
+  }
+  wap_batch *b = new (std::nothrow) wap_batch();
+  if (!b) return fail(WAP_ERR_OOM, "host allocation failed");
+  b->hs.assign(handles, handles + n);
+  b->kg2 = kg_two_pass(h0);
+  b->precision = h0->d.precision;
+  b->equation = h0->d.equation;
+  b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;
+  b->stream = h0->stream;
+  int rc = b->precision == WAP_PREC_F64 ? batch_build<double>(b) : batch_build<float>(b);
+  if (rc) {
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return WAP_OK;
+}
+
+int wap_batch_destroy(wap_batch *b) {
+  if (!b) return WAP_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  else (void)hipDeviceSynchronize();
+  batch_free(b);
+  delete b;
+  return WAP_OK;
+}
+
+int wap_batch_set_stream(wap_batch *b, void *stream) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  b->own_stream = false;
+  b->stream = (hipStream_t)stream;
+  return WAP_OK;
+}
+
+int wap_batch_tiles(wap_batch *b, int64_t *tiles, int32_t *tile_y) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  if (tiles) *tiles = b->total_tiles;
+  if (tile_y) *tile_y = b->tile_y;
+  return WAP_OK;
+}
+
+int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  if (nsteps < 0) return fail(WAP_ERR_ARG, "nsteps < 0");
+  if (nsteps == 0) return WAP_OK;
+  int rc = batch_phase_dispatch(b, nsteps, step0);
+  if (rc) return rc;
+  return batch_dispatch(b, nsteps);
+}
+
+int wap_batch_capture(wap_batch *b, int64_t nsteps) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  if (nsteps < 1) return fail(WAP_ERR_ARG, "nsteps < 1");
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (!b->stream) {
+    // capture needs a stream of its own (default priority)
+    WAP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->own_stream = true;
+  }
+  // allocate the phase table before capture; its contents are written per replay
+  int rc = batch_phase_dispatch(b, nsteps, 0);
+  if (rc) return rc;
+  WAP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  rc = batch_dispatch(b, nsteps);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  WAP_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_steps = nsteps;
+  return WAP_OK;
+}
+
+int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  if (!b->exec) return fail(WAP_ERR_STATE, "no captured graph (wap_batch_capture)");
+  int rc = batch_phase_dispatch(b, b->graph_steps, step0);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    WAP_HIP(hipEventCreate(&e0));
+    WAP_HIP(hipEventCreate(&e1));
+    WAP_HIP(hipEventRecord(e0, b->stream));
+  }
+  WAP_HIP(hipGraphLaunch(b->exec, b->stream));
+  if (ms) {
+    WAP_HIP(hipEventRecord(e1, b->stream));
+    WAP_HIP(hipEventSynchronize(e1));
+    float f = 0.f;
+    WAP_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return WAP_OK;
+}
+
+int wap_batch_synchronize(wap_batch *b) {
+  if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
+  WAP_HIP(hipStreamSynchronize(b->stream));
   return WAP_OK;
 }
 

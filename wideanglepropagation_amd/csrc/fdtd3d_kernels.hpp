@@ -105,9 +105,11 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
 }
 
 // Source band selection without dynamic indexing of the kernarg arrays.
+// ph multiplies the coefficients: (1, 0) when they already carry e^{-i w t}
+// (single grid), the per-step phasor of the instance in batched launches.
 template <typename R>
-__device__ __forceinline__ bool src_pick(const PassArgs<R> &p, int k, R &keep, cplx<R> &coef,
-                                         cplx<R> &coef_pi) {
+__device__ __forceinline__ bool src_pick(const PassArgs<R> &p, int k, cplx<R> ph, R &keep,
+                                         cplx<R> &coef, cplx<R> &coef_pi) {
   const int b = k - p.src_k0;
   if (b < 0 || b >= p.src_n) return false;
 #pragma unroll
@@ -118,18 +120,20 @@ __device__ __forceinline__ bool src_pick(const PassArgs<R> &p, int k, R &keep, c
       coef_pi = p.src_coef_pi[q];
     }
   }
+  coef = cmul<R>(coef, ph);
+  coef_pi = cmul<R>(coef_pi, ph);
   return true;
 }
 
 // Load psi component of the stage input; P1 applies the soft source on load.
 template <typename R, int STAGE>
-__device__ __forceinline__ cplx<R> load_in(const PassArgs<R> &p, const cplx<R> *arr, size_t idx,
-                                           int k, int ij) {
+__device__ __forceinline__ cplx<R> load_in(const PassArgs<R> &p, cplx<R> ph, const cplx<R> *arr,
+                                           size_t idx, int k, int ij) {
   cplx<R> v = arr[idx];
   if (STAGE == 1) {
     R keep;
     cplx<R> coef, coef_pi;
-    if (src_pick(p, k, keep, coef, coef_pi)) {
+    if (src_pick(p, k, ph, keep, coef, coef_pi)) {
       cplx<R> inc = coef;
       if (p.prof) inc = cmul<R>(coef, p.prof[ij]);
       v = caxpy<R>(cscale<R>(v, keep), R(1), inc);
@@ -138,11 +142,10 @@ __device__ __forceinline__ cplx<R> load_in(const PassArgs<R> &p, const cplx<R> *
   return v;
 }
 
+// One workgroup's column tile of one stage pass.
 template <typename R, int EQ, int STAGE, bool KAHAN>
-__global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
+__device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<R> ph) {
   using C = cplx<R>;
-  const int ntiles = p.ntx * p.nty * p.ntz;
-  const int tile = xcd_tile(blockIdx.x, ntiles);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int ty = rest % p.nty;
@@ -164,13 +167,14 @@ __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
   const int o_yp = jp * nx + i, o_ym = jm * nx + i;
 
   const C zero = mkc<R>(R(0), R(0));
-  C ym = (k0 > 0) ? load_in<R, STAGE>(p, p.yin_psi, (size_t)(k0 - 1) * plane + ij, k0 - 1, ij) : zero;
-  C yc = load_in<R, STAGE>(p, p.yin_psi, (size_t)k0 * plane + ij, k0, ij);
+  C ym = (k0 > 0) ? load_in<R, STAGE>(p, ph, p.yin_psi, (size_t)(k0 - 1) * plane + ij, k0 - 1, ij)
+                   : zero;
+  C yc = load_in<R, STAGE>(p, ph, p.yin_psi, (size_t)k0 * plane + ij, k0, ij);
 
   for (int k = k0; k < k1; ++k) {
     const size_t base = (size_t)k * plane;
     const size_t idx = base + ij;
-    C yp = (k + 1 < nz) ? load_in<R, STAGE>(p, p.yin_psi, idx + plane, k + 1, ij) : zero;
+    C yp = (k + 1 < nz) ? load_in<R, STAGE>(p, ph, p.yin_psi, idx + plane, k + 1, ij) : zero;
     const bool interior = (k > 0) && (k < nz - 1);
     const R v = p.V[idx];
 
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
     C q = zero;
     R keep = R(1);
     C coef = zero, coef_pi = zero;
-    const bool in_band = src_pick(p, k, keep, coef, coef_pi);
+    const bool in_band = src_pick(p, k, ph, keep, coef, coef_pi);
     C inc = coef, inc_pi = coef_pi;
     if (in_band && p.prof) {
       const C pr = p.prof[ij];
@@ -192,10 +196,10 @@ __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
 
     C kpsi = zero, kpi = zero;
     if (interior) {
-      const C xp = load_in<R, STAGE>(p, p.yin_psi, base + o_xp, k, o_xp);
-      const C xm = load_in<R, STAGE>(p, p.yin_psi, base + o_xm, k, o_xm);
-      const C yyp = load_in<R, STAGE>(p, p.yin_psi, base + o_yp, k, o_yp);
-      const C yym = load_in<R, STAGE>(p, p.yin_psi, base + o_ym, k, o_ym);
+      const C xp = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_xp, k, o_xp);
+      const C xm = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_xm, k, o_xm);
+      const C yyp = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_yp, k, o_yp);
+      const C yym = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_ym, k, o_ym);
       const C wzp = p.zc[2 * k];
       const C wzm = p.zc[2 * k + 1];
       const C lx = cadd<R>(csub<R>(xp, yc), csub<R>(xm, yc));
@@ -270,6 +274,33 @@ __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
   }
 }
 
+template <typename R, int EQ, int STAGE, bool KAHAN>
+__global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
+  rk4_column<R, EQ, STAGE, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz),
+                                  mkc<R>(R(1), R(0)));
+}
+
+// Many independent grids in one launch.  args[inst] holds instance inst's
+// arguments for this stage with time-free source coefficients; ph[inst] is
+// its e^{-i w t} for this step; tile_begin[inst] is the prefix sum of the
+// instances' tile counts (tile_begin[ninst] = total).  All instances share
+// the workgroup shape (64 x tile_y); tiles are equal-sized units of work.
+__device__ __forceinline__ int find_instance(const int *__restrict__ tile_begin, int ninst, int t) {
+  int inst = 0;
+  while (inst + 1 < ninst && tile_begin[inst + 1] <= t) ++inst;
+  return inst;
+}
+
+template <typename R, int EQ, int STAGE, bool KAHAN>
+__global__ __launch_bounds__(512) void rk4_pass_batched(const PassArgs<R> *__restrict__ args,
+                                                        const int *__restrict__ tile_begin,
+                                                        int ninst,
+                                                        const cplx<R> *__restrict__ ph) {
+  const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
+  const int inst = find_instance(tile_begin, ninst, t);
+  rk4_column<R, EQ, STAGE, KAHAN>(args[inst], t - tile_begin[inst], ph[inst]);
+}
+
 // ------------------------------------------------- Klein-Gordon in 2 passes
 //
 // With y = (psi, pi), k = M y, M = [[0, I], [A, D]], A = c^2 Lap - (m^2 - V^2),
@@ -323,8 +354,8 @@ struct KG2Args {
 };
 
 template <typename R>
-__device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, int k, int ij, R &keep, cplx<R> &inc,
-                                        cplx<R> &inc_pi) {
+__device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, cplx<R> ph, int k, int ij, R &keep,
+                                        cplx<R> &inc, cplx<R> &inc_pi) {
   const int b = k - p.src_k0;
   if (b < 0 || b >= p.src_n) return false;
   cplx<R> c = mkc<R>(R(0), R(0)), cq = c;
@@ -337,6 +368,8 @@ __device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, int k, int ij, R &k
       cq = p.src_coef_pi[t];
     }
   }
+  c = cmul<R>(c, ph);
+  cq = cmul<R>(cq, ph);
   if (p.prof) {
     const cplx<R> pr = p.prof[ij];
     c = cmul<R>(c, pr);
@@ -349,11 +382,11 @@ __device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, int k, int ij, R &k
 
 // injected state (s, q) at voxel (k, ij)
 template <typename R>
-__device__ __forceinline__ void kg2_state(const KG2Args<R> &p, size_t idx, int k, int ij, bool want_s,
-                                          cplx<R> &s, cplx<R> &q) {
+__device__ __forceinline__ void kg2_state(const KG2Args<R> &p, cplx<R> ph, size_t idx, int k, int ij,
+                                          bool want_s, cplx<R> &s, cplx<R> &q) {
   R keep;
   cplx<R> inc, incq;
-  const bool band = kg2_src(p, k, ij, keep, inc, incq);
+  const bool band = kg2_src(p, ph, k, ij, keep, inc, incq);
   if (want_s) {
     s = p.st_psi[idx];
     if (band) s = caxpy<R>(cscale<R>(s, keep), R(1), inc);
@@ -382,10 +415,8 @@ __device__ __forceinline__ cplx<R> kg2_damp(R v, R g, cplx<R> z) {
 }
 
 template <typename R, int PASS, bool KAHAN>
-__global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
+__device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R> ph) {
   using C = cplx<R>;
-  const int ntiles = p.ntx * p.nty * p.ntz;
-  const int tile = xcd_tile(blockIdx.x, ntiles);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int ty = rest % p.nty;
@@ -410,7 +441,7 @@ __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
   auto fields = [&](int k, int off, C &a, C &b) {
     const size_t id = (size_t)k * plane + off;
     if (PASS == 1) {
-      kg2_state<R>(p, id, k, off, true, a, b);
+      kg2_state<R>(p, ph, id, k, off, true, a, b);
       if (k == 0 || k == nz - 1) b = zero;
     } else {
       a = p.y3[id];
@@ -434,7 +465,7 @@ __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
       C q = bc;  // q^ == q on interior planes
       if (!interior) {
         C dummy;
-        kg2_state<R>(p, idx, k, ij, false, dummy, q);
+        kg2_state<R>(p, ph, idx, k, ij, false, dummy, q);
       }
       if (interior) {
         C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
@@ -466,7 +497,7 @@ __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
       }
     } else {
       C s = zero, q;
-      kg2_state<R>(p, idx, k, ij, KAHAN, s, q);
+      kg2_state<R>(p, ph, idx, k, ij, KAHAN, s, q);
       C dpsi = zero, dpi = zero;  // increments beyond P, Q
       if (interior) {
         C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
@@ -493,7 +524,7 @@ __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
       } else {
         R keep;
         C inc, incq;
-        const bool band = kg2_src(p, k, ij, keep, inc, incq);
+        const bool band = kg2_src(p, ph, k, ij, keep, inc, incq);
         C cs = p.cmp_psi[idx], cq = p.cmp_pi[idx];
         if (band) {
           cs = cscale<R>(cs, keep);
@@ -514,6 +545,21 @@ __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
     ac = ap;
     bc = bp;
   }
+}
+
+template <typename R, int PASS, bool KAHAN>
+__global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
+  kg2_column<R, PASS, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), mkc<R>(R(1), R(0)));
+}
+
+template <typename R, int PASS, bool KAHAN>
+__global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__restrict__ args,
+                                                        const int *__restrict__ tile_begin,
+                                                        int ninst,
+                                                        const cplx<R> *__restrict__ ph) {
+  const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
+  const int inst = find_instance(tile_begin, ninst, t);
+  kg2_column<R, PASS, KAHAN>(args[inst], t - tile_begin[inst], ph[inst]);
 }
 
 // ---------------------------------------------------------------- utilities
