@@ -184,6 +184,32 @@ int wap_batch_capture(wap_batch *b, int64_t nsteps);
 int wap_batch_replay(wap_batch *b, int64_t step0, double *ms);
 int wap_batch_synchronize(wap_batch *b);
 
+/*
+ * Discrete adjoint (no reference counterpart: the reference has no gradient).
+ * Gradient of a real loss of exit-plane values with respect to V, through the
+ * RK4 steps, the PML / sponge and the soft source.  Use: run the forward with
+ * checkpoints; then per segment restore a checkpoint with wap_set_state,
+ * wap_adjoint_record each step of the segment into slots 0..m-1 (this also
+ * advances the handle), and walk the segment backwards with wap_adjoint_step
+ * (slot m-1 .. 0), adding exit-plane seeds with wap_adjoint_seed.  f64 and f32
+ * handles; the records use the 4-pass stage schedule.
+ */
+typedef struct wap_adjoint wap_adjoint;
+
+/* Adjoint workspace with `slots` per-step records (4 stage inputs per record). */
+int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out);
+int wap_adjoint_destroy(wap_adjoint *a);
+/* lambda = 0, dL/dV = 0. */
+int wap_adjoint_reset(wap_adjoint *a);
+/* One forward step (its time as wap_step(h, 1, step, t0)), recording its stage inputs. */
+int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot);
+/* lambda <- adjoint of the recorded step; dL/dV += its contribution. */
+int wap_adjoint_step(wap_adjoint *a, int slot);
+/* lambda_psi[k] += conj(c) g for a device complex128 (ny,nx) plane g. */
+int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c_im);
+/* dL/dV as float64 (nz,ny,nx); synchronises. */
+int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,15 +197,17 @@ double step_time(const wap_handle *h, int64_t step0, int64_t n, double t0) {
   return t0 + (double)(step0 + n) * h->d.dt;
 }
 
-template <typename R>
-void fill_source(const wap_handle *h, double t, wap::PassArgs<R> &a) {
+// Source coefficients of a launch: time-free band weights/phasors plus the
+// step's e^{-i w t} (the kernel multiplies them, as batched launches do).
+template <typename A>
+void fill_source(const wap_handle *h, double t, A &a) {
+  using R = decltype(a.hx);
   a.src_k0 = h->src_k0;
   a.src_n = h->nsrc;
-  const std::complex<double> ph = std::polar(1.0, -h->d.e_sim * t);
   const std::complex<double> mi_w(0.0, -h->d.e_sim);
   for (int b = 0; b < wap::kMaxBand; ++b) {
     if (b < h->nsrc) {
-      const std::complex<double> c = h->src_spatial[b] * ph;
+      const std::complex<double> c = h->src_spatial[b];
       const std::complex<double> cq = mi_w * c;
       a.src_keep[b] = (R)h->src_keep[b];
       a.src_coef[b] = wap::cplx<R>{(R)c.real(), (R)c.imag()};
@@ -216,6 +218,8 @@ void fill_source(const wap_handle *h, double t, wap::PassArgs<R> &a) {
       a.src_coef_pi[b] = wap::cplx<R>{(R)0, (R)0};
     }
   }
+  const std::complex<double> ph = std::polar(1.0, -h->d.e_sim * t);
+  a.src_ph = wap::cplx<R>{(R)ph.real(), (R)ph.imag()};
 }
 
 template <typename R>
@@ -244,6 +248,8 @@ wap::PassArgs<R> base_args(const wap_handle *h) {
   a.acc_pi = (C *)h->acc_pi;
   a.cmp_psi = (C *)h->cmp_psi;
   a.cmp_pi = (C *)h->cmp_pi;
+  a.rec_psi = nullptr;
+  a.rec_pi = nullptr;
   return a;
 }
 
@@ -289,7 +295,7 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
   for (int s = 0; s < 4; ++s) a[s] = stage_args<R>(h, s + 1, h->tile_y);
   for (int64_t n = 0; n < nsteps; ++n) {
     const double t = step_time(h, step0, n, t0);
-    for (int s = 0; s < 4; ++s) fill_source<R>(h, t, a[s]);
+    for (int s = 0; s < 4; ++s) fill_source(h, t, a[s]);
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
     hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 1, K>), grid, block, 0, h->stream, a[0]);
     if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
@@ -361,16 +367,8 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   const dim3 block(wap::kWaveX, h->tile_y);
   const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
   wap::KG2Args<R> a = kg2_args<R>(h);
-  wap::PassArgs<R> src{};
   for (int64_t n = 0; n < nsteps; ++n) {
-    fill_source<R>(h, step_time(h, step0, n, t0), src);
-    a.src_k0 = src.src_k0;
-    a.src_n = src.src_n;
-    for (int b = 0; b < wap::kMaxBand; ++b) {
-      a.src_keep[b] = src.src_keep[b];
-      a.src_coef[b] = src.src_coef[b];
-      a.src_coef_pi[b] = src.src_coef_pi[b];
-    }
+    fill_source(h, step_time(h, step0, n, t0), a);
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
     hipLaunchKernelGGL((wap::kg2_pass<R, 1, K>), grid, block, 0, h->stream, a);
     if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
@@ -866,26 +864,10 @@ void batch_free(wap_batch *b) {
   b->d_phase = nullptr;
 }
 
-// time-free source coefficients: the kernel multiplies by the step's phasor
+// batched launches read the per-step phasor from a device table instead of src_ph
 template <typename A>
 void timeless_source(const wap_handle *h, A &a) {
-  using R = decltype(a.hx);
-  a.src_k0 = h->src_k0;
-  a.src_n = h->nsrc;
-  const std::complex<double> mi_w(0.0, -h->d.e_sim);
-  for (int q = 0; q < wap::kMaxBand; ++q) {
-    if (q < h->nsrc) {
-      const std::complex<double> c = h->src_spatial[q];
-      const std::complex<double> cq = mi_w * c;
-      a.src_keep[q] = (R)h->src_keep[q];
-      a.src_coef[q] = wap::cplx<R>{(R)c.real(), (R)c.imag()};
-      a.src_coef_pi[q] = wap::cplx<R>{(R)cq.real(), (R)cq.imag()};
-    } else {
-      a.src_keep[q] = (R)1;
-      a.src_coef[q] = wap::cplx<R>{(R)0, (R)0};
-      a.src_coef_pi[q] = wap::cplx<R>{(R)0, (R)0};
-    }
-  }
+  fill_source(h, 0.0, a);
 }
 
 template <typename R>
@@ -1121,6 +1103,216 @@ int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
 int wap_batch_synchronize(wap_batch *b) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   WAP_HIP(hipStreamSynchronize(b->stream));
+  return WAP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ adjoint
+struct wap_adjoint {
+  wap_handle *h = nullptr;
+  int slots = 0;
+  int ncomp = 1;
+  std::vector<void *> rec;   // [slot][stage 0..3][comp]
+  void *lam[2] = {nullptr, nullptr};
+  void *acc[2] = {nullptr, nullptr};
+  void *kba[2] = {nullptr, nullptr};
+  void *kbb[2] = {nullptr, nullptr};
+  double *vbar = nullptr;
+};
+
+namespace {
+
+void adjoint_free(wap_adjoint *a) {
+  for (void *&p : a->rec)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  void **bufs[] = {&a->lam[0], &a->lam[1], &a->acc[0], &a->acc[1], &a->kba[0], &a->kba[1],
+                   &a->kbb[0], &a->kbb[1], (void **)&a->vbar};
+  for (void **b : bufs)
+    if (*b) { (void)hipFree(*b); *b = nullptr; }
+}
+
+void *rec_ptr(const wap_adjoint *a, int slot, int stage, int comp) {
+  return a->rec[((size_t)slot * 4 + stage) * a->ncomp + comp];
+}
+
+template <typename R, int EQ>
+int adjoint_record_t(wap_adjoint *ad, int64_t step, double t0, int slot) {
+  using C = wap::cplx<R>;
+  wap_handle *h = ad->h;
+  const dim3 block(wap::kWaveX, h->tile_y);
+  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
+  wap::PassArgs<R> a[4];
+  for (int s = 0; s < 4; ++s) a[s] = stage_args<R>(h, s + 1, h->tile_y);
+  const int q = EQ == 1 ? 1 : 0;
+  // P1: state -> Y2 (records Y1 = injected state)
+  a[0].out_psi = (C *)rec_ptr(ad, slot, 1, 0);
+  a[0].out_pi = (C *)rec_ptr(ad, slot, 1, q);
+  a[0].rec_psi = (C *)rec_ptr(ad, slot, 0, 0);
+  a[0].rec_pi = (C *)rec_ptr(ad, slot, 0, q);
+  // P2: Y2 -> Y3, P3: Y3 -> Y4, P4: Y4 -> state
+  for (int s = 1; s < 4; ++s) {
+    a[s].yin_psi = (const C *)rec_ptr(ad, slot, s, 0);
+    a[s].yin_pi = (const C *)rec_ptr(ad, slot, s, q);
+    if (s < 3) {
+      a[s].out_psi = (C *)rec_ptr(ad, slot, s + 1, 0);
+      a[s].out_pi = (C *)rec_ptr(ad, slot, s + 1, q);
+    }
+  }
+  if (EQ == 0)
+    for (int s = 0; s < 4; ++s) { a[s].yin_pi = nullptr; a[s].out_pi = nullptr; a[s].rec_pi = nullptr; }
+  const double t = step_time(h, step, 0, t0);
+  for (int s = 0; s < 4; ++s) fill_source(h, t, a[s]);
+  hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 1, false>), grid, block, 0, h->stream, a[0]);
+  hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 2, false>), grid, block, 0, h->stream, a[1]);
+  hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 3, false>), grid, block, 0, h->stream, a[2]);
+  hipLaunchKernelGGL((wap::rk4_pass<R, EQ, 4, false>), grid, block, 0, h->stream, a[3]);
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+template <typename R, int EQ>
+int adjoint_step_t(wap_adjoint *ad, int slot) {
+  using C = wap::cplx<R>;
+  wap_handle *h = ad->h;
+  const dim3 block(wap::kWaveX, h->tile_y);
+  const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
+  wap::AdjArgs<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.lam_psi = (C *)ad->lam[0];
+  a.lam_pi = (C *)ad->lam[1];
+  a.acc_psi = (C *)ad->acc[0];
+  a.acc_pi = (C *)ad->acc[1];
+  a.vbar = ad->vbar;
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.gam = (const R *)h->gam;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_y = h->tile_y; a.tile_z = h->tile_z;
+  a.ntx = h->ntx; a.nty = h->nty; a.ntz = h->ntz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy; a.m2 = (R)h->m2; a.h = (R)h->d.dt;
+  a.src_k0 = h->src_k0;
+  a.src_n = h->nsrc;
+  for (int q = 0; q < wap::kMaxBand; ++q) a.src_keep[q] = q < h->nsrc ? (R)h->src_keep[q] : (R)1;
+  const int q = EQ == 1 ? 1 : 0;
+  void *const *kin[4] = {nullptr, ad->kba, ad->kbb, ad->kba};   // by stage S = 4, 3, 2, 1
+  void *const *kout[4] = {ad->kba, ad->kbb, ad->kba, nullptr};
+  for (int r = 0; r < 4; ++r) {
+    const int S = 4 - r;
+    a.y_psi = (const C *)rec_ptr(ad, slot, S - 1, 0);
+    a.y_pi = (const C *)rec_ptr(ad, slot, S - 1, q);
+    a.kb_in_psi = kin[r] ? (const C *)kin[r][0] : nullptr;
+    a.kb_in_pi = kin[r] ? (const C *)kin[r][1] : nullptr;
+    a.kb_out_psi = kout[r] ? (C *)kout[r][0] : nullptr;
+    a.kb_out_pi = kout[r] ? (C *)kout[r][1] : nullptr;
+    switch (S) {
+      case 4: hipLaunchKernelGGL((wap::adj_pass<R, EQ, 4>), grid, block, 0, h->stream, a); break;
+      case 3: hipLaunchKernelGGL((wap::adj_pass<R, EQ, 3>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL((wap::adj_pass<R, EQ, 2>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::adj_pass<R, EQ, 1>), grid, block, 0, h->stream, a); break;
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
+  if (!h || !out) return fail(WAP_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (slots < 1) return fail(WAP_ERR_ARG, "slots must be >= 1");
+  if (is_kahan(h)) return fail(WAP_ERR_ARG, "the adjoint supports f64 and f32 handles");
+  wap_adjoint *a = new (std::nothrow) wap_adjoint();
+  if (!a) return fail(WAP_ERR_OOM, "host allocation failed");
+  a->h = h;
+  a->slots = slots;
+  a->ncomp = is_kg(h) ? 2 : 1;
+  const size_t fb = h->n * h->cs;
+  a->rec.assign((size_t)slots * 4 * a->ncomp, nullptr);
+  int rc = WAP_OK;
+  for (size_t t = 0; t < a->rec.size() && !rc; ++t) {
+    hipError_t e = hipMalloc(&a->rec[t], fb);
+    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint records: ") + hipGetErrorString(e));
+  }
+  for (int c = 0; c < a->ncomp && !rc; ++c) {
+    void **bufs[] = {&a->lam[c], &a->acc[c], &a->kba[c], &a->kbb[c]};
+    for (void **b : bufs) {
+      hipError_t e = hipMalloc(b, fb);
+      if (e != hipSuccess) { rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e)); break; }
+    }
+  }
+  if (!rc) {
+    hipError_t e = hipMalloc((void **)&a->vbar, h->n * sizeof(double));
+    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    adjoint_free(a);
+    delete a;
+    return rc;
+  }
+  *out = a;
+  return wap_adjoint_reset(a);
+}
+
+int wap_adjoint_destroy(wap_adjoint *a) {
+  if (!a) return WAP_OK;
+  (void)hipStreamSynchronize(a->h->stream);
+  adjoint_free(a);
+  delete a;
+  return WAP_OK;
+}
+
+int wap_adjoint_reset(wap_adjoint *a) {
+  if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
+  const size_t fb = a->h->n * a->h->cs;
+  for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam[c], 0, fb, a->h->stream));
+  WAP_HIP(hipMemsetAsync(a->vbar, 0, a->h->n * sizeof(double), a->h->stream));
+  return WAP_OK;
+}
+
+int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
+  if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
+  if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
+  if (!a->h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  const bool kg = is_kg(a->h);
+  if (is_f64(a->h))
+    return kg ? adjoint_record_t<double, 1>(a, step, t0, slot) : adjoint_record_t<double, 0>(a, step, t0, slot);
+  return kg ? adjoint_record_t<float, 1>(a, step, t0, slot) : adjoint_record_t<float, 0>(a, step, t0, slot);
+}
+
+int wap_adjoint_step(wap_adjoint *a, int slot) {
+  if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
+  if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
+  const bool kg = is_kg(a->h);
+  if (is_f64(a->h)) return kg ? adjoint_step_t<double, 1>(a, slot) : adjoint_step_t<double, 0>(a, slot);
+  return kg ? adjoint_step_t<float, 1>(a, slot) : adjoint_step_t<float, 0>(a, slot);
+}
+
+int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c_im) {
+  if (!a || !g) return fail(WAP_ERR_ARG, "NULL argument");
+  wap_handle *h = a->h;
+  if (k < 0 || k >= h->d.nz) return fail(WAP_ERR_ARG, "plane index out of range");
+  const int n = (int)h->plane;
+  const dim3 grid((n + 255) / 256), block(256);
+  const size_t off = (size_t)k * h->plane;
+  if (is_f64(h))
+    hipLaunchKernelGGL(wap::seed_plane<double>, grid, block, 0, h->stream, (double2 *)a->lam[0],
+                       off, n, (const double2 *)g, c_re, c_im);
+  else
+    hipLaunchKernelGGL(wap::seed_plane<float>, grid, block, 0, h->stream, (float2 *)a->lam[0], off,
+                       n, (const double2 *)g, c_re, c_im);
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
+int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device) {
+  if (!a || !out) return fail(WAP_ERR_ARG, "NULL argument");
+  WAP_HIP(hipMemcpyAsync(out, a->vbar, a->h->n * sizeof(double),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->h->stream));
+  WAP_HIP(hipStreamSynchronize(a->h->stream));
   return WAP_OK;
 }
 

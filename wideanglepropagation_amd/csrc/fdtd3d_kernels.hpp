@@ -78,6 +78,8 @@ struct PassArgs {
   C *out_pi;
   C *cmp_psi;          // Kahan compensation (P4 only)
   C *cmp_pi;
+  C *rec_psi;          // P1: optional copy of the injected state (adjoint recording)
+  C *rec_pi;
   const R *V;          // potential (nz,ny,nx)
   const C *zc;         // per plane k: zc[2k] = z+ weight, zc[2k+1] = z- weight
   const R *gam;        // KG sponge damping per plane
@@ -90,8 +92,9 @@ struct PassArgs {
   R w_acc, w_y;        // stage weights
   int src_k0, src_n;   // source planes src_k0 .. src_k0+src_n-1
   R src_keep[kMaxBand];
-  C src_coef[kMaxBand];     // a_b e^{i k (z-zK)} e^{-i w t}
+  C src_coef[kMaxBand];     // a_b e^{i k (z-zK)}
   C src_coef_pi[kMaxBand];  // (-i w) * src_coef   (KG)
+  C src_ph;                 // e^{-i w t} of this step (single-grid launches)
 };
 
 // Workgroup id -> tile id such that the workgroups of XCD x (id % 8 == x)
@@ -105,8 +108,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
 }
 
 // Source band selection without dynamic indexing of the kernarg arrays.
-// ph multiplies the coefficients: (1, 0) when they already carry e^{-i w t}
-// (single grid), the per-step phasor of the instance in batched launches.
+// ph = e^{-i w t} of the step multiplies the time-free coefficients (the same
+// arithmetic in single-grid and batched launches).
 template <typename R>
 __device__ __forceinline__ bool src_pick(const PassArgs<R> &p, int k, cplx<R> ph, R &keep,
                                          cplx<R> &coef, cplx<R> &coef_pi) {
@@ -232,6 +235,10 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
     }
 
     if (STAGE == 1) {
+      if (p.rec_psi) {
+        p.rec_psi[idx] = b;
+        if (EQ == 1) p.rec_pi[idx] = bq;
+      }
       p.acc_psi[idx] = KAHAN ? cscale<R>(kpsi, p.w_acc) : caxpy<R>(b, p.w_acc, kpsi);
       p.out_psi[idx] = caxpy<R>(b, p.w_y, kpsi);
       if (EQ == 1) {
@@ -276,8 +283,7 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
 
 template <typename R, int EQ, int STAGE, bool KAHAN>
 __global__ __launch_bounds__(512) void rk4_pass(const PassArgs<R> p) {
-  rk4_column<R, EQ, STAGE, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz),
-                                  mkc<R>(R(1), R(0)));
+  rk4_column<R, EQ, STAGE, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
 }
 
 // Many independent grids in one launch.  args[inst] holds instance inst's
@@ -351,6 +357,7 @@ struct KG2Args {
   R src_keep[kMaxBand];
   C src_coef[kMaxBand];
   C src_coef_pi[kMaxBand];
+  C src_ph;
 };
 
 template <typename R>
@@ -549,7 +556,7 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
 
 template <typename R, int PASS, bool KAHAN>
 __global__ __launch_bounds__(512) void kg2_pass(const KG2Args<R> p) {
-  kg2_column<R, PASS, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), mkc<R>(R(1), R(0)));
+  kg2_column<R, PASS, KAHAN>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
 }
 
 template <typename R, int PASS, bool KAHAN>
@@ -560,6 +567,200 @@ __global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__rest
   const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
   const int inst = find_instance(tile_begin, ninst, t);
   kg2_column<R, PASS, KAHAN>(args[inst], t - tile_begin[inst], ph[inst]);
+}
+
+// ------------------------------------------------------------------ adjoint
+//
+// Reverse sweep of one RK4 step u' = b + h/6 (k1 + 2k2 + 2k3 + k4), k_s = M Y_s,
+// Y1 = b = K u + c (soft source), Y2 = b + h/2 k1, Y3 = b + h/2 k2, Y4 = b + h k3.
+// With lam = dL/du' (gradients of real L packed as d/dRe + i d/dIm):
+//   S=4: kb4 = h/6 lam          Yb4 = M^H kb4   acc  = lam + Yb4   kb3 = h/3 lam + h Yb4
+//   S=3:                        Yb3 = M^H kb3   acc += Yb3        kb2 = h/3 lam + h/2 Yb3
+//   S=2:                        Yb2 = M^H kb2   acc += Yb2        kb1 = h/6 lam + h/2 Yb2
+//   S=1:                        Yb1 = M^H kb1   lam <- K (acc + Yb1)
+//   every S: dL/dV += Re(conj(kb_S) dM/dV Y_S) on interior voxels.
+// M has zero rows on the frozen planes, so M^H acts on kb masked to the
+// interior.  M^H of the z operator L_z y_k = cp_k (y_{k+1} - y_k) - cm_k (y_k -
+// y_{k-1}) is (L_z^H x)_j = conj(cp_{j-1}) x_{j-1} - conj(cp_j + cm_j) x_j +
+// conj(cm_{j+1}) x_{j+1}.  Schrodinger: M = -i H, M^H x = i H^H x,
+// dM/dV Y = -i Y.  Klein-Gordon: M^H (x_psi, x_pi) = (A^H x_pi,
+// x_psi + conj(D) x_pi), dM/dV Y = (0, 2 V Y_psi - 2 i Y_pi).
+template <typename R>
+struct AdjArgs {
+  using C = cplx<R>;
+  C *lam_psi, *lam_pi;          // dL/du' in; dL/du out (S=1)
+  const C *kb_in_psi, *kb_in_pi;  // kb_S (S < 4)
+  C *kb_out_psi, *kb_out_pi;     // kb_{S-1} (S > 1)
+  C *acc_psi, *acc_pi;           // partial dL/db
+  const C *y_psi, *y_pi;         // forward stage input Y_S
+  double *vbar;                  // dL/dV accumulator
+  const R *V;
+  const C *zc;
+  const R *gam;
+  int nx, ny, nz;
+  int tile_y, tile_z;
+  int ntx, nty, ntz;
+  R hx, hy, m2, h;
+  int src_k0, src_n;
+  R src_keep[kMaxBand];
+};
+
+template <typename R, int EQ, int S>
+__device__ __forceinline__ void adj_load(const AdjArgs<R> &p, size_t id, int k, cplx<R> &xs,
+                                         cplx<R> &xq) {
+  const cplx<R> zero = mkc<R>(R(0), R(0));
+  if (k == 0 || k == p.nz - 1) {
+    xs = zero;
+    xq = zero;
+    return;
+  }
+  if (S == 4) {
+    xs = cscale<R>(p.lam_psi[id], p.h / R(6));
+    xq = EQ == 1 ? cscale<R>(p.lam_pi[id], p.h / R(6)) : zero;
+  } else {
+    xs = p.kb_in_psi[id];
+    xq = EQ == 1 ? p.kb_in_pi[id] : zero;
+  }
+}
+
+template <typename R, int EQ, int S>
+__global__ __launch_bounds__(512) void adj_pass(const AdjArgs<R> p) {
+  using C = cplx<R>;
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int ty = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int i = tx * kWaveX + threadIdx.x;
+  const int j = ty * p.tile_y + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(p.nz, k0 + p.tile_z);
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const size_t plane = (size_t)nx * ny;
+  const int ij = j * nx + i;
+  const int ip = (i + 1 == nx) ? 0 : i + 1;
+  const int im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1;
+  const int jm = (j == 0) ? ny - 1 : j - 1;
+  const C zero = mkc<R>(R(0), R(0));
+  // the stencil acts on x_psi (Schrodinger) or x_pi (Klein-Gordon)
+  C sm = zero, sc, sp;   // stencil component at k-1, k, k+1
+  C xs_c, xq_c;          // both components at k
+  {
+    C a, b;
+    if (k0 > 0) {
+      adj_load<R, EQ, S>(p, (size_t)(k0 - 1) * plane + ij, k0 - 1, a, b);
+      sm = EQ == 1 ? b : a;
+    }
+    adj_load<R, EQ, S>(p, (size_t)k0 * plane + ij, k0, xs_c, xq_c);
+    sc = EQ == 1 ? xq_c : xs_c;
+  }
+  for (int k = k0; k < k1; ++k) {
+    const size_t base = (size_t)k * plane;
+    const size_t idx = base + ij;
+    C xs_p = zero, xq_p = zero;
+    if (k + 1 < nz) adj_load<R, EQ, S>(p, idx + plane, k + 1, xs_p, xq_p);
+    sp = EQ == 1 ? xq_p : xs_p;
+    const bool interior = (k > 0) && (k < nz - 1);
+    const R v = p.V[idx];
+    // lateral part (the x, y operators are real and symmetric); zero on frozen planes
+    C lat = zero;
+    if (interior) {
+      C a0, b0, a1, b1, a2, b2, a3, b3;
+      adj_load<R, EQ, S>(p, base + j * nx + ip, k, a0, b0);
+      adj_load<R, EQ, S>(p, base + j * nx + im, k, a1, b1);
+      adj_load<R, EQ, S>(p, base + jp * nx + i, k, a2, b2);
+      adj_load<R, EQ, S>(p, base + jm * nx + i, k, a3, b3);
+      const C n0 = EQ == 1 ? b0 : a0, n1 = EQ == 1 ? b1 : a1;
+      const C n2 = EQ == 1 ? b2 : a2, n3 = EQ == 1 ? b3 : a3;
+      lat = cadd<R>(cscale<R>(cadd<R>(csub<R>(n0, sc), csub<R>(n1, sc)), p.hx),
+                    cscale<R>(cadd<R>(csub<R>(n2, sc), csub<R>(n3, sc)), p.hy));
+    }
+    // z part: conj(cp_{k-1}) x_{k-1} - conj(cp_k + cm_k) x_k + conj(cm_{k+1}) x_{k+1}
+    C lz = zero;
+    {
+      const C cpm = k > 0 ? p.zc[2 * (k - 1)] : zero;
+      const C cpk = p.zc[2 * k], cmk = p.zc[2 * k + 1];
+      const C cmp1 = k + 1 < nz ? p.zc[2 * (k + 1) + 1] : zero;
+      const C ccpm = mkc<R>(cpm.x, -cpm.y), cdk = mkc<R>(cpk.x + cmk.x, -(cpk.y + cmk.y));
+      const C ccmp = mkc<R>(cmp1.x, -cmp1.y);
+      lz = cadd<R>(csub<R>(cmul<R>(ccpm, sm), cmul<R>(cdk, sc)), cmul<R>(ccmp, sp));
+    }
+    const C lap = cadd<R>(lat, lz);
+    C yb_s, yb_q = zero;
+    if (EQ == 0) {
+      const C hh = caxpy<R>(lap, v, sc);       // H^H x (x masked)
+      yb_s = mkc<R>(-hh.y, hh.x);              // i H^H x
+    } else {
+      const R g = p.gam[k];
+      const R w = p.m2 - v * v;
+      yb_s = csub<R>(lap, cscale<R>(sc, w));   // A^H x_pi
+      // x_psi + (2 i V - 2 g) x_pi
+      yb_q = mkc<R>(xs_c.x - R(2) * g * xq_c.x - R(2) * v * xq_c.y,
+                    xs_c.y + R(2) * v * xq_c.x - R(2) * g * xq_c.y);
+    }
+    // dL/dV
+    if (interior) {
+      const C ys = p.y_psi[idx];
+      double dv;
+      if (EQ == 0) {
+        // Re(conj(x) (-i Y)) = Im(conj(x) Y)
+        dv = (double)xs_c.x * (double)ys.y - (double)xs_c.y * (double)ys.x;
+      } else {
+        const C yq = p.y_pi[idx];
+        // Re(conj(x_pi) (2 V Y_psi - 2 i Y_pi))
+        const double tr = 2.0 * (double)v * (double)ys.x + 2.0 * (double)yq.y;
+        const double ti = 2.0 * (double)v * (double)ys.y - 2.0 * (double)yq.x;
+        dv = (double)xq_c.x * tr + (double)xq_c.y * ti;
+      }
+      p.vbar[idx] += dv;
+    }
+    const C ls = p.lam_psi[idx];
+    const C lq = EQ == 1 ? p.lam_pi[idx] : zero;
+    if (S == 4) {
+      p.acc_psi[idx] = cadd<R>(ls, yb_s);
+      p.kb_out_psi[idx] = cadd<R>(cscale<R>(ls, p.h / R(3)), cscale<R>(yb_s, p.h));
+      if (EQ == 1) {
+        p.acc_pi[idx] = cadd<R>(lq, yb_q);
+        p.kb_out_pi[idx] = cadd<R>(cscale<R>(lq, p.h / R(3)), cscale<R>(yb_q, p.h));
+      }
+    } else if (S == 3 || S == 2) {
+      const R wl = S == 3 ? p.h / R(3) : p.h / R(6);
+      p.acc_psi[idx] = cadd<R>(p.acc_psi[idx], yb_s);
+      p.kb_out_psi[idx] = cadd<R>(cscale<R>(ls, wl), cscale<R>(yb_s, p.h * R(0.5)));
+      if (EQ == 1) {
+        p.acc_pi[idx] = cadd<R>(p.acc_pi[idx], yb_q);
+        p.kb_out_pi[idx] = cadd<R>(cscale<R>(lq, wl), cscale<R>(yb_q, p.h * R(0.5)));
+      }
+    } else {
+      R keep = R(1);
+      const int bnd = k - p.src_k0;
+      if (bnd >= 0 && bnd < p.src_n) {
+#pragma unroll
+        for (int q = 0; q < kMaxBand; ++q)
+          if (q == bnd) keep = p.src_keep[q];
+      }
+      p.lam_psi[idx] = cscale<R>(cadd<R>(p.acc_psi[idx], yb_s), keep);
+      if (EQ == 1) p.lam_pi[idx] = cscale<R>(cadd<R>(p.acc_pi[idx], yb_q), keep);
+    }
+    sm = sc;
+    sc = sp;
+    xs_c = xs_p;
+    xq_c = xq_p;
+  }
+}
+
+// lam[off + t] += conj(c) g[t]  (an exit-plane seed)
+template <typename R>
+__global__ void seed_plane(cplx<R> *lam, size_t off, int n, const double2 *g, double cr, double ci) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double2 gv = g[t];
+  cplx<R> l = lam[off + t];
+  l.x += (R)(cr * gv.x + ci * gv.y);
+  l.y += (R)(cr * gv.y - ci * gv.x);
+  lam[off + t] = l;
 }
 
 // ---------------------------------------------------------------- utilities
