@@ -20,12 +20,17 @@ def _setup(eq, precision, rng, shape=(22, 10, 12)):
     else:
         p = klein_gordon_params(0.05, ext, shape, pml_thick=0.15, k_source=4, precision=precision)
     V = 300.0 * rng.standard_normal(shape)
+    # an O(1) initial state, so the exit plane is populated from the first step
+    psi0 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    pi0 = -1j * p.e_sim * psi0 if eq == "klein_gordon" else None
+    p.init = (psi0, pi0)
     return p, V
 
 
 def _loss(V, p, nsteps, k_exit, samples, target, C):
     from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
-    E = exit_wave(V, p, nsteps, k_exit, samples=samples, checkpoint_every=C)
+    E = exit_wave(V, p, nsteps, k_exit, samples=samples, checkpoint_every=C, psi0=p.init[0],
+                  pi0=p.init[1])
     I = far_field_intensity(E)
     return torch.sum((I - target) ** 2)
 
@@ -41,7 +46,8 @@ def test_adjoint_matches_finite_differences(eq, rng):
     V = torch.tensor(Vn, dtype=torch.float64, device="cuda", requires_grad=True)
     with torch.no_grad():
         from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
-        E0 = exit_wave(V.detach(), p, nsteps, k_exit, samples=samples)
+        E0 = exit_wave(V.detach(), p, nsteps, k_exit, samples=samples, psi0=p.init[0],
+                       pi0=p.init[1])
         target = 0.7 * far_field_intensity(E0) + 0.1 * far_field_intensity(E0).mean()
     L = _loss(V, p, nsteps, k_exit, samples, target, C)
     L.backward()
@@ -76,7 +82,7 @@ def test_f32_adjoint_close_to_f64(rng):
     for p, dt in ((p64, torch.float64), (p32, torch.float32)):
         V = torch.tensor(Vn, dtype=dt, device="cuda", requires_grad=True)
         from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
-        E = exit_wave(V, p, nsteps, k_exit)
+        E = exit_wave(V, p, nsteps, k_exit, psi0=p64.init[0], pi0=p64.init[1])
         L = torch.sum(far_field_intensity(E) ** 2)
         L.backward()
         grads.append(V.grad.detach().double().cpu().numpy())

@@ -107,42 +107,50 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   return xcd * q + (xcd < rem ? xcd : rem) + pos;
 }
 
-// Source band selection without dynamic indexing of the kernarg arrays.
-// ph = e^{-i w t} of the step multiplies the time-free coefficients (the same
-// arithmetic in single-grid and batched launches).
+// Soft-source coefficients of one plane.  The band is a few planes, so
+// membership is uniform over a plane: it is evaluated once per plane (and
+// the kernarg arrays are selected without dynamic indexing).  ph = e^{-i w t}
+// of the step multiplies the time-free coefficients (the same arithmetic in
+// single-grid and batched launches).
 template <typename R>
-__device__ __forceinline__ bool src_pick(const PassArgs<R> &p, int k, cplx<R> ph, R &keep,
-                                         cplx<R> &coef, cplx<R> &coef_pi) {
+struct SrcPlane {
+  bool band;
+  R keep;
+  cplx<R> c;   // a_b e^{i k (z_k - z_K)} e^{-i w t}
+  cplx<R> cq;  // (-i w) c   (KG pi)
+};
+
+template <typename R, typename A>
+__device__ __forceinline__ SrcPlane<R> src_plane(const A &p, int k, cplx<R> ph) {
+  SrcPlane<R> s;
+  s.band = false;
+  s.keep = R(1);
+  s.c = mkc<R>(R(0), R(0));
+  s.cq = s.c;
   const int b = k - p.src_k0;
-  if (b < 0 || b >= p.src_n) return false;
+  if (b >= 0 && b < p.src_n) {
+    s.band = true;
 #pragma unroll
-  for (int q = 0; q < kMaxBand; ++q) {
-    if (q == b) {
-      keep = p.src_keep[q];
-      coef = p.src_coef[q];
-      coef_pi = p.src_coef_pi[q];
+    for (int q = 0; q < kMaxBand; ++q) {
+      if (q == b) {
+        s.keep = p.src_keep[q];
+        s.c = p.src_coef[q];
+        s.cq = p.src_coef_pi[q];
+      }
     }
+    s.c = cmul<R>(s.c, ph);
+    s.cq = cmul<R>(s.cq, ph);
   }
-  coef = cmul<R>(coef, ph);
-  coef_pi = cmul<R>(coef_pi, ph);
-  return true;
+  return s;
 }
 
-// Load psi component of the stage input; P1 applies the soft source on load.
-template <typename R, int STAGE>
-__device__ __forceinline__ cplx<R> load_in(const PassArgs<R> &p, cplx<R> ph, const cplx<R> *arr,
-                                           size_t idx, int k, int ij) {
-  cplx<R> v = arr[idx];
-  if (STAGE == 1) {
-    R keep;
-    cplx<R> coef, coef_pi;
-    if (src_pick(p, k, ph, keep, coef, coef_pi)) {
-      cplx<R> inc = coef;
-      if (p.prof) inc = cmul<R>(coef, p.prof[ij]);
-      v = caxpy<R>(cscale<R>(v, keep), R(1), inc);
-    }
-  }
-  return v;
+// keep v + c P(ij) on source planes, v elsewhere
+template <typename R>
+__device__ __forceinline__ cplx<R> inj(cplx<R> v, const SrcPlane<R> &s, cplx<R> c,
+                                       const cplx<R> *prof, int ij) {
+  if (!s.band) return v;
+  const cplx<R> inc = prof ? cmul<R>(c, prof[ij]) : c;
+  return caxpy<R>(cscale<R>(v, s.keep), R(1), inc);
 }
 
 // One workgroup's column tile of one stage pass.
@@ -168,41 +176,53 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
   const int jm = (j == 0) ? ny - 1 : j - 1;
   const int o_xp = j * nx + ip, o_xm = j * nx + im;
   const int o_yp = jp * nx + i, o_ym = jm * nx + i;
+  const C *prof = p.prof;
 
   const C zero = mkc<R>(R(0), R(0));
-  C ym = (k0 > 0) ? load_in<R, STAGE>(p, ph, p.yin_psi, (size_t)(k0 - 1) * plane + ij, k0 - 1, ij)
-                   : zero;
-  C yc = load_in<R, STAGE>(p, ph, p.yin_psi, (size_t)k0 * plane + ij, k0, ij);
+  // stage input psi component: P1 applies the soft source on load
+  C ym = zero;
+  if (k0 > 0) {
+    ym = p.yin_psi[(size_t)(k0 - 1) * plane + ij];
+    if (STAGE == 1) {
+      const SrcPlane<R> sm = src_plane<R>(p, k0 - 1, ph);
+      ym = inj<R>(ym, sm, sm.c, prof, ij);
+    }
+  }
+  SrcPlane<R> sk = src_plane<R>(p, k0, ph);
+  C yc = p.yin_psi[(size_t)k0 * plane + ij];
+  if (STAGE == 1) yc = inj<R>(yc, sk, sk.c, prof, ij);
 
   for (int k = k0; k < k1; ++k) {
     const size_t base = (size_t)k * plane;
     const size_t idx = base + ij;
-    C yp = (k + 1 < nz) ? load_in<R, STAGE>(p, ph, p.yin_psi, idx + plane, k + 1, ij) : zero;
+    const SrcPlane<R> sn = src_plane<R>(p, k + 1, ph);
+    C yp = zero;
+    if (k + 1 < nz) {
+      yp = p.yin_psi[idx + plane];
+      if (STAGE == 1) yp = inj<R>(yp, sn, sn.c, prof, ij);
+    }
     const bool interior = (k > 0) && (k < nz - 1);
     const R v = p.V[idx];
 
-    // stage input, pi component (KG), and its injected value on P1
+    // stage input, pi component (KG), injected on P1
     C q = zero;
-    R keep = R(1);
-    C coef = zero, coef_pi = zero;
-    const bool in_band = src_pick(p, k, ph, keep, coef, coef_pi);
-    C inc = coef, inc_pi = coef_pi;
-    if (in_band && p.prof) {
-      const C pr = p.prof[ij];
-      inc = cmul<R>(coef, pr);
-      inc_pi = cmul<R>(coef_pi, pr);
-    }
     if (EQ == 1) {
       q = p.yin_pi[idx];
-      if (STAGE == 1 && in_band) q = caxpy<R>(cscale<R>(q, keep), R(1), inc_pi);
+      if (STAGE == 1) q = inj<R>(q, sk, sk.cq, prof, ij);
     }
 
     C kpsi = zero, kpi = zero;
     if (interior) {
-      const C xp = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_xp, k, o_xp);
-      const C xm = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_xm, k, o_xm);
-      const C yyp = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_yp, k, o_yp);
-      const C yym = load_in<R, STAGE>(p, ph, p.yin_psi, base + o_ym, k, o_ym);
+      C xp = p.yin_psi[base + o_xp];
+      C xm = p.yin_psi[base + o_xm];
+      C yyp = p.yin_psi[base + o_yp];
+      C yym = p.yin_psi[base + o_ym];
+      if (STAGE == 1 && sk.band) {
+        xp = inj<R>(xp, sk, sk.c, prof, o_xp);
+        xm = inj<R>(xm, sk, sk.c, prof, o_xm);
+        yyp = inj<R>(yyp, sk, sk.c, prof, o_yp);
+        yym = inj<R>(yym, sk, sk.c, prof, o_ym);
+      }
       const C wzp = p.zc[2 * k];
       const C wzm = p.zc[2 * k + 1];
       const C lx = cadd<R>(csub<R>(xp, yc), csub<R>(xm, yc));
@@ -226,12 +246,8 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
     // base b = inject(psi_n): P1 has it in yc/q; P2, P3 and Kahan P4 reload it.
     C b = yc, bq = q;
     if (STAGE == 2 || STAGE == 3 || (STAGE == 4 && KAHAN)) {
-      b = p.st_psi[idx];
-      if (in_band) b = caxpy<R>(cscale<R>(b, keep), R(1), inc);
-      if (EQ == 1) {
-        bq = p.st_pi[idx];
-        if (in_band) bq = caxpy<R>(cscale<R>(bq, keep), R(1), inc_pi);
-      }
+      b = inj<R>(p.st_psi[idx], sk, sk.c, prof, ij);
+      if (EQ == 1) bq = inj<R>(p.st_pi[idx], sk, sk.cq, prof, ij);
     }
 
     if (STAGE == 1) {
@@ -259,7 +275,7 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
       } else {
         // true state = hi - comp; the source scales both by keep.
         C c = p.cmp_psi[idx];
-        if (in_band) c = cscale<R>(c, keep);
+        if (sk.band) c = cscale<R>(c, sk.keep);
         const C tot = caxpy<R>(p.acc_psi[idx], p.w_acc, kpsi);
         const C yk = csub<R>(tot, c);
         const C t = cadd<R>(b, yk);
@@ -267,7 +283,7 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
         p.out_psi[idx] = t;
         if (EQ == 1) {
           C cq = p.cmp_pi[idx];
-          if (in_band) cq = cscale<R>(cq, keep);
+          if (sk.band) cq = cscale<R>(cq, sk.keep);
           const C totq = caxpy<R>(p.acc_pi[idx], p.w_acc, kpi);
           const C ykq = csub<R>(totq, cq);
           const C tq = cadd<R>(bq, ykq);
@@ -278,6 +294,7 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
     }
     ym = yc;
     yc = yp;
+    sk = sn;
   }
 }
 
@@ -360,48 +377,6 @@ struct KG2Args {
   C src_ph;
 };
 
-template <typename R>
-__device__ __forceinline__ bool kg2_src(const KG2Args<R> &p, cplx<R> ph, int k, int ij, R &keep,
-                                        cplx<R> &inc, cplx<R> &inc_pi) {
-  const int b = k - p.src_k0;
-  if (b < 0 || b >= p.src_n) return false;
-  cplx<R> c = mkc<R>(R(0), R(0)), cq = c;
-  keep = R(1);
-#pragma unroll
-  for (int t = 0; t < kMaxBand; ++t) {
-    if (t == b) {
-      keep = p.src_keep[t];
-      c = p.src_coef[t];
-      cq = p.src_coef_pi[t];
-    }
-  }
-  c = cmul<R>(c, ph);
-  cq = cmul<R>(cq, ph);
-  if (p.prof) {
-    const cplx<R> pr = p.prof[ij];
-    c = cmul<R>(c, pr);
-    cq = cmul<R>(cq, pr);
-  }
-  inc = c;
-  inc_pi = cq;
-  return true;
-}
-
-// injected state (s, q) at voxel (k, ij)
-template <typename R>
-__device__ __forceinline__ void kg2_state(const KG2Args<R> &p, cplx<R> ph, size_t idx, int k, int ij,
-                                          bool want_s, cplx<R> &s, cplx<R> &q) {
-  R keep;
-  cplx<R> inc, incq;
-  const bool band = kg2_src(p, ph, k, ij, keep, inc, incq);
-  if (want_s) {
-    s = p.st_psi[idx];
-    if (band) s = caxpy<R>(cscale<R>(s, keep), R(1), inc);
-  }
-  q = p.st_pi[idx];
-  if (band) q = caxpy<R>(cscale<R>(q, keep), R(1), incq);
-}
-
 // A y without the diagonal term: weighted 7-point Laplacian (c^2 in the weights)
 template <typename R>
 __device__ __forceinline__ cplx<R> kg2_lap(const KG2Args<R> &p, int k, cplx<R> c, cplx<R> xp,
@@ -443,13 +418,15 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
   const int o_xp = j * nx + ip, o_xm = j * nx + im;
   const int o_yp = jp * nx + i, o_ym = jm * nx + i;
   const C zero = mkc<R>(R(0), R(0));
+  const C *prof = p.prof;
 
-  // stencil field values at (k, off): pass 1 -> (s, q^), pass 2 -> (Y3, Y4)
-  auto fields = [&](int k, int off, C &a, C &b) {
+  // stencil field values at (k, off): pass 1 -> (s, q^) injected, q^ = 0 on the
+  // frozen planes; pass 2 -> (Y3, Y4)
+  auto fields = [&](int k, int off, const SrcPlane<R> &sp, C &a, C &b) {
     const size_t id = (size_t)k * plane + off;
     if (PASS == 1) {
-      kg2_state<R>(p, ph, id, k, off, true, a, b);
-      if (k == 0 || k == nz - 1) b = zero;
+      a = inj<R>(p.st_psi[id], sp, sp.c, prof, off);
+      b = (k == 0 || k == nz - 1) ? zero : inj<R>(p.st_pi[id], sp, sp.cq, prof, off);
     } else {
       a = p.y3[id];
       b = p.y4[id];
@@ -457,29 +434,33 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
   };
 
   C am = zero, bm = zero, ac, bc, ap, bp;
-  if (k0 > 0) fields(k0 - 1, ij, am, bm);
-  fields(k0, ij, ac, bc);
+  if (k0 > 0) {
+    const SrcPlane<R> sm = src_plane<R>(p, k0 - 1, ph);
+    fields(k0 - 1, ij, sm, am, bm);
+  }
+  SrcPlane<R> sk = src_plane<R>(p, k0, ph);
+  fields(k0, ij, sk, ac, bc);
   for (int k = k0; k < k1; ++k) {
     const size_t idx = (size_t)k * plane + ij;
-    if (k + 1 < nz) fields(k + 1, ij, ap, bp);
+    const SrcPlane<R> sn = src_plane<R>(p, k + 1, ph);
+    if (k + 1 < nz) fields(k + 1, ij, sn, ap, bp);
     else ap = bp = zero;
     const bool interior = (k > 0) && (k < nz - 1);
     const R v = p.V[idx];
     const R g = p.gam[k];
     const R w = p.m2 - v * v;
+    C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
+    if (interior) {
+      fields(k, o_xp, sk, ax0, bx0);
+      fields(k, o_xm, sk, ax1, bx1);
+      fields(k, o_yp, sk, ay0, by0);
+      fields(k, o_ym, sk, ay1, by1);
+    }
     if (PASS == 1) {
       const C s = ac;
-      C q = bc;  // q^ == q on interior planes
-      if (!interior) {
-        C dummy;
-        kg2_state<R>(p, ph, idx, k, ij, false, dummy, q);
-      }
+      // q^ == q on interior planes
+      const C q = interior ? bc : inj<R>(p.st_pi[idx], sk, sk.cq, prof, ij);
       if (interior) {
-        C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
-        fields(k, o_xp, ax0, bx0);
-        fields(k, o_xm, ax1, bx1);
-        fields(k, o_yp, ay0, by0);
-        fields(k, o_ym, ay1, by1);
         const C As = csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am), cscale<R>(ac, w));
         const C Aq = csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm), cscale<R>(bc, w));
         const C f = cadd<R>(As, kg2_damp<R>(v, g, q));
@@ -503,15 +484,9 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
         p.qq[idx] = KAHAN ? zero : q;
       }
     } else {
-      C s = zero, q;
-      kg2_state<R>(p, ph, idx, k, ij, KAHAN, s, q);
+      const C q = inj<R>(p.st_pi[idx], sk, sk.cq, prof, ij);
       C dpsi = zero, dpi = zero;  // increments beyond P, Q
       if (interior) {
-        C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
-        fields(k, o_xp, ax0, bx0);
-        fields(k, o_xm, ax1, bx1);
-        fields(k, o_yp, ay0, by0);
-        fields(k, o_ym, ay1, by1);
         const C z3 = p.z3[idx];
         const C k3 = cadd<R>(csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am),
                                      cscale<R>(ac, w)),
@@ -529,13 +504,11 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
         p.out_psi[idx] = cadd<R>(P, dpsi);
         p.out_pi[idx] = cadd<R>(Q, dpi);
       } else {
-        R keep;
-        C inc, incq;
-        const bool band = kg2_src(p, ph, k, ij, keep, inc, incq);
+        const C s = inj<R>(p.st_psi[idx], sk, sk.c, prof, ij);
         C cs = p.cmp_psi[idx], cq = p.cmp_pi[idx];
-        if (band) {
-          cs = cscale<R>(cs, keep);
-          cq = cscale<R>(cq, keep);
+        if (sk.band) {
+          cs = cscale<R>(cs, sk.keep);
+          cq = cscale<R>(cq, sk.keep);
         }
         const C ys = csub<R>(cadd<R>(P, dpsi), cs);
         const C ts = cadd<R>(s, ys);
@@ -551,6 +524,7 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
     bm = bc;
     ac = ap;
     bc = bp;
+    sk = sn;
   }
 }
 
