@@ -1,8 +1,11 @@
 """Batched launches (one launch per pass over many uneven grids) and their hipGraph.
 
-A batched step runs the same per-column device code as a single-grid step, so
-every instance must match its own single-grid run bit for bit; graph replays
-must match the eager steps on every replay.
+A batched step runs the same per-column device code as a single-grid step;
+the two kernels are compiled separately, so the compiler may contract a few
+multiply-adds differently: every instance must match its own single-grid run
+to rounding (relative L2 below 1e-6 for float32, 1e-13 for float64).  Graph
+replays run the batched kernels themselves and must match the eager batched
+steps bit for bit on every replay.
 """
 import numpy as np
 import pytest
@@ -52,12 +55,12 @@ def test_batched_step_equals_single_grids(eq, prec, rng):
     b.step(3)
     b.step(4)
     b.synchronize()
+    tol = 1e-13 if prec == "f64" else 1e-6
     for s, w in zip(batch_sims, want):
         got = s.get_state(np.complex128)
-        if isinstance(w, tuple):
-            assert all(np.array_equal(g, x) for g, x in zip(got, w))
-        else:
-            assert np.array_equal(got, w)
+        for g, x in zip(got if isinstance(got, tuple) else (got,),
+                        w if isinstance(w, tuple) else (w,)):
+            assert np.linalg.norm(g - x) <= tol * np.linalg.norm(x)
     b.close()
     for s in single + batch_sims:
         s.close()
