@@ -167,6 +167,8 @@ def main():
     step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
     norm = sim.norm2()
     sim.close()
+    from wideanglepropagation_amd._lib import stream_bandwidth
+    copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
 
     cpu = None
     if not args.no_cpu:
@@ -201,6 +203,8 @@ def main():
         "detail": {"pass_ms": [round(a, 5) for a in avg],
                    "algorithmic_bytes_per_voxel_step": sum(pb),
                    "step_effective_GBs": round(step_gbs, 1),
+                   "measured_stream_copy_GBs": round(copy_gbs, 1),
+                   "measured_stream_read_GBs": round(read_gbs, 1),
                    "norm2_final": norm},
     }
     print(json.dumps(out), flush=True)

@@ -210,6 +210,11 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
 /* dL/dV as float64 (nz,ny,nx); synchronises. */
 int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device);
 
+/* Measured HBM bandwidth reference (no reference counterpart): a STREAM copy
+ * (read + write) and a read-only sweep of `bytes`-sized device buffers, reps
+ * launches each, timed with HIP events on the null stream; GB/s = 1e9 B/s. */
+int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, double *read_gbs);
+
 #ifdef __cplusplus
 }
 #endif

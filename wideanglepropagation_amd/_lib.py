@@ -78,6 +78,7 @@ SIGNATURES = {
     "wap_adjoint_step": (_c_int, [_vp, _c_int]),
     "wap_adjoint_seed": (_c_int, [_vp, _c_int, _vp, _c_dbl, _c_dbl]),
     "wap_adjoint_get_grad": (_c_int, [_vp, _vp, _c_int]),
+    "wap_stream_bandwidth": (_c_int, [_c_i64, _c_int, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_dbl)]),
 }
 
 _LIB = None
@@ -139,3 +140,11 @@ def make_desc(p):
     d.kg_gamma_max = float(p.kg_gamma_max)
     d.src_alpha = float(p.src_alpha)
     return d
+
+
+def stream_bandwidth(nbytes=2 << 30, reps=10):
+    """(copy GB/s, read GB/s) of STREAM kernels on nbytes-sized buffers."""
+    c, r = ctypes.c_double(), ctypes.c_double()
+    check(lib().wap_stream_bandwidth(int(nbytes), int(reps), ctypes.byref(c), ctypes.byref(r)),
+          "wap_stream_bandwidth")
+    return c.value, r.value

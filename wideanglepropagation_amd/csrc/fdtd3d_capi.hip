@@ -1317,3 +1317,51 @@ int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device) {
 }
 
 }  // extern "C"
+
+extern "C" int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, double *read_gbs) {
+  if (bytes < (1 << 20) || reps < 1 || !copy_gbs || !read_gbs)
+    return fail(WAP_ERR_ARG, "need bytes >= 1 MiB, reps >= 1 and output pointers");
+  const size_t n = (size_t)bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  float *sink = nullptr;
+  WAP_HIP(hipMalloc((void **)&a, n * 16));
+  hipError_t e = hipMalloc((void **)&b, n * 16);
+  if (e == hipSuccess) e = hipMalloc((void **)&sink, 16);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return fail(WAP_ERR_OOM, std::string("wap_stream_bandwidth: ") + hipGetErrorString(e));
+  }
+  int rc = WAP_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  do {
+    if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+      rc = fail(WAP_ERR_HIP, "wap_stream_bandwidth setup failed");
+      break;
+    }
+    const dim3 grid(256 * 8 * 8), block(256);
+    float ms = 0.f;
+    hipLaunchKernelGGL(wap::stream_copy, grid, block, 0, nullptr, a, b, n);  // warm-up
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(wap::stream_copy, grid, block, 0, nullptr, a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *copy_gbs = 2.0 * (double)n * 16 * reps / (ms * 1e-3) / 1e9;
+    hipLaunchKernelGGL(wap::stream_read, grid, block, 0, nullptr, a, n, sink);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(wap::stream_read, grid, block, 0, nullptr, a, n, sink);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *read_gbs = (double)n * 16 * reps / (ms * 1e-3) / 1e9;
+    if (hipGetLastError() != hipSuccess) rc = fail(WAP_ERR_HIP, "stream kernels failed");
+  } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(sink);
+  return rc;
+}

@@ -834,4 +834,24 @@ __global__ __launch_bounds__(256) void sum_partials(const double *partial, int n
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// STREAM-style copy / read of n 16-byte words (bandwidth reference for the roofline)
+__global__ __launch_bounds__(256) void stream_copy(const float4 *__restrict__ a,
+                                                   float4 *__restrict__ b, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) b[t] = a[t];
+}
+
+__global__ __launch_bounds__(256) void stream_read(const float4 *__restrict__ a, size_t n,
+                                                   float *__restrict__ sink) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  float acc = 0.f;
+  for (; t < n; t += stride) {
+    const float4 v = a[t];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  if (acc == 12345.678f) sink[0] = acc;  // keeps the loads; never true for zero input
+}
+
 }  // namespace wap
