@@ -35,13 +35,14 @@ PASS_BYTES = {
     ("kg_2pass", "klein_gordon", "f32"): (60, 68),
     ("kg_2pass", "klein_gordon", "f32_kahan"): (60, 108),
     ("kg_2pass", "klein_gordon", "f64"): (120, 136),
+    # both stencil sweeps in one pass
+    ("kg_fused", "klein_gordon", "f32"): (36,),
+    ("kg_fused", "klein_gordon", "f64"): (72,),
 }
 
 
 def schedule_of(p):
-    if p.equation == "klein_gordon" and p.schedule != "stages":
-        return "kg_2pass"
-    return "stages"
+    return p.resolved_schedule()
 
 
 def parse():
@@ -56,7 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
-    ap.add_argument("--schedule", default="auto", choices=["auto", "stages", "kg_2pass"])
+    ap.add_argument("--schedule", default="auto",
+                    choices=["auto", "stages", "kg_2pass", "kg_fused"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration)")
     return ap.parse_args()
@@ -195,8 +197,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": (f"kg2_pass pass {dom + 1}" if sched == "kg_2pass"
-                                else f"rk4_pass stage {dom + 1}"),
+                     "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
+                                "kg_fused": "kg1_pass"}.get(sched, f"rk4_pass stage {dom + 1}"),
                      "bytes_per_launch": dom_bytes,
                      "mean_launch_ms": round(avg[dom], 5)},
         "cpu_baseline": cpu,

@@ -70,7 +70,7 @@ def _run_both(p, V, st0, nsteps, profile=None):
 
 
 @pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
-                                       ("klein_gordon", "stages")])
+                                       ("klein_gordon", "stages"), ("klein_gordon", "kg_fused")])
 def test_f64_matches_oracle(eq, sched, rng):
     print(f"case f64 {eq} {sched} 16x16x24", flush=True)
     p = _params((24, 16, 16), eq, schedule=sched)
@@ -79,22 +79,28 @@ def test_f64_matches_oracle(eq, sched, rng):
     assert rel_l2(got, want) < 1e-12
 
 
-@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+@pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
+                                       ("klein_gordon", "kg_fused")])
 @pytest.mark.parametrize("shape,ty,tz", [((9, 5, 70), 1, 1), ((9, 5, 70), 2, 3),
-                                          ((11, 13, 130), 8, 100), ((7, 3, 2), 4, 0)])
-def test_f64_ragged_shapes_and_tiles(shape, ty, tz, eq, rng):
-    print(f"case ragged {eq} {shape} ty={ty} tz={tz}", flush=True)
-    p = _params(shape, eq, tile_y=ty, tile_z=tz, pml_thick=0.1, k_source=2)
+                                          ((11, 13, 130), 8, 100), ((7, 3, 2), 4, 0),
+                                          ((12, 40, 125), 16, 5), ((10, 3, 61), 4, 2)])
+def test_f64_ragged_shapes_and_tiles(shape, ty, tz, eq, sched, rng):
+    print(f"case ragged {eq} {sched} {shape} ty={ty} tz={tz}", flush=True)
+    if ty == 16 and sched != "kg_fused":
+        ty = 8
+    p = _params(shape, eq, tile_y=ty, tile_z=tz, pml_thick=0.1, k_source=2, schedule=sched)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 6)
     assert rel_l2(got, want) < 1e-12
 
 
 @pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
-                                       ("klein_gordon", "stages")])
+                                       ("klein_gordon", "stages"), ("klein_gordon", "kg_fused")])
 @pytest.mark.parametrize("prec", ["f32", "f32_kahan"])
 def test_f32_matches_oracle(eq, sched, prec, rng):
     print(f"case {prec} {eq} {sched}", flush=True)
+    if sched == "kg_fused" and prec == "f32_kahan":
+        pytest.skip("the fused schedule runs f32 and f64")
     p = _params((24, 16, 16), eq, precision=prec, schedule=sched)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 20)
@@ -120,17 +126,21 @@ def test_schrodinger_gaussian_64_f64_200_steps():
     assert rel_l2(got, want) < 1e-11
 
 
-def test_f32_is_bitwise_deterministic(rng):
+@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+def test_f32_is_bitwise_deterministic(eq, rng):
     FDTD3D = _gpu()
-    p = _params((40, 48, 96), precision="f32")
+    p = _params((40, 48, 96), eq, precision="f32")
     V, st0 = _random_case(p, rng)
     outs = []
     for _ in range(2):
-        sim = FDTD3D(p, V=V.astype(np.float32), psi0=st0)
+        sim = FDTD3D(p, V=V.astype(np.float32))
+        sim.set_state(*(st0 if isinstance(st0, tuple) else (st0,)))
         sim.step(10)
         outs.append(sim.get_state(np.complex64))
         sim.close()
-    assert np.array_equal(outs[0], outs[1])
+    a, b = outs
+    for x, y in zip(a if isinstance(a, tuple) else (a,), b if isinstance(b, tuple) else (b,)):
+        assert np.array_equal(x, y)
 
 
 def test_norm_plane_and_accumulator(rng):
@@ -161,7 +171,9 @@ def test_errors_are_reported_not_fatal():
         FDTD3D(p.with_(nz=2))
     with pytest.raises(WapError, match="source plane"):
         FDTD3D(p.with_(src_k=99))
-    with pytest.raises(WapError, match="2-pass schedule"):
+    with pytest.raises(WapError, match="for the Klein-Gordon equation"):
         FDTD3D(p.with_(schedule="kg_2pass"))
+    with pytest.raises(WapError, match="for the Klein-Gordon equation"):
+        FDTD3D(p.with_(schedule="kg_fused"))
     with pytest.raises(ValueError):
         FDTD3D(p, V=np.zeros(7))

@@ -30,8 +30,7 @@ def _cache_path():
 
 
 def _key(p):
-    sched = "kg_2pass" if (p.equation == "klein_gordon" and p.schedule != "stages") else "stages"
-    return f"{p.equation}/{p.precision}/{sched}/{p.nz}x{p.ny}x{p.nx}"
+    return f"{p.equation}/{p.precision}/{p.resolved_schedule()}/{p.nz}x{p.ny}x{p.nx}"
 
 
 def load_cache():
@@ -72,6 +71,8 @@ def tune(params, tile_ys=TILE_Y, tile_zs=TILE_Z, steps=5, warmup=2, use_cache=Tr
          verbose=False):
     """Return (tile_y, tile_z, ms_per_step, table) of the fastest candidate."""
     key = _key(params)
+    if params.resolved_schedule() == "kg_fused" and tile_ys == TILE_Y:
+        tile_ys = (4, 8, 16)   # the fused pass needs >= 3 rows (one output row + halo)
     cache = load_cache() if use_cache else {}
     if use_cache and key in cache:
         e = cache[key]

@@ -27,8 +27,7 @@ class BatchedFDTD:
         groups = {}
         for s in self.sims:
             p = s.params
-            sched = "kg_2pass" if (p.equation == "klein_gordon" and p.schedule != "stages") \
-                else "stages"
+            sched = "stages" if p.resolved_schedule() == "stages" else "kg_2pass"
             groups.setdefault((p.equation, p.precision, sched), []).append(s)
         self.groups = []
         self.stream = torch.cuda.current_stream()

@@ -318,8 +318,64 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
   return WAP_OK;
 }
 
+// the schedule a handle runs (WAP_SCHED_STAGES, _KG_2PASS or _KG_FUSED)
+int sched_of(const wap_handle *h) {
+  if (!is_kg(h)) return WAP_SCHED_STAGES;
+  if (h->d.schedule == WAP_SCHED_AUTO)
+    return is_kahan(h) ? WAP_SCHED_KG_2PASS : WAP_SCHED_KG_FUSED;
+  return h->d.schedule;
+}
+
 bool kg_two_pass(const wap_handle *h) {
-  return is_kg(h) && h->d.schedule != WAP_SCHED_STAGES;
+  const int s = sched_of(h);
+  return s == WAP_SCHED_KG_2PASS || s == WAP_SCHED_KG_FUSED;
+}
+
+int kg1_rows(const wap_handle *h) {
+  const int ty = h->d.tile_y;
+  return (ty == 4 || ty == 8 || ty == 16) ? ty : 8;
+}
+
+template <typename R>
+int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                  double *pass_ms) {
+  using C = wap::cplx<R>;
+  const int rows = kg1_rows(h);
+  const int ntx = (h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2);
+  const int nty = (h->d.ny + rows - 3) / (rows - 2);
+  const dim3 block(wap::kWaveX, rows);
+  const dim3 grid((unsigned)(ntx * nty * h->ntz));
+  wap::KG1Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.gam = (const R *)h->gam;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = h->tile_z;
+  a.ntx = ntx; a.nty = nty; a.ntz = h->ntz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy; a.m2 = (R)h->m2; a.h = (R)h->d.dt;
+  for (int64_t n = 0; n < nsteps; ++n) {
+    fill_source(h, step_time(h, step0, n, t0), a);
+    a.in_psi = (const C *)h->psi;
+    a.in_pi = (const C *)h->pi;
+    a.out_psi = (C *)h->ya_psi;
+    a.out_pi = (C *)h->ya_pi;
+    if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
+    hipLaunchKernelGGL((wap::kg1_pass<R>), grid, block, 0, h->stream, a);
+    // the new state is in the other buffer
+    std::swap(h->psi, h->ya_psi);
+    std::swap(h->pi, h->ya_pi);
+    if (ev) {
+      WAP_HIP(hipEventRecord(ev[1], h->stream));
+      WAP_HIP(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      WAP_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      pass_ms[0] += ms;
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
 }
 
 template <typename R>
@@ -390,6 +446,10 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
+  if (sched_of(h) == WAP_SCHED_KG_FUSED) {
+    if (h->d.precision == WAP_PREC_F64) return run_steps_kg1<double>(h, nsteps, step0, t0, ev, pass_ms);
+    return run_steps_kg1<float>(h, nsteps, step0, t0, ev, pass_ms);
+  }
   if (kg_two_pass(h)) {
     switch (h->d.precision) {
       case WAP_PREC_F64: return run_steps_kg2<double, false>(h, nsteps, step0, t0, ev, pass_ms);
@@ -453,13 +513,16 @@ int validate(const wap_desc *d) {
     if (d->src_k < 0 || d->src_k >= d->nz)
       return fail(WAP_ERR_ARG, "source plane k_source must lie inside the grid");
   }
-  if (d->tile_y < 0 || d->tile_y > 8 || (d->tile_y & (d->tile_y - 1)))
-    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8");
+  if (d->tile_y < 0 || d->tile_y > 16 || (d->tile_y & (d->tile_y - 1)))
+    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8, 16");
   if (d->tile_z < 0) return fail(WAP_ERR_ARG, "tile_z must be >= 0");
-  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_KG_2PASS)
+  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_KG_FUSED)
     return fail(WAP_ERR_ARG, "unknown schedule");
-  if (d->schedule == WAP_SCHED_KG_2PASS && d->equation != WAP_EQ_KLEIN_GORDON)
-    return fail(WAP_ERR_ARG, "the 2-pass schedule is for the Klein-Gordon equation");
+  if ((d->schedule == WAP_SCHED_KG_2PASS || d->schedule == WAP_SCHED_KG_FUSED) &&
+      d->equation != WAP_EQ_KLEIN_GORDON)
+    return fail(WAP_ERR_ARG, "the 2-pass and fused schedules are for the Klein-Gordon equation");
+  if (d->schedule == WAP_SCHED_KG_FUSED && d->precision == WAP_PREC_F32_KAHAN)
+    return fail(WAP_ERR_ARG, "the fused schedule runs f32 and f64 (f32_kahan uses 2 passes)");
   return WAP_OK;
 }
 
@@ -821,6 +884,15 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (sched_of(h) == WAP_SCHED_KG_FUSED) {
+    const int rows = kg1_rows(h);
+    if (tile_y) *tile_y = rows;
+    if (tile_z) *tile_z = h->tile_z;
+    if (groups)
+      *groups = (int64_t)((h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2)) *
+                ((h->d.ny + rows - 3) / (rows - 2)) * h->ntz;
+    return WAP_OK;
+  }
   if (tile_y) *tile_y = h->tile_y;
   if (tile_z) *tile_z = h->tile_z;
   if (groups) *groups = (int64_t)h->ntx * h->nty * h->ntz;
@@ -846,6 +918,7 @@ struct wap_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
+  std::vector<void *> baked_psi;  // state pointers the uploaded arguments refer to
 };
 
 namespace {
@@ -870,6 +943,36 @@ void timeless_source(const wap_handle *h, A &a) {
   fill_source(h, 0.0, a);
 }
 
+// (Re)write the per-instance arguments into the batch's device arrays (the
+// arrays themselves never move: a captured graph keeps pointing at them).
+template <typename R>
+int batch_upload(wap_batch *b) {
+  const int n = (int)b->hs.size();
+  if (b->kg2) {
+    std::vector<wap::KG2Args<R>> a(n);
+    for (int i = 0; i < n; ++i) {
+      a[i] = kg2_args<R>(b->hs[i], b->tile_y);
+      timeless_source(b->hs[i], a[i]);
+    }
+    WAP_HIP(hipStreamSynchronize(b->stream));
+    WAP_HIP(hipMemcpy(b->d_args[0], a.data(), n * sizeof(wap::KG2Args<R>), hipMemcpyHostToDevice));
+  } else {
+    WAP_HIP(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < 4; ++s) {
+      std::vector<wap::PassArgs<R>> a(n);
+      for (int i = 0; i < n; ++i) {
+        a[i] = stage_args<R>(b->hs[i], s + 1, b->tile_y);
+        timeless_source(b->hs[i], a[i]);
+      }
+      WAP_HIP(hipMemcpy(b->d_args[s], a.data(), n * sizeof(wap::PassArgs<R>),
+                        hipMemcpyHostToDevice));
+    }
+  }
+  b->baked_psi.resize(n);
+  for (int i = 0; i < n; ++i) b->baked_psi[i] = b->hs[i]->psi;
+  return WAP_OK;
+}
+
 template <typename R>
 int batch_build(wap_batch *b) {
   const int n = (int)b->hs.size();
@@ -882,26 +985,16 @@ int batch_build(wap_batch *b) {
   b->total_tiles = begin[n];
   WAP_HIP(hipMalloc((void **)&b->d_tile_begin, (n + 1) * sizeof(int)));
   WAP_HIP(hipMemcpy(b->d_tile_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-  if (b->kg2) {
-    std::vector<wap::KG2Args<R>> a(n);
-    for (int i = 0; i < n; ++i) {
-      a[i] = kg2_args<R>(b->hs[i], b->tile_y);
-      timeless_source(b->hs[i], a[i]);
-    }
-    WAP_HIP(hipMalloc(&b->d_args[0], n * sizeof(wap::KG2Args<R>)));
-    WAP_HIP(hipMemcpy(b->d_args[0], a.data(), n * sizeof(wap::KG2Args<R>), hipMemcpyHostToDevice));
-  } else {
-    for (int s = 0; s < 4; ++s) {
-      std::vector<wap::PassArgs<R>> a(n);
-      for (int i = 0; i < n; ++i) {
-        a[i] = stage_args<R>(b->hs[i], s + 1, b->tile_y);
-        timeless_source(b->hs[i], a[i]);
-      }
-      WAP_HIP(hipMalloc(&b->d_args[s], n * sizeof(wap::PassArgs<R>)));
-      WAP_HIP(hipMemcpy(b->d_args[s], a.data(), n * sizeof(wap::PassArgs<R>),
-                        hipMemcpyHostToDevice));
-    }
-  }
+  const size_t one = b->kg2 ? sizeof(wap::KG2Args<R>) : sizeof(wap::PassArgs<R>);
+  for (int s = 0; s < (b->kg2 ? 1 : 4); ++s) WAP_HIP(hipMalloc(&b->d_args[s], n * one));
+  return batch_upload<R>(b);
+}
+
+// single-grid fused steps swap a handle's state buffers: re-point the arguments
+int batch_refresh(wap_batch *b) {
+  for (size_t i = 0; i < b->hs.size(); ++i)
+    if (b->hs[i]->psi != b->baked_psi[i])
+      return b->precision == WAP_PREC_F64 ? batch_upload<double>(b) : batch_upload<float>(b);
   return WAP_OK;
 }
 
@@ -1043,7 +1136,9 @@ int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   if (nsteps < 0) return fail(WAP_ERR_ARG, "nsteps < 0");
   if (nsteps == 0) return WAP_OK;
-  int rc = batch_phase_dispatch(b, nsteps, step0);
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  rc = batch_phase_dispatch(b, nsteps, step0);
   if (rc) return rc;
   return batch_dispatch(b, nsteps);
 }
@@ -1059,7 +1154,9 @@ int wap_batch_capture(wap_batch *b, int64_t nsteps) {
     b->own_stream = true;
   }
   // allocate the phase table before capture; its contents are written per replay
-  int rc = batch_phase_dispatch(b, nsteps, 0);
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  rc = batch_phase_dispatch(b, nsteps, 0);
   if (rc) return rc;
   WAP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   rc = batch_dispatch(b, nsteps);
@@ -1079,7 +1176,9 @@ int wap_batch_capture(wap_batch *b, int64_t nsteps) {
 int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   if (!b->exec) return fail(WAP_ERR_STATE, "no captured graph (wap_batch_capture)");
-  int rc = batch_phase_dispatch(b, b->graph_steps, step0);
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  rc = batch_phase_dispatch(b, b->graph_steps, step0);
   if (rc) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms) {

@@ -543,6 +543,210 @@ __global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__rest
   kg2_column<R, PASS, KAHAN>(args[inst], t - tile_begin[inst], ph[inst]);
 }
 
+// ------------------------------------------- Klein-Gordon in 1 fused pass
+//
+// The two stencil sweeps of the 2-pass schedule run in one pass over the
+// grid: each workgroup computes the pass-1 quantities (Y3, Y4, Z3, P, Q) of
+// one plane for its 64 x TY columns, stages Y3 / Y4 in LDS, and applies the
+// pass-2 stencil to the plane below it from LDS (lateral neighbours) and
+// registers (the +-z neighbours of its own column).  Workgroup tiles overlap
+// by one column / row on each side: the border lanes / rows compute pass-1
+// values for the halo and write nothing, the inner (64 - 2) x (TY - 2)
+// columns write psi', pi'.  The state ping-pongs between two buffers (a tile
+// still reads the old state of its neighbours' columns).  HBM bytes per voxel
+// (complex64, float32 V): read psi, pi, V (20) + write psi', pi' (16) = 36 B,
+// plus the halo columns / planes re-read through L2.
+constexpr int kKG1MaxRows = 16;
+
+template <typename R>
+struct KG1Args {
+  using C = cplx<R>;
+  const C *in_psi, *in_pi;  // state n
+  C *out_psi, *out_pi;      // state n + 1 (another buffer)
+  const R *V;
+  const C *zc;
+  const R *gam;
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  R hx, hy, m2, h;
+  int src_k0, src_n;
+  R src_keep[kMaxBand];
+  C src_coef[kMaxBand];
+  C src_coef_pi[kMaxBand];
+  C src_ph;
+};
+
+template <typename R>
+struct KG1Plane {  // pass-1 results of one voxel
+  cplx<R> y3, y4, z3, pp, qq;
+};
+
+template <typename R>
+__global__ __launch_bounds__(1024) void kg1_pass(const KG1Args<R> p) {
+  using C = cplx<R>;
+  __shared__ C l3[kKG1MaxRows * kWaveX];
+  __shared__ C l4[kKG1MaxRows * kWaveX];
+  const int TY = blockDim.y;
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  // unwrapped column of this lane; the tile's outputs are lanes 1..62, rows 1..TY-2
+  const int gi = tx * (kWaveX - 2) - 1 + lx;
+  const int gj = tyi * (TY - 2) - 1 + ly;
+  const int i = ((gi % nx) + nx) % nx;
+  const int j = ((gj % ny) + ny) % ny;
+  const bool writer = lx >= 1 && lx <= kWaveX - 2 && ly >= 1 && ly <= TY - 2 && gi < nx && gj < ny;
+  const size_t plane = (size_t)nx * ny;
+  const int ij = j * nx + i;
+  const int ip = (i + 1 == nx) ? 0 : i + 1;
+  const int im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1;
+  const int jm = (j == 0) ? ny - 1 : j - 1;
+  const int o_xp = j * nx + ip, o_xm = j * nx + im;
+  const int o_yp = jp * nx + i, o_ym = jm * nx + i;
+  const int sl = ly * kWaveX + lx;
+  const C zero = mkc<R>(R(0), R(0));
+  const C ph = p.src_ph;
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+
+  // injected state at (m, off)
+  auto state = [&](int m, int off, const SrcPlane<R> &sp, C &s, C &q) {
+    const size_t id = (size_t)m * plane + off;
+    s = inj<R>(p.in_psi[id], sp, sp.c, prof, off);
+    q = inj<R>(p.in_pi[id], sp, sp.cq, prof, off);
+  };
+  // pass-1 quantities at plane m from the column queue (s, q at m-1, m, m+1)
+  auto pass1 = [&](int m, const SrcPlane<R> &sp, C sm_, C qm_, C sc_, C qc_, C sp_, C qp_,
+                   R v) -> KG1Plane<R> {
+    KG1Plane<R> o;
+    if (m == 0 || m == nz - 1) {
+      o.y3 = sc_;
+      o.y4 = sc_;
+      o.z3 = qc_;
+      o.pp = sc_;
+      o.qq = qc_;
+      return o;
+    }
+    C ax0, ax1, ay0, ay1, bx0, bx1, by0, by1;
+    state(m, o_xp, sp, ax0, bx0);
+    state(m, o_xm, sp, ax1, bx1);
+    state(m, o_yp, sp, ay0, by0);
+    state(m, o_ym, sp, ay1, by1);
+    const C qhm = (m - 1 == 0) ? zero : qm_;        // q^ : 0 on the frozen planes
+    const C qhp = (m + 1 == nz - 1) ? zero : qp_;
+    const R g = p.gam[m];
+    const R w = p.m2 - v * v;
+    const C wzp = p.zc[2 * m], wzm = p.zc[2 * m + 1];
+    auto lap = [&](C c, C xp, C xm, C yp, C ym, C zp, C zm) {
+      const C lxx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
+      const C lyy = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
+      const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
+      return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
+    };
+    const C As = csub<R>(lap(sc_, ax0, ax1, ay0, ay1, sp_, sm_), cscale<R>(sc_, w));
+    const C Aq = csub<R>(lap(qc_, bx0, bx1, by0, by1, qhp, qhm), cscale<R>(qc_, w));
+    const C f = cadd<R>(As, kg2_damp<R>(v, g, qc_));
+    const C u = caxpy<R>(qc_, p.h * R(0.5), f);
+    const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
+    o.y3 = caxpy<R>(sc_, p.h * R(0.5), u);
+    o.z3 = caxpy<R>(qc_, p.h * R(0.5), gg);
+    o.y4 = caxpy<R>(sc_, p.h, o.z3);
+    o.pp = cadd<R>(sc_, cscale<R>(cadd<R>(qc_, cscale<R>(cadd<R>(u, o.z3), R(2))), p.h / R(6)));
+    o.qq = cadd<R>(qc_, cscale<R>(caxpy<R>(f, R(2), gg), p.h / R(6)));
+    return o;
+  };
+
+  // column queue of the injected state: planes a = m-1, b = m, c = m+1
+  C sa = zero, qa = zero, sb = zero, qb = zero, sc = zero, qc = zero;
+  SrcPlane<R> spb = src_plane<R>(p, k0 - 1, ph);
+  if (k0 - 2 >= 0) {
+    const SrcPlane<R> t = src_plane<R>(p, k0 - 2, ph);
+    state(k0 - 2, ij, t, sa, qa);
+  }
+  if (k0 - 1 >= 0) state(k0 - 1, ij, spb, sb, qb);
+  SrcPlane<R> spc = src_plane<R>(p, k0, ph);
+  state(k0, ij, spc, sc, qc);
+  // pass-1 at k0-1 (only its Y3 / Y4 are used: the -z neighbour of plane k0)
+  C y3m = zero, y4m = zero;
+  if (k0 - 1 >= 0) {
+    const KG1Plane<R> o = pass1(k0 - 1, spb, sa, qa, sb, qb, sc, qc, p.V[(size_t)(k0 - 1) * plane + ij]);
+    y3m = o.y3;
+    y4m = o.y4;
+  }
+  // pass-1 at k0
+  sa = sb; qa = qb; sb = sc; qb = qc;
+  spb = spc;
+  spc = src_plane<R>(p, k0 + 1, ph);
+  sc = zero; qc = zero;
+  if (k0 + 1 < nz) state(k0 + 1, ij, spc, sc, qc);
+  R vk = p.V[(size_t)k0 * plane + ij];
+  KG1Plane<R> cur = pass1(k0, spb, sa, qa, sb, qb, sc, qc, vk);
+  C qk = qb;  // injected pi at k0
+  l3[sl] = cur.y3;
+  l4[sl] = cur.y4;
+  __syncthreads();
+
+  for (int k = k0; k < k1; ++k) {
+    // pass-1 at k+1: needs the state at k+2
+    sa = sb; qa = qb; sb = sc; qb = qc;
+    spb = spc;
+    spc = src_plane<R>(p, k + 2, ph);
+    sc = zero; qc = zero;
+    if (k + 2 < nz) state(k + 2, ij, spc, sc, qc);
+    KG1Plane<R> nxt;
+    R vn = R(0);
+    if (k + 1 < nz) {
+      vn = p.V[(size_t)(k + 1) * plane + ij];
+      nxt = pass1(k + 1, spb, sa, qa, sb, qb, sc, qc, vn);
+    } else {
+      nxt.y3 = nxt.y4 = nxt.z3 = nxt.pp = nxt.qq = zero;
+    }
+    const C qn = qb;  // injected pi at k+1
+    // pass-2 at k
+    if (writer) {
+      const size_t idx = (size_t)k * plane + ij;
+      C opsi = cur.pp, opi = cur.qq;
+      if (k > 0 && k < nz - 1) {
+        const R g = p.gam[k];
+        const R w = p.m2 - vk * vk;
+        const C wzp = p.zc[2 * k], wzm = p.zc[2 * k + 1];
+        auto lap = [&](const C *L, C c, C zp, C zm) {
+          const C lxx = cadd<R>(csub<R>(L[sl + 1], c), csub<R>(L[sl - 1], c));
+          const C lyy = cadd<R>(csub<R>(L[sl + kWaveX], c), csub<R>(L[sl - kWaveX], c));
+          const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
+          return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
+        };
+        const C k3 = cadd<R>(csub<R>(lap(l3, cur.y3, nxt.y3, y3m), cscale<R>(cur.y3, w)),
+                             kg2_damp<R>(vk, g, cur.z3));
+        const C wv = caxpy<R>(qk, p.h, k3);
+        const C k4 = cadd<R>(csub<R>(lap(l4, cur.y4, nxt.y4, y4m), cscale<R>(cur.y4, w)),
+                             kg2_damp<R>(vk, g, wv));
+        opsi = cadd<R>(cur.pp, cscale<R>(wv, p.h / R(6)));
+        opi = cadd<R>(cur.qq, cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6)));
+      }
+      p.out_psi[idx] = opsi;
+      p.out_pi[idx] = opi;
+    }
+    __syncthreads();  // every lane has read plane k from LDS
+    l3[sl] = nxt.y3;
+    l4[sl] = nxt.y4;
+    __syncthreads();
+    y3m = cur.y3;
+    y4m = cur.y4;
+    cur = nxt;
+    vk = vn;
+    qk = qn;
+  }
+}
+
 // ------------------------------------------------------------------ adjoint
 //
 // Reverse sweep of one RK4 step u' = b + h/6 (k1 + 2k2 + 2k3 + k4), k_s = M Y_s,

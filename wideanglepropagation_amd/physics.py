@@ -98,15 +98,15 @@ class FDTDParams:
     src_alpha: float = SRC_ALPHA
     tile_y: int = 0
     tile_z: int = 0
-    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass"
+    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused"
 
     def __post_init__(self):
         if self.equation not in EQUATIONS:
             raise ValueError(f"equation must be one of {EQUATIONS}")
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
-        if self.schedule not in ("auto", "stages", "kg_2pass"):
-            raise ValueError("schedule must be 'auto', 'stages' or 'kg_2pass'")
+        if self.schedule not in ("auto", "stages", "kg_2pass", "kg_fused"):
+            raise ValueError("schedule must be 'auto', 'stages', 'kg_2pass' or 'kg_fused'")
         if self.time_mode not in ("exact", "f32"):
             raise ValueError("time_mode must be 'exact' or 'f32'")
         self.extent = tuple(float(v) for v in self.extent)
@@ -129,6 +129,14 @@ class FDTDParams:
     def z_coords(self):
         z0, z1 = self.extent[4], self.extent[5]
         return z0 + np.arange(self.nz) * (z1 - z0) / (self.nz - 1.0)
+
+    def resolved_schedule(self):
+        """The pass schedule the library runs for these parameters."""
+        if self.equation != "klein_gordon":
+            return "stages"
+        if self.schedule == "auto":
+            return "kg_2pass" if self.precision == "f32_kahan" else "kg_fused"
+        return self.schedule
 
     def with_(self, **kw):
         return replace(self, **kw)
