@@ -85,8 +85,9 @@ class ExitWave(torch.autograd.Function):
         E = torch.zeros((p.ny, p.nx), dtype=torch.complex128, device="cuda")
         tmp = torch.zeros_like(E)
         ckpt = {}
+        keep = bool(ctx.needs_input_grad[0])   # no checkpoints when no gradient is wanted
         for n in range(nsteps):
-            if n % C == 0:
+            if keep and n % C == 0:
                 ckpt[n] = sim.get_state(np.complex128 if p.precision == "f64" else np.complex64,
                                         device=True)
             sim.step(1)
@@ -95,6 +96,11 @@ class ExitWave(torch.autograd.Function):
                 tmp.zero_()
                 sim.accumulate_plane(k_exit, p.e_sim * (sim.t0 + s * p.dt), tmp)
                 E += samples[s] * tmp
+        if not keep:
+            sim.close()
+            ctx.sim = None
+            ctx.ckpt = None
+            return E
         ctx.sim = sim
         ctx.ckpt = ckpt
         ctx.opts = dict(opts, samples=samples, checkpoint_every=C)

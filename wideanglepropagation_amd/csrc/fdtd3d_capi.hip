@@ -321,8 +321,8 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
 // the schedule a handle runs (WAP_SCHED_STAGES, _KG_2PASS or _KG_FUSED)
 int sched_of(const wap_handle *h) {
   if (!is_kg(h)) return WAP_SCHED_STAGES;
-  if (h->d.schedule == WAP_SCHED_AUTO)
-    return is_kahan(h) ? WAP_SCHED_KG_2PASS : WAP_SCHED_KG_FUSED;
+  if (h->d.schedule == WAP_SCHED_AUTO)  // the fused pass spills registers in float64
+    return h->d.precision == WAP_PREC_F32 ? WAP_SCHED_KG_FUSED : WAP_SCHED_KG_2PASS;
   return h->d.schedule;
 }
 
@@ -333,7 +333,7 @@ bool kg_two_pass(const wap_handle *h) {
 
 int kg1_rows(const wap_handle *h) {
   const int ty = h->d.tile_y;
-  return (ty == 4 || ty == 8 || ty == 16) ? ty : 8;
+  return (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
 }
 
 template <typename R>

@@ -135,7 +135,7 @@ class FDTDParams:
         if self.equation != "klein_gordon":
             return "stages"
         if self.schedule == "auto":
-            return "kg_2pass" if self.precision == "f32_kahan" else "kg_fused"
+            return "kg_fused" if self.precision == "f32" else "kg_2pass"
         return self.schedule
 
     def with_(self, **kw):

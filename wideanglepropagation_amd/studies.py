@@ -1,0 +1,281 @@
+"""Workload drivers: precision study, batched probes, adjoint gradient.
+
+``python -m wideanglepropagation_amd.studies precision|batched|adjoint [--quick]``
+prints one JSON object per study.  Sizes follow the solver's standard
+workloads:
+
+precision  Klein-Gordon 200 keV plane wave through a 400-atom slab
+           (seed 0) on 512 x 512 x 256 (dx = dy = 0.05 A, dz = 0.005 A,
+           PML 16 planes), 4000 steps, in f32 (fused pass), f32_kahan and f64.
+           Exit wave = demodulated average of the exit plane over the last 40 %
+           of 40 frames (the notebooks' recipe); far field = |fftshift(fft2)|^2.
+           Reports relative L2 of exit wave and far field against f64, the norm
+           drift trajectory and throughput of each.
+batched    48 instances, sizes from {96^3, 128^3, 160x160x256, 192^3}, beam
+           energies U[60, 300] keV (seed 1), convergent Gaussian probes at
+           random lateral positions over a shared 200-atom slab; alternating
+           Schrodinger / Klein-Gordon, complex64, 1000 steps in one hipGraph
+           per equation group.  Reports aggregate Gvoxel-steps/s and the
+           launch shapes chosen by the autotuner per size class.
+adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
+           L = || |fft2(exit)|^2 - I_target ||^2 with I_target from a copy of the
+           150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
+           (seed 3).  sqrt(T) checkpointing.  Checks dL/dV against central
+           differences of the f64 forward at 32 random voxels (seed 4) and reports
+           backward / forward time.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .engine import FDTD3D
+from .physics import (klein_gordon_params, schrodinger_params, wavelength_from_kinetic_energy,
+                      beam_constants)
+from .workloads import kg_slab, atom_potential, random_slab, centred_extent
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def far_field(e):
+    return np.abs(np.fft.fftshift(np.fft.fft2(e))) ** 2
+
+
+def run_exit_wave(p, V, nsteps, n_frames, k_exit, norm_every=None):
+    """Step p/V for nsteps; return (exit wave, norm trajectory, seconds of stepping)."""
+    sim = FDTD3D(p, V=V)
+    spf = nsteps // n_frames
+    start = int(0.6 * n_frames)
+    acc = torch.zeros((p.ny, p.nx), dtype=torch.complex128, device="cuda")
+    norms = []
+    t_step = 0.0
+    for f in range(n_frames):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(spf)
+        sim.synchronize()
+        t_step += time.perf_counter() - t0
+        if f >= start:
+            t = (f + 1) * spf * p.dt
+            sim.accumulate_plane(k_exit, p.e_sim * t, acc)
+        if norm_every and (f + 1) % norm_every == 0:
+            norms.append(((f + 1) * spf, sim.norm2()))
+    ex = (acc / max(1, n_frames - start)).cpu().numpy()
+    final_norm = sim.norm2()
+    sim.close()
+    return ex, norms, final_norm, t_step
+
+
+def precision_study(quick=False):
+    shape = (128, 128, 128) if quick else (256, 512, 512)
+    nsteps = 400 if quick else 4000
+    n_frames = 40
+    out = {"study": "precision", "grid_nz_ny_nx": list(shape), "steps": nsteps, "runs": {}}
+    ref = None
+    for prec in ("f64", "f32_kahan", "f32"):
+        p, V = kg_slab(shape, precision=prec, device="cuda")
+        V = V.to(torch.float32)  # every run sees the same float32-rounded potential
+        V = V.to(torch.float64) if prec == "f64" else V
+        k_exit = p.nz - 16 - 3
+        ex, norms, n_final, secs = run_exit_wave(p, V, nsteps, n_frames, k_exit, norm_every=4)
+        del V
+        torch.cuda.empty_cache()
+        r = {"schedule": p.resolved_schedule(),
+             "gvoxel_steps_per_s": p.voxels * nsteps / secs / 1e9,
+             "norm2_final": n_final, "norm2_trajectory": norms,
+             "exit_wave_norm": float(np.linalg.norm(ex))}
+        if prec == "f64":
+            ref = (ex, far_field(ex), n_final, dict(norms))
+        else:
+            r["rel_l2_exit_wave_vs_f64"] = rel_l2(ex, ref[0])
+            r["rel_l2_far_field_vs_f64"] = rel_l2(far_field(ex), ref[1])
+            r["norm_drift_vs_f64"] = (n_final - ref[2]) / ref[2]
+            r["norm_drift_trajectory"] = [(s, (n - ref[3][s]) / ref[3][s]) for s, n in norms]
+        out["runs"][prec] = r
+        print(f"[precision] {prec}: " + json.dumps({k: v for k, v in r.items()
+                                                    if "trajectory" not in k}), flush=True)
+    return out
+
+
+def probe_profile(shape, extent, x0, y0, lam, width=1.0, focus=-5.0):
+    """Convergent Gaussian probe at (x0, y0): Gaussian envelope with a quadratic phase
+    focusing at distance ``focus`` beyond the source plane (paraxial)."""
+    nz, ny, nx = shape
+    x = np.linspace(extent[0], extent[1], nx)
+    y = np.linspace(extent[2], extent[3], ny)
+    Lx, Ly = nx * (x[1] - x[0]), ny * (y[1] - y[0])
+    X = (x[None, :] - x0 + Lx / 2) % Lx - Lx / 2
+    Y = (y[:, None] - y0 + Ly / 2) % Ly - Ly / 2
+    r2 = X ** 2 + Y ** 2
+    k = 2 * math.pi / lam
+    return np.exp(-r2 / (2 * width ** 2)) * np.exp(1j * k * r2 / (2 * focus))
+
+
+def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
+    from .batched import BatchedFDTD
+    from .autotune import tune
+    rng = np.random.default_rng(1)
+    classes = [(96, 96, 96), (128, 128, 128), (256, 160, 160), (192, 192, 192)]
+    if quick:
+        classes = [(32, 24, 24), (40, 32, 32), (48, 24, 24), (36, 36, 36)]
+        n_inst, nsteps = 8, 50
+    spacing, spacing_z = 0.05, 0.005
+    # shared slab: 200 atoms in the largest lateral cell, wrapped into each grid
+    big = max(c[1] for c in classes) * spacing
+    pos, Zs, scr = random_slab(200, ((-big / 2, big / 2), (-big / 2, big / 2)), (-0.2, 0.2), 11)
+    sims, meta = [], []
+    shapes = [classes[i] for i in rng.integers(0, len(classes), n_inst)]
+    energies = rng.uniform(60e3, 300e3, n_inst)
+    tuned = {}
+    for i, (shape, ekin) in enumerate(zip(shapes, energies)):
+        eq = "schrodinger" if i % 2 == 0 else "klein_gordon"
+        lam = wavelength_from_kinetic_energy(ekin)
+        ext = centred_extent(shape, spacing, spacing_z)
+        mk = schrodinger_params if eq == "schrodinger" else klein_gordon_params
+        p = mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18, precision="f32",
+               schedule="kg_2pass" if eq == "klein_gordon" else "auto")
+        key = (eq, shape)
+        if autotune and key not in tuned:
+            ty, tz, ms, _ = tune(p, tile_ys=(4,), tile_zs=(4, 8, 16, 32), steps=3, warmup=1)
+            tuned[key] = (ty, tz, ms)
+        if key in tuned:
+            p = p.with_(tile_y=tuned[key][0], tile_z=tuned[key][1])
+        V = atom_potential(shape, ext, pos, Zs, scr, r_soft=spacing, cutoff=4.0, device="cuda")
+        x0, y0 = rng.uniform(ext[0], ext[1]), rng.uniform(ext[2], ext[3])
+        prof = probe_profile(shape, ext, x0, y0, lam)
+        sims.append(FDTD3D(p, V=V.to(torch.float32), source_profile=prof))
+        meta.append({"eq": eq, "shape": list(shape), "E_kin_eV": float(ekin)})
+        del V
+    b = BatchedFDTD(sims, tile_y=4)
+    chunk = min(nsteps, 100)
+    b.capture(chunk)
+    b.replay()          # warm-up
+    b.synchronize()
+    for s in sims:
+        s.set_state(None)
+    b.step_index = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    done = 0
+    while done < nsteps:
+        dev_ms += b.replay(timed=True)
+        done += chunk
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    vox = sum(s.params.voxels for s in sims)
+    norms = [s.norm2() for s in sims]
+    out = {"study": "batched", "instances": n_inst, "steps": done,
+           "total_voxels": vox, "gvoxel_steps_per_s": vox * done / wall / 1e9,
+           "gvoxel_steps_per_s_device": vox * done / (dev_ms * 1e-3) / 1e9,
+           "graph_steps_per_launch": chunk, "launch_groups": b.tiles(),
+           "autotuned": {f"{k[0]}/{'x'.join(map(str, k[1]))}": {"tile_y": v[0], "tile_z": v[1],
+                                                                  "ms_per_step": v[2]}
+                         for k, v in tuned.items()},
+           "all_finite": bool(np.all(np.isfinite(norms))),
+           "instances_meta": meta[:6]}
+    b.close()
+    for s in sims:
+        s.close()
+    return out
+
+
+def adjoint_study(quick=False, n_fd=32):
+    from .adjoint import exit_wave, far_field_intensity
+    shape = (64, 32, 32) if quick else (256, 128, 128)
+    nsteps = 150 if quick else 1500
+    if quick:
+        n_fd = 6
+    spacing, spacing_z = 0.05, 0.005
+    ext = centred_extent(shape, spacing, spacing_z)
+    half = ext[5] - 20 * spacing_z
+    pos, Zs, scr = random_slab(150, ((ext[0], ext[1]), (ext[2], ext[3])), (-half, half), 2)
+    pert = pos + np.random.default_rng(3).normal(0.0, 0.05, pos.shape)
+    p32, V = kg_slab(shape, precision="f32", device="cuda", positions=(pos, Zs, scr),
+                     schedule="kg_2pass")
+    _, Vt = kg_slab(shape, precision="f32", device="cuda", positions=(pert, Zs, scr))
+    k_exit = p32.nz - 16 - 3
+    n_frames = 10
+    spf = nsteps // n_frames
+    samples = {(f + 1) * spf: 1.0 / 4 for f in range(6, 10)}
+    V32 = V.to(torch.float32)
+    with torch.no_grad():
+        It = far_field_intensity(exit_wave(Vt.to(torch.float32), p32, nsteps, k_exit,
+                                           samples=samples))
+
+    def loss_of(Vx, p):
+        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples)
+        return torch.sum((far_field_intensity(E) - It) ** 2)
+
+    # forward only (timed), then forward + backward (timed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        L0 = loss_of(V32, p32)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    Vg = V32.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L = loss_of(Vg, p32)
+    L.backward()
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    g32 = Vg.grad.double().cpu().numpy()
+    # f64 adjoint for the same loss
+    p64 = p32.with_(precision="f64")
+    Vd = V32.double().clone().requires_grad_(True)
+    L64 = loss_of(Vd, p64)
+    L64.backward()
+    g64 = Vd.grad.cpu().numpy()
+    # central differences of the f64 forward at random voxels (and the largest-gradient ones)
+    rng = np.random.default_rng(4)
+    idx = [tuple(int(rng.integers(0, s)) for s in shape) for _ in range(n_fd // 2)]
+    top = np.argsort(np.abs(g64).ravel())[-(n_fd - len(idx)):]
+    idx += [tuple(int(v) for v in np.unravel_index(t, shape)) for t in top]
+    hstep = 1e-3 * float(V32.abs().max())
+    rows = []
+    Vbase = V32.double()
+    for ix in idx:
+        Vp = Vbase.clone()
+        Vp[ix] += hstep
+        Vm = Vbase.clone()
+        Vm[ix] -= hstep
+        with torch.no_grad():
+            fd = (loss_of(Vp, p64).item() - loss_of(Vm, p64).item()) / (2 * hstep)
+        rows.append((ix, fd, float(g64[ix]), float(g32[ix])))
+    gmax = float(np.abs(g64).max())
+    err64 = [abs(r[2] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
+    err32 = [abs(r[3] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
+    return {"study": "adjoint", "grid_nz_ny_nx": list(shape), "steps": nsteps,
+            "checkpoint_every": math.ceil(math.sqrt(nsteps)), "loss": float(L0),
+            "t_forward_s": t_fwd, "t_forward_backward_s": t_fb,
+            "backward_over_forward": (t_fb - t_fwd) / t_fwd,
+            "fd_step": hstep, "fd_voxels": len(rows),
+            "max_rel_err_f64_adjoint_vs_fd": float(max(err64)),
+            "median_rel_err_f64_adjoint_vs_fd": float(np.median(err64)),
+            "max_rel_err_f32_adjoint_vs_fd": float(max(err32)),
+            "median_rel_err_f32_adjoint_vs_fd": float(np.median(err32)),
+            "rel_l2_f32_vs_f64_gradient": rel_l2(g32, g64),
+            "samples": [{"voxel": list(r[0]), "fd": r[1], "adj_f64": r[2], "adj_f32": r[3]}
+                        for r in rows[:6]]}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["precision", "batched", "adjoint"])
+    ap.add_argument("--quick", action="store_true")
+    a = ap.parse_args(argv)
+    fn = {"precision": precision_study, "batched": batched_study, "adjoint": adjoint_study}
+    out = fn[a.study](quick=a.quick)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,8 +18,8 @@ Standard workloads (all lengths in A, energies in eV, time in hbar/eV):
                              one Z = 14 atom (screening 0.3 A) at the centre,
                              source off, 200 steps.
   kg_slab                    Klein-Gordon plane wave (soft source at PML + 2),
-                             random atom slab, PML 16 cells (the 256x256x512 and
-                             512x512x256 grids).
+                             random atom slab, PML 16 planes, dx = dy = 0.05 A,
+                             dz = 0.005 A (the 256x256x512 and 512x512x256 grids).
 """
 import math
 
@@ -31,11 +31,12 @@ from .physics import (schrodinger_params, klein_gordon_params, beam_constants,
 KE2 = 14.399645  # eV A
 
 
-def centred_extent(shape, spacing):
+def centred_extent(shape, spacing, spacing_z=None):
     nz, ny, nx = shape
+    dz = spacing if spacing_z is None else spacing_z
     hx = (nx - 1) * spacing / 2.0
     hy = (ny - 1) * spacing / 2.0
-    hz = (nz - 1) * spacing / 2.0
+    hz = (nz - 1) * dz / 2.0
     return (-hx, hx, -hy, hy, -hz, hz)
 
 
@@ -44,7 +45,9 @@ def atom_potential(shape, extent, positions, Z, screening, r_soft, cutoff=8.0, d
     """Sum of screened-Coulomb atoms on the grid; returns a torch tensor (nz, ny, nx).
 
     positions (n,3) as (x, y, z) in A; Z, screening (n,) arrays.  x, y periodic
-    with period n*d (the grid has n points, spacing d = extent/(n-1)).
+    with period n*d (the grid has n points, spacing d = extent/(n-1)).  Each
+    atom is added over its cutoff box on a laterally padded grid (plain slice
+    adds, no scatter); the padding margins are folded back periodically.
     """
     import torch
     dtype = dtype or torch.float64
@@ -53,35 +56,48 @@ def atom_potential(shape, extent, positions, Z, screening, r_soft, cutoff=8.0, d
     dx = (x1 - x0) / (nx - 1)
     dy = (y1 - y0) / (ny - 1)
     dz = (z1 - z0) / (nz - 1)
-    V = torch.zeros(shape, dtype=dtype, device=device)
     positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
     Z = np.broadcast_to(np.asarray(Z, dtype=np.float64), (len(positions),))
     screening = np.broadcast_to(np.asarray(screening, dtype=np.float64), (len(positions),))
+    if len(positions) == 0:
+        return torch.zeros(shape, dtype=dtype, device=device)
+    rmax = cutoff * float(np.max(screening))
+    mx = min(int(math.ceil(rmax / dx)) + 1, nx)   # lateral padding (at most one period)
+    my = min(int(math.ceil(rmax / dy)) + 1, ny)
+    Vp = torch.zeros((nz, ny + 2 * my, nx + 2 * mx), dtype=dtype, device=device)
     for (px, py, pz), z_at, a in zip(positions, Z, screening):
         rc = cutoff * a
         ci, cj, ck = (px - x0) / dx, (py - y0) / dy, (pz - z0) / dz
-        ri, rj, rk = int(math.ceil(rc / dx)), int(math.ceil(rc / dy)), int(math.ceil(rc / dz))
-        ii = np.arange(int(math.floor(ci)) - ri, int(math.ceil(ci)) + ri + 1)
-        jj = np.arange(int(math.floor(cj)) - rj, int(math.ceil(cj)) + rj + 1)
-        kk = np.arange(max(0, int(math.floor(ck)) - rk), min(nz, int(math.ceil(ck)) + rk + 1))
-        if len(kk) == 0:
+        # wrap the centre into the primary cell, then take its box on the padded grid
+        ci = ci % nx
+        cj = cj % ny
+        ri = min(int(math.ceil(rc / dx)), mx - 1)
+        rj = min(int(math.ceil(rc / dy)), my - 1)
+        rk = int(math.ceil(rc / dz))
+        i0, i1 = int(math.floor(ci)) - ri, int(math.floor(ci)) + ri + 1
+        j0, j1 = int(math.floor(cj)) - rj, int(math.floor(cj)) + rj + 1
+        k0, k1 = max(0, int(math.floor(ck)) - rk), min(nz, int(math.floor(ck)) + rk + 2)
+        if k1 <= k0:
             continue
-        if len(ii) >= nx:
-            ii = np.arange(nx) + int(math.floor(ci)) - nx // 2
-        if len(jj) >= ny:
-            jj = np.arange(ny) + int(math.floor(cj)) - ny // 2
-        # minimum-image offsets of the (unwrapped) window indices
-        xs = torch.as_tensor((ii - ci) * dx, dtype=dtype, device=device)
-        ys = torch.as_tensor((jj - cj) * dy, dtype=dtype, device=device)
-        zs = torch.as_tensor(kk * dz + z0 - pz, dtype=dtype, device=device)
-        r2 = zs[:, None, None] ** 2 + ys[None, :, None] ** 2 + xs[None, None, :] ** 2
+        xs = torch.arange(i0, i1, dtype=dtype, device=device) - ci
+        ys = torch.arange(j0, j1, dtype=dtype, device=device) - cj
+        zs = torch.arange(k0, k1, dtype=dtype, device=device) * dz + z0 - pz
+        r2 = (zs[:, None, None] ** 2 + (ys[None, :, None] * dy) ** 2
+              + (xs[None, None, :] * dx) ** 2)
         r = torch.sqrt(r2)
         v = -KE2 * z_at * torch.exp(-r / a) / torch.sqrt(r2 + r_soft * r_soft)
-        iw = torch.as_tensor(np.mod(ii, nx), device=device)
-        jw = torch.as_tensor(np.mod(jj, ny), device=device)
-        kw = torch.as_tensor(kk, device=device)
-        V.index_put_((kw[:, None, None], jw[None, :, None], iw[None, None, :]), v,
-                     accumulate=True)
+        Vp[k0:k1, j0 + my:j1 + my, i0 + mx:i1 + mx] += v
+    # fold the margins back into the periodic cell
+    V = Vp[:, my:my + ny, mx:mx + nx].clone()
+    V[:, :, nx - mx:] += Vp[:, my:my + ny, :mx]
+    V[:, :, :mx] += Vp[:, my:my + ny, mx + nx:]
+    top = Vp[:, :my, :]
+    bot = Vp[:, my + ny:, :]
+    for band, dst in ((top, slice(ny - my, ny)), (bot, slice(0, my))):
+        row = band[:, :, mx:mx + nx].clone()
+        row[:, :, nx - mx:] += band[:, :, :mx]
+        row[:, :, :mx] += band[:, :, mx + nx:]
+        V[:, dst, :] += row
     return V
 
 
@@ -113,16 +129,25 @@ def schrodinger_gaussian_64(precision="f64", nsteps=200, n=64, spacing=0.05):
     return p, V, psi0, nsteps
 
 
-def kg_slab(shape, precision="f32", n_atoms=400, seed=0, spacing=0.05, pml_cells=16,
-            slab=20.0, device="cpu", kinetic_ev=200e3):
-    """Klein-Gordon plane wave through a random atom slab; returns (params, V torch)."""
+def kg_slab(shape, precision="f32", n_atoms=400, seed=0, spacing=0.05, spacing_z=0.005,
+            pml_cells=16, slab=20.0, device="cpu", kinetic_ev=200e3, schedule="auto",
+            positions=None, cutoff=8.0):
+    """Klein-Gordon plane wave through a random atom slab; returns (params, V torch).
+
+    Lateral spacing 0.05 A; along z 0.005 A so that the 200 keV carrier
+    (k = 250.5 / A) propagates on the grid (k dz = 1.25 rad; at dz = 0.05 A it
+    is evanescent).  The slab is clipped to the interior between the PML
+    layers; ``positions`` = (pos, Z, screening) overrides the random atoms.
+    """
     lam = wavelength_from_kinetic_energy(kinetic_ev)
-    ext = centred_extent(shape, spacing)
-    nz, ny, nx = shape
-    half = min(slab / 2.0, ext[5] - (pml_cells + 4) * spacing)
-    pos, Z, scr = random_slab(n_atoms, ((ext[0], ext[1] + spacing), (ext[2], ext[3] + spacing)),
-                              (-half, half), seed)
-    p = klein_gordon_params(lam, ext, shape, pml_thick=pml_cells * spacing,
-                            k_source=pml_cells + 2, precision=precision)
-    V = atom_potential(shape, ext, pos, Z, scr, r_soft=spacing, device=device)
+    ext = centred_extent(shape, spacing, spacing_z)
+    half = min(slab / 2.0, ext[5] - (pml_cells + 4) * spacing_z)
+    if positions is None:
+        pos, Z, scr = random_slab(n_atoms, ((ext[0], ext[1] + spacing),
+                                            (ext[2], ext[3] + spacing)), (-half, half), seed)
+    else:
+        pos, Z, scr = positions
+    p = klein_gordon_params(lam, ext, shape, pml_thick=pml_cells * spacing_z,
+                            k_source=pml_cells + 2, precision=precision, schedule=schedule)
+    V = atom_potential(shape, ext, pos, Z, scr, r_soft=spacing, cutoff=cutoff, device=device)
     return p, V
