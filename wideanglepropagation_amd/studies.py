@@ -199,6 +199,9 @@ def adjoint_study(quick=False, n_fd=32):
     pert = pos + np.random.default_rng(3).normal(0.0, 0.05, pos.shape)
     p32, V = kg_slab(shape, precision="f32", device="cuda", positions=(pos, Zs, scr),
                      schedule="kg_2pass")
+    # dt = 1e-6 (stable: max |w| dt ~ 0.9): 1500 steps carry the wave front across the
+    # 1.1 A between the source and the exit plane (at 5e-7 it would not yet arrive)
+    p32 = p32.with_(dt=1e-6)
     _, Vt = kg_slab(shape, precision="f32", device="cuda", positions=(pert, Zs, scr))
     k_exit = p32.nz - 16 - 3
     n_frames = 10

@@ -141,7 +141,7 @@ def kg_slab(shape, precision="f32", n_atoms=400, seed=0, spacing=0.05, spacing_z
     """
     lam = wavelength_from_kinetic_energy(kinetic_ev)
     ext = centred_extent(shape, spacing, spacing_z)
-    half = min(slab / 2.0, ext[5] - (pml_cells + 4) * spacing_z)
+    half = max(0.0, min(slab / 2.0, ext[5] - (pml_cells + 4) * spacing_z))
     if positions is None:
         pos, Z, scr = random_slab(n_atoms, ((ext[0], ext[1] + spacing),
                                             (ext[2], ext[3] + spacing)), (-half, half), seed)
