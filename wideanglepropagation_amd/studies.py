@@ -151,6 +151,8 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
         prof = probe_profile(shape, ext, x0, y0, lam)
         sims.append(FDTD3D(p, V=V.to(torch.float32), source_profile=prof))
         meta.append({"eq": eq, "shape": list(shape), "E_kin_eV": float(ekin)})
+        if (i + 1) % 8 == 0:
+            print(f"[batched] {i + 1}/{n_inst} instances set up", file=sys.stderr, flush=True)
         del V
     b = BatchedFDTD(sims, tile_y=4)
     chunk = min(nsteps, 100)
@@ -231,6 +233,8 @@ def adjoint_study(quick=False, n_fd=32):
     torch.cuda.synchronize()
     t_fb = time.perf_counter() - t0
     g32 = Vg.grad.double().cpu().numpy()
+    print(f"[adjoint] forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s", file=sys.stderr,
+          flush=True)
     # f64 adjoint for the same loss
     p64 = p32.with_(precision="f64")
     Vd = V32.double().clone().requires_grad_(True)
@@ -253,6 +257,9 @@ def adjoint_study(quick=False, n_fd=32):
         with torch.no_grad():
             fd = (loss_of(Vp, p64).item() - loss_of(Vm, p64).item()) / (2 * hstep)
         rows.append((ix, fd, float(g64[ix]), float(g32[ix])))
+        if len(rows) % 8 == 0:
+            print(f"[adjoint] {len(rows)}/{len(idx)} finite differences", file=sys.stderr,
+                  flush=True)
     gmax = float(np.abs(g64).max())
     err64 = [abs(r[2] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
     err32 = [abs(r[3] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
