@@ -212,6 +212,16 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
 /* dL/dV as float64 (nz,ny,nx); synchronises. */
 int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device);
 
+/* Potential of screened-Coulomb atoms on a grid (replaces the abTEM potentials the
+ * reference notebooks build, FDTDmethod/fdtd_3D_baseline_comparison.ipynb:129-180):
+ * V = sum over atoms and their periodic x, y images within cutoff*s of
+ * -14.399645 Z exp(-r/s) / sqrt(r^2 + r_soft^2) eV.  atoms: host float64 records
+ * (x, y, z, Z, s) in A; extent (xmin, xmax, ymin, ymax, zmin, zmax); out: device
+ * (nz, ny, nx) of dtype WAP_DT_F32 / F64, written on `stream`; synchronises. */
+int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
+                       const double *extent, double r_soft, double cutoff, void *out, int dtype,
+                       void *stream);
+
 /* Measured HBM bandwidth reference (no reference counterpart): a STREAM copy
  * (read + write) and a read-only sweep of `bytes`-sized device buffers, reps
  * launches each, timed with HIP events on the null stream; GB/s = 1e9 B/s. */

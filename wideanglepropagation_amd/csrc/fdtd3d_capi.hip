@@ -1464,3 +1464,40 @@ extern "C" int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, d
   (void)hipFree(sink);
   return rc;
 }
+
+extern "C" int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
+                                  const double *extent, double r_soft, double cutoff, void *out,
+                                  int dtype, void *stream) {
+  if (!atoms || n_atoms < 0 || !extent || !out) return fail(WAP_ERR_ARG, "NULL argument");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(WAP_ERR_ARG, "grid must be at least 2 x 2 x 2");
+  if (dtype != WAP_DT_F32 && dtype != WAP_DT_F64)
+    return fail(WAP_ERR_ARG, "V dtype must be WAP_DT_F32 or WAP_DT_F64");
+  if (!(cutoff > 0.0)) return fail(WAP_ERR_ARG, "cutoff must be > 0");
+  const double dx = (extent[1] - extent[0]) / (nx - 1.0);
+  const double dy = (extent[3] - extent[2]) / (ny - 1.0);
+  const double dz = (extent[5] - extent[4]) / (nz - 1.0);
+  double *d_atoms = nullptr;
+  const size_t ab = (size_t)(n_atoms > 0 ? n_atoms : 1) * 5 * sizeof(double);
+  WAP_HIP(hipMalloc((void **)&d_atoms, ab));
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(d_atoms, atoms, (size_t)n_atoms * 5 * sizeof(double),
+                                hipMemcpyHostToDevice, st);
+  const size_t total = (size_t)nx * ny * nz;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  const double ke2 = 14.399645;  // e^2 / (4 pi eps0) in eV A
+  if (e == hipSuccess) {
+    if (dtype == WAP_DT_F64)
+      hipLaunchKernelGGL(wap::atom_potential_kernel<double>, grid, block, 0, st, d_atoms, n_atoms,
+                         nx, ny, nz, extent[0], extent[2], extent[4], dx, dy, dz, r_soft * r_soft,
+                         cutoff, ke2, (double *)out);
+    else
+      hipLaunchKernelGGL(wap::atom_potential_kernel<float>, grid, block, 0, st, d_atoms, n_atoms,
+                         nx, ny, nz, extent[0], extent[2], extent[4], dx, dy, dz, r_soft * r_soft,
+                         cutoff, ke2, (float *)out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d_atoms);
+  if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("wap_atom_potential: ") + hipGetErrorString(e));
+  return WAP_OK;
+}

@@ -1038,6 +1038,58 @@ __global__ __launch_bounds__(256) void sum_partials(const double *partial, int n
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Screened-Coulomb atom sum V(r) = sum_a sum_images -KE2 Z_a exp(-r/s_a) / sqrt(r^2 + r_s^2)
+// over the periodic x, y images with r <= cutoff s_a (z is not periodic).
+// atoms: n records (x, y, z, Z, s) in float64; staged through LDS in chunks.
+constexpr int kAtomChunk = 256;
+
+template <typename R>
+__global__ __launch_bounds__(256) void atom_potential_kernel(
+    const double *__restrict__ atoms, int n, int nx, int ny, int nz, double x0, double y0,
+    double z0, double dx, double dy, double dz, double rsoft2, double cutoff, double ke2,
+    R *__restrict__ V) {
+  __shared__ double sa[kAtomChunk * 5];
+  const size_t total = (size_t)nx * ny * nz;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < total;
+  const int i = live ? (int)(t % nx) : 0;
+  const int j = live ? (int)((t / nx) % ny) : 0;
+  const int k = live ? (int)(t / ((size_t)nx * ny)) : 0;
+  const double x = x0 + i * dx, y = y0 + j * dy, z = z0 + k * dz;
+  const double Lx = nx * dx, Ly = ny * dy;
+  double acc = 0.0;
+  for (int c0 = 0; c0 < n; c0 += kAtomChunk) {
+    const int m = min(kAtomChunk, n - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < m * 5; q += blockDim.x) sa[q] = atoms[(size_t)c0 * 5 + q];
+    __syncthreads();
+    if (!live) continue;
+    for (int a = 0; a < m; ++a) {
+      const double s = sa[a * 5 + 4];
+      const double rc = cutoff * s;
+      const double ez = z - sa[a * 5 + 2];
+      if (fabs(ez) > rc) continue;
+      double ex = x - sa[a * 5 + 0];
+      double ey = y - sa[a * 5 + 1];
+      ex -= Lx * floor(ex / Lx + 0.5);   // nearest image, then neighbours within rc
+      ey -= Ly * floor(ey / Ly + 0.5);
+      const int mx = (int)ceil(rc / Lx), my = (int)ceil(rc / Ly);
+      const double zq = sa[a * 5 + 3];
+      for (int sy = -my; sy <= my; ++sy) {
+        const double fy = ey + sy * Ly;
+        for (int sx = -mx; sx <= mx; ++sx) {
+          const double fx = ex + sx * Lx;
+          const double r2 = fx * fx + fy * fy + ez * ez;
+          if (r2 > rc * rc) continue;
+          const double r = sqrt(r2);
+          acc += -ke2 * zq * exp(-r / s) / sqrt(r2 + rsoft2);
+        }
+      }
+    }
+  }
+  if (live) V[t] = (R)acc;
+}
+
 // STREAM-style copy / read of n 16-byte words (bandwidth reference for the roofline)
 __global__ __launch_bounds__(256) void stream_copy(const float4 *__restrict__ a,
                                                    float4 *__restrict__ b, size_t n) {

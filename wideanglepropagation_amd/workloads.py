@@ -8,9 +8,11 @@ regularised ``1/sqrt(r^2 + r_s^2)`` form the reference itself uses in 2D
 
     V(r) = - KE2 * Z * exp(-r / a) / sqrt(r^2 + r_s^2)     [eV],  KE2 = e^2/(4 pi eps0) = 14.399645 eV A
 
-summed over atoms with minimum-image distances along the periodic x and y
-axes (z is not periodic).  Atoms contribute only inside a cutoff box
-(default 8 screening lengths, where exp(-r/a) < 3.4e-4).
+summed over atoms and their periodic images along x and y (z is not
+periodic) within a cutoff sphere of 8 screening lengths (exp(-r/a) < 3.4e-4
+there).  On the GPU the sum is one HIP launch (``wap_atom_potential``); the
+CPU builder adds each atom over its cutoff box and needs the cutoff below
+half the lateral cell.
 
 Standard workloads (all lengths in A, energies in eV, time in hbar/eV):
   schrodinger_gaussian_64    64^3 Schrodinger, complex128, dx = 0.05, PML 8 cells,
@@ -61,6 +63,12 @@ def atom_potential(shape, extent, positions, Z, screening, r_soft, cutoff=8.0, d
     screening = np.broadcast_to(np.asarray(screening, dtype=np.float64), (len(positions),))
     if len(positions) == 0:
         return torch.zeros(shape, dtype=dtype, device=device)
+    if str(device).startswith("cuda"):
+        return _atom_potential_hip(shape, extent, positions, Z, screening, r_soft, cutoff, device,
+                                   dtype)
+    if cutoff * float(np.max(screening)) * 2 >= min(nx * dx, ny * dy):
+        return _atom_potential_images(shape, extent, positions, Z, screening, r_soft, cutoff,
+                                      device, dtype)
     rmax = cutoff * float(np.max(screening))
     mx = min(int(math.ceil(rmax / dx)) + 1, nx)   # lateral padding (at most one period)
     my = min(int(math.ceil(rmax / dy)) + 1, ny)
@@ -86,6 +94,7 @@ def atom_potential(shape, extent, positions, Z, screening, r_soft, cutoff=8.0, d
               + (xs[None, None, :] * dx) ** 2)
         r = torch.sqrt(r2)
         v = -KE2 * z_at * torch.exp(-r / a) / torch.sqrt(r2 + r_soft * r_soft)
+        v = torch.where(r2 <= rc * rc, v, torch.zeros_like(v))
         Vp[k0:k1, j0 + my:j1 + my, i0 + mx:i1 + mx] += v
     # fold the margins back into the periodic cell
     V = Vp[:, my:my + ny, mx:mx + nx].clone()
@@ -99,6 +108,48 @@ def atom_potential(shape, extent, positions, Z, screening, r_soft, cutoff=8.0, d
         row[:, :, :mx] += band[:, :, mx + nx:]
         V[:, dst, :] += row
     return V
+
+
+def _atom_potential_images(shape, extent, positions, Z, screening, r_soft, cutoff, device, dtype):
+    """CPU sum over every periodic image within the cutoff (cutoffs beyond half the cell)."""
+    import torch
+    nz, ny, nx = shape
+    x0, x1, y0, y1, z0, z1 = extent
+    x = torch.linspace(x0, x1, nx, dtype=dtype)
+    y = torch.linspace(y0, y1, ny, dtype=dtype)
+    z = torch.linspace(z0, z1, nz, dtype=dtype)
+    Lx, Ly = nx * (x1 - x0) / (nx - 1), ny * (y1 - y0) / (ny - 1)
+    V = torch.zeros(shape, dtype=dtype)
+    for (px, py, pz), z_at, a in zip(positions, Z, screening):
+        rc = cutoff * a
+        mx, my = int(math.ceil(rc / Lx)) + 1, int(math.ceil(rc / Ly)) + 1
+        ez = (z - pz)[:, None, None]
+        for sy in range(-my, my + 1):
+            ey = (y - py + sy * Ly)[None, :, None]
+            for sx in range(-mx, mx + 1):
+                ex = (x - px + sx * Lx)[None, None, :]
+                r2 = ex * ex + ey * ey + ez * ez
+                v = -KE2 * z_at * torch.exp(-torch.sqrt(r2) / a) / torch.sqrt(r2 + r_soft ** 2)
+                V += torch.where(r2 <= rc * rc, v, torch.zeros_like(v))
+    return V.to(device)
+
+
+def _atom_potential_hip(shape, extent, positions, Z, screening, r_soft, cutoff, device, dtype):
+    """The same sum on the GPU in one launch (``wap_atom_potential``)."""
+    import ctypes
+    import torch
+    from ._lib import check, lib, DT_F32, DT_F64
+    nz, ny, nx = shape
+    recs = np.ascontiguousarray(np.column_stack([positions, Z, screening]).astype(np.float64))
+    ext = np.ascontiguousarray(np.asarray(extent, dtype=np.float64))
+    out = torch.empty(shape, dtype=dtype, device=device)
+    code = DT_F64 if dtype == torch.float64 else DT_F32
+    check(lib().wap_atom_potential(ctypes.c_void_p(recs.ctypes.data), len(recs), nx, ny, nz,
+                                   ctypes.c_void_p(ext.ctypes.data), float(r_soft), float(cutoff),
+                                   ctypes.c_void_p(out.data_ptr()), code,
+                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+          "wap_atom_potential")
+    return out
 
 
 def random_slab(n_atoms, cell_xy, slab_z, seed, z_choices=(14, 29, 79), screening=(0.3, 0.5)):
