@@ -45,6 +45,21 @@ def schedule_of(p):
     return p.resolved_schedule()
 
 
+def profiled_traffic(kernel, p):
+    """HBM bytes per launch of this kernel on this workload, from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy); None if not profiled."""
+    import re
+    eq = "kg" if p.equation == "klein_gordon" else "schr"
+    name = f"{kernel}_{eq}_{p.nx}x{p.ny}x{p.nz}_{p.precision}.md"
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*(\d+)", f.read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +187,9 @@ def main():
     from wideanglepropagation_amd._lib import stream_bandwidth
     copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
 
+    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass"}.get(
+        sched, f"rk4_pass_s{dom + 1}")
+    traffic = profiled_traffic(kernel, p)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p.equation, shape)
@@ -196,7 +214,7 @@ def main():
                    "equation": p.equation, "schedule": sched, "tile_y": ty, "tile_z": tz},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
                                 "kg_fused": "kg1_pass"}.get(sched, f"rk4_pass stage {dom + 1}"),
                      "bytes_per_launch": dom_bytes,

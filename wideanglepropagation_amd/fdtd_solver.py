@@ -178,7 +178,7 @@ class RelativisticSolver3D:
                               pml_sigma0=float(np.float32(float("%.4f" % pml_sigma))),
                               pml_theta=float(np.float32(float("%.4f" % pml_theta))),
                               pml_power=float(np.float32(float("%.1f" % pml_power))),
-                              src_k=int(k_source), **extra)
+                              src_alpha=float(np.float32(0.05)), src_k=int(k_source), **extra)
         return FDTDParams(nx=nx, ny=ny, nz=nz, extent=tuple(extent), lap_coeff=0.5,
                           e_sim=0.5 * p_inf * p_inf, k_beam=p_inf, pml_thick=pml_thick,
                           pml_sigma0=pml_sigma, pml_theta=pml_theta, pml_power=pml_power,

@@ -197,7 +197,8 @@ def schrodinger_params(wavelength_angstrom, extent, shape, pml_thick=0.5, pml_si
         base = dict(lap_coeff=float(a32), e_sim=float(e_sim), k_beam=float(k_beam), extent=ext,
                     pml_thick=_dec(pml_thick, "%.4f"), pml_sigma0=_dec(pml_sigma, "%.4f"),
                     pml_theta=_dec(pml_theta, "%.4f"), pml_power=_dec(pml_power, "%.1f"),
-                    dt=float(np.float32(kw.pop("dt", DT_DEFAULT))), time_mode="f32")
+                    dt=float(np.float32(kw.pop("dt", DT_DEFAULT))), time_mode="f32",
+                    src_alpha=float(np.float32(SRC_ALPHA)))   # the float literal 0.05f
     else:
         base = dict(lap_coeff=c["A"], e_sim=c["E_sim"], k_beam=c["P_INF"], extent=tuple(extent),
                     pml_thick=pml_thick, pml_sigma0=pml_sigma, pml_theta=pml_theta,
