@@ -336,6 +336,33 @@ int kg1_rows(const wap_handle *h) {
   return (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
 }
 
+// The fused pass holds 16 waves per CU (4 per SIMD at ~120 VGPRs), i.e. 16 / rows
+// workgroups per CU on 256 CUs.  Pick the z chunk that minimises
+// (full rounds of workgroups) x (planes per chunk + 2 halo planes): few, long
+// chunks amortise the halo planes, but a last round that is mostly empty costs
+// a whole round.
+int kg1_tile_z(const wap_handle *h) {
+  if (h->d.tile_z > 0) return h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
+  const int rows = kg1_rows(h);
+  const long cols = (long)((h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2)) *
+                    ((h->d.ny + rows - 3) / (rows - 2));
+  const long slots = 256L * (16 / rows);
+  const int cand[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+  int best = h->d.nz;
+  double best_cost = 1e300;
+  for (int t : cand) {
+    const int tz = t < h->d.nz ? t : h->d.nz;
+    const long groups = cols * ((h->d.nz + tz - 1) / tz);
+    const double cost = (double)((groups + slots - 1) / slots) * (tz + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = tz;
+    }
+    if (t >= h->d.nz) break;
+  }
+  return best;
+}
+
 template <typename R>
 int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                   double *pass_ms) {
@@ -343,8 +370,11 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   const int rows = kg1_rows(h);
   const int ntx = (h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2);
   const int nty = (h->d.ny + rows - 3) / (rows - 2);
+  const int tz = kg1_tile_z(h);
+  const int ntz = (h->d.nz + tz - 1) / tz;
   const dim3 block(wap::kWaveX, rows);
-  const dim3 grid((unsigned)(ntx * nty * h->ntz));
+  const dim3 grid((unsigned)(ntx * nty * ntz));
+
   wap::KG1Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
   a.V = (const R *)h->V;
@@ -352,8 +382,8 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   a.gam = (const R *)h->gam;
   a.prof = (const C *)h->prof;
   a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
-  a.tile_z = h->tile_z;
-  a.ntx = ntx; a.nty = nty; a.ntz = h->ntz;
+  a.tile_z = tz;
+  a.ntx = ntx; a.nty = nty; a.ntz = ntz;
   a.hx = (R)h->hx; a.hy = (R)h->hy; a.m2 = (R)h->m2; a.h = (R)h->d.dt;
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
@@ -362,7 +392,11 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
     a.out_psi = (C *)h->ya_psi;
     a.out_pi = (C *)h->ya_pi;
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    hipLaunchKernelGGL((wap::kg1_pass<R>), grid, block, 0, h->stream, a);
+    switch (rows) {
+      case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 4>), grid, block, 0, h->stream, a); break;
+      case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 8>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::kg1_pass<R, 16>), grid, block, 0, h->stream, a); break;
+    }
     // the new state is in the other buffer
     std::swap(h->psi, h->ya_psi);
     std::swap(h->pi, h->ya_pi);
@@ -886,11 +920,12 @@ int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *g
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (sched_of(h) == WAP_SCHED_KG_FUSED) {
     const int rows = kg1_rows(h);
+    const int tz = kg1_tile_z(h);
     if (tile_y) *tile_y = rows;
-    if (tile_z) *tile_z = h->tile_z;
+    if (tile_z) *tile_z = tz;
     if (groups)
       *groups = (int64_t)((h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2)) *
-                ((h->d.ny + rows - 3) / (rows - 2)) * h->ntz;
+                ((h->d.ny + rows - 3) / (rows - 2)) * ((h->d.nz + tz - 1) / tz);
     return WAP_OK;
   }
   if (tile_y) *tile_y = h->tile_y;

@@ -558,6 +558,8 @@ __global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__rest
 // plus the halo columns / planes re-read through L2.
 constexpr int kKG1MaxRows = 16;
 
+
+
 template <typename R>
 struct KG1Args {
   using C = cplx<R>;
@@ -583,12 +585,26 @@ struct KG1Plane {  // pass-1 results of one voxel
   cplx<R> y3, y4, z3, pp, qq;
 };
 
-template <typename R>
-__global__ __launch_bounds__(1024) void kg1_pass(const KG1Args<R> p) {
+// Loop over the pass-1 planes m of the chunk (plus one below and one above):
+//   top of iteration m: LDS S = injected state of plane m (every lane's column),
+//                       LDS Y = Y3 / Y4 of plane m - 1;
+//   compute pass-1 at m (lateral state from S; the tile's border lanes read
+//   the one neighbour outside the tile from global memory) and pass-2 at m - 1
+//   (lateral Y3 / Y4 from Y); write Y <- plane m, S <- plane m + 1 into the
+//   other buffer of each pair; barrier.
+// Each lane loads its own column one plane ahead (state, V, border neighbours),
+// so global latency overlaps the compute of the current plane.
+template <typename R, int TY>
+__global__ __launch_bounds__(kWaveX * TY) void kg1_pass(const KG1Args<R> p) {
   using C = cplx<R>;
-  __shared__ C l3[kKG1MaxRows * kWaveX];
-  __shared__ C l4[kKG1MaxRows * kWaveX];
-  const int TY = blockDim.y;
+  // double-buffered planes: iteration m reads buffer m & 1 and writes buffer (m + 1) & 1,
+  // so one barrier per plane separates every write from the reads that follow it.
+  // Y3 [0, 2 npl), Y4 [2 npl, 4 npl), state psi [4 npl, 6 npl), state pi [6 npl, 8 npl)
+  constexpr int npl = kWaveX * TY;
+  __shared__ C lq3b[2][npl];
+  __shared__ C lq4b[2][npl];
+  __shared__ C lssb[2][npl];
+  __shared__ C lsqb[2][npl];
   const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
@@ -596,7 +612,6 @@ __global__ __launch_bounds__(1024) void kg1_pass(const KG1Args<R> p) {
   const int tz = rest / p.nty;
   const int lx = threadIdx.x, ly = threadIdx.y;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  // unwrapped column of this lane; the tile's outputs are lanes 1..62, rows 1..TY-2
   const int gi = tx * (kWaveX - 2) - 1 + lx;
   const int gj = tyi * (TY - 2) - 1 + ly;
   const int i = ((gi % nx) + nx) % nx;
@@ -604,12 +619,13 @@ __global__ __launch_bounds__(1024) void kg1_pass(const KG1Args<R> p) {
   const bool writer = lx >= 1 && lx <= kWaveX - 2 && ly >= 1 && ly <= TY - 2 && gi < nx && gj < ny;
   const size_t plane = (size_t)nx * ny;
   const int ij = j * nx + i;
-  const int ip = (i + 1 == nx) ? 0 : i + 1;
-  const int im = (i == 0) ? nx - 1 : i - 1;
-  const int jp = (j + 1 == ny) ? 0 : j + 1;
-  const int jm = (j == 0) ? ny - 1 : j - 1;
-  const int o_xp = j * nx + ip, o_xm = j * nx + im;
-  const int o_yp = jp * nx + i, o_ym = jm * nx + i;
+  // the one neighbour of a border lane that lies outside the tile
+  const bool bxm = lx == 0, bxp = lx == kWaveX - 1, bym = ly == 0, byp = ly == TY - 1;
+  const int ox = bxm ? j * nx + ((i == 0) ? nx - 1 : i - 1)
+                     : j * nx + ((i + 1 == nx) ? 0 : i + 1);
+  const int oy = bym ? ((j == 0) ? ny - 1 : j - 1) * nx + i
+                     : ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+  const bool bx = bxm || bxp, by = bym || byp;
   const int sl = ly * kWaveX + lx;
   const C zero = mkc<R>(R(0), R(0));
   const C ph = p.src_ph;
@@ -617,133 +633,155 @@ __global__ __launch_bounds__(1024) void kg1_pass(const KG1Args<R> p) {
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
 
-  // injected state at (m, off)
-  auto state = [&](int m, int off, const SrcPlane<R> &sp, C &s, C &q) {
-    const size_t id = (size_t)m * plane + off;
-    s = inj<R>(p.in_psi[id], sp, sp.c, prof, off);
-    q = inj<R>(p.in_pi[id], sp, sp.cq, prof, off);
+  // raw loads of one plane for this lane (own column, border neighbours, V)
+  struct Raw {
+    C s, q, xs, xq, ys, yq;
+    R v;
   };
-  // pass-1 quantities at plane m from the column queue (s, q at m-1, m, m+1)
-  auto pass1 = [&](int m, const SrcPlane<R> &sp, C sm_, C qm_, C sc_, C qc_, C sp_, C qp_,
-                   R v) -> KG1Plane<R> {
-    KG1Plane<R> o;
-    if (m == 0 || m == nz - 1) {
-      o.y3 = sc_;
-      o.y4 = sc_;
-      o.z3 = qc_;
-      o.pp = sc_;
-      o.qq = qc_;
-      return o;
+  auto load_plane = [&](int m, Raw &r) {
+    r.s = r.q = r.xs = r.xq = r.ys = r.yq = zero;
+    r.v = R(0);
+    if (m < 0 || m >= nz) return;
+    const size_t b = (size_t)m * plane;
+    r.s = p.in_psi[b + ij];
+    r.q = p.in_pi[b + ij];
+    r.v = p.V[b + ij];
+    if (bx) {
+      r.xs = p.in_psi[b + ox];
+      r.xq = p.in_pi[b + ox];
     }
-    C ax0, ax1, ay0, ay1, bx0, bx1, by0, by1;
-    state(m, o_xp, sp, ax0, bx0);
-    state(m, o_xm, sp, ax1, bx1);
-    state(m, o_yp, sp, ay0, by0);
-    state(m, o_ym, sp, ay1, by1);
-    const C qhm = (m - 1 == 0) ? zero : qm_;        // q^ : 0 on the frozen planes
-    const C qhp = (m + 1 == nz - 1) ? zero : qp_;
-    const R g = p.gam[m];
-    const R w = p.m2 - v * v;
-    const C wzp = p.zc[2 * m], wzm = p.zc[2 * m + 1];
-    auto lap = [&](C c, C xp, C xm, C yp, C ym, C zp, C zm) {
-      const C lxx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
-      const C lyy = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
-      const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
-      return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
-    };
-    const C As = csub<R>(lap(sc_, ax0, ax1, ay0, ay1, sp_, sm_), cscale<R>(sc_, w));
-    const C Aq = csub<R>(lap(qc_, bx0, bx1, by0, by1, qhp, qhm), cscale<R>(qc_, w));
-    const C f = cadd<R>(As, kg2_damp<R>(v, g, qc_));
-    const C u = caxpy<R>(qc_, p.h * R(0.5), f);
-    const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
-    o.y3 = caxpy<R>(sc_, p.h * R(0.5), u);
-    o.z3 = caxpy<R>(qc_, p.h * R(0.5), gg);
-    o.y4 = caxpy<R>(sc_, p.h, o.z3);
-    o.pp = cadd<R>(sc_, cscale<R>(cadd<R>(qc_, cscale<R>(cadd<R>(u, o.z3), R(2))), p.h / R(6)));
-    o.qq = cadd<R>(qc_, cscale<R>(caxpy<R>(f, R(2), gg), p.h / R(6)));
-    return o;
+    if (by) {
+      r.ys = p.in_psi[b + oy];
+      r.yq = p.in_pi[b + oy];
+    }
+  };
+  auto inject_plane = [&](int m, Raw &r) {
+    if (m < 0 || m >= nz) return;
+    const SrcPlane<R> sp = src_plane<R>(p, m, ph);
+    if (!sp.band) return;
+    r.s = inj<R>(r.s, sp, sp.c, prof, ij);
+    r.q = inj<R>(r.q, sp, sp.cq, prof, ij);
+    if (bx) {
+      r.xs = inj<R>(r.xs, sp, sp.c, prof, ox);
+      r.xq = inj<R>(r.xq, sp, sp.cq, prof, ox);
+    }
+    if (by) {
+      r.ys = inj<R>(r.ys, sp, sp.c, prof, oy);
+      r.yq = inj<R>(r.yq, sp, sp.cq, prof, oy);
+    }
   };
 
-  // column queue of the injected state: planes a = m-1, b = m, c = m+1
-  C sa = zero, qa = zero, sb = zero, qb = zero, sc = zero, qc = zero;
-  SrcPlane<R> spb = src_plane<R>(p, k0 - 1, ph);
-  if (k0 - 2 >= 0) {
-    const SrcPlane<R> t = src_plane<R>(p, k0 - 2, ph);
-    state(k0 - 2, ij, t, sa, qa);
-  }
-  if (k0 - 1 >= 0) state(k0 - 1, ij, spb, sb, qb);
-  SrcPlane<R> spc = src_plane<R>(p, k0, ph);
-  state(k0, ij, spc, sc, qc);
-  // pass-1 at k0-1 (only its Y3 / Y4 are used: the -z neighbour of plane k0)
-  C y3m = zero, y4m = zero;
-  if (k0 - 1 >= 0) {
-    const KG1Plane<R> o = pass1(k0 - 1, spb, sa, qa, sb, qb, sc, qc, p.V[(size_t)(k0 - 1) * plane + ij]);
-    y3m = o.y3;
-    y4m = o.y4;
-  }
-  // pass-1 at k0
-  sa = sb; qa = qb; sb = sc; qb = qc;
-  spb = spc;
-  spc = src_plane<R>(p, k0 + 1, ph);
-  sc = zero; qc = zero;
-  if (k0 + 1 < nz) state(k0 + 1, ij, spc, sc, qc);
-  R vk = p.V[(size_t)k0 * plane + ij];
-  KG1Plane<R> cur = pass1(k0, spb, sa, qa, sb, qb, sc, qc, vk);
-  C qk = qb;  // injected pi at k0
-  l3[sl] = cur.y3;
-  l4[sl] = cur.y4;
+  const int m_start = max(k0 - 1, 0);
+  const int m_end = (k1 == nz) ? nz - 1 : k1;  // inclusive
+  // column queue: a = m-1, b = m, c = m+1 (injected); pre = raw m+2
+  Raw ra, rb, rc, pre;
+  load_plane(m_start - 1, ra);
+  load_plane(m_start, rb);
+  load_plane(m_start + 1, rc);
+  inject_plane(m_start - 1, ra);
+  inject_plane(m_start, rb);
+  inject_plane(m_start + 1, rc);
+  lssb[m_start & 1][sl] = rb.s;
+  lsqb[m_start & 1][sl] = rb.q;
   __syncthreads();
 
-  for (int k = k0; k < k1; ++k) {
-    // pass-1 at k+1: needs the state at k+2
-    sa = sb; qa = qb; sb = sc; qb = qc;
-    spb = spc;
-    spc = src_plane<R>(p, k + 2, ph);
-    sc = zero; qc = zero;
-    if (k + 2 < nz) state(k + 2, ij, spc, sc, qc);
-    KG1Plane<R> nxt;
-    R vn = R(0);
-    if (k + 1 < nz) {
-      vn = p.V[(size_t)(k + 1) * plane + ij];
-      nxt = pass1(k + 1, spb, sa, qa, sb, qb, sc, qc, vn);
+  KG1Plane<R> prev;                 // pass-1 results at m-1
+  prev.y3 = prev.y4 = prev.z3 = prev.pp = prev.qq = zero;
+  C y3mm = zero, y4mm = zero;       // Y3 / Y4 at m-2
+  R vprev = R(0);
+
+  for (int m = m_start; m <= m_end; ++m) {
+    const int rd = m & 1, wr = rd ^ 1;
+    const C *lss = lssb[rd], *lsq = lsqb[rd];     // state of plane m
+    const C *lq3 = lq3b[rd], *lq4 = lq4b[rd];     // Y3 / Y4 of plane m - 1
+    load_plane(m + 2, pre);         // prefetch for the next iteration
+    // ---- pass-1 at m
+    KG1Plane<R> cur;
+    const C sb = rb.s, qb = rb.q;
+    if (m == 0 || m == nz - 1) {
+      cur.y3 = sb;
+      cur.y4 = sb;
+      cur.z3 = qb;
+      cur.pp = sb;
+      cur.qq = qb;
     } else {
-      nxt.y3 = nxt.y4 = nxt.z3 = nxt.pp = nxt.qq = zero;
+      // LDS neighbours; border lanes take the global one (their LDS index is clamped to
+      // their own slot so no read leaves the array)
+      const int sxp = bxp ? sl : sl + 1, sxm = bxm ? sl : sl - 1;
+      const int syp = byp ? sl : sl + kWaveX, sym = bym ? sl : sl - kWaveX;
+      const C axp = bxp ? rb.xs : lss[sxp], bxp_ = bxp ? rb.xq : lsq[sxp];
+      const C axm = bxm ? rb.xs : lss[sxm], bxm_ = bxm ? rb.xq : lsq[sxm];
+      const C ayp = byp ? rb.ys : lss[syp], byp_ = byp ? rb.yq : lsq[syp];
+      const C aym = bym ? rb.ys : lss[sym], bym_ = bym ? rb.yq : lsq[sym];
+      const C qhm = (m - 1 == 0) ? zero : ra.q;   // q^ : 0 on the frozen planes
+      const C qhp = (m + 1 == nz - 1) ? zero : rc.q;
+      const R v = rb.v;
+      const R g = p.gam[m];
+      const R w = p.m2 - v * v;
+      const C wzp = p.zc[2 * m], wzm = p.zc[2 * m + 1];
+      auto lap = [&](C c, C xp, C xm, C yp, C ym, C zp, C zm) {
+        const C lxx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
+        const C lyy = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
+        const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
+        return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
+      };
+      const C As = csub<R>(lap(sb, axp, axm, ayp, aym, rc.s, ra.s), cscale<R>(sb, w));
+      const C Aq = csub<R>(lap(qb, bxp_, bxm_, byp_, bym_, qhp, qhm), cscale<R>(qb, w));
+      const C f = cadd<R>(As, kg2_damp<R>(v, g, qb));
+      const C u = caxpy<R>(qb, p.h * R(0.5), f);
+      const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
+      cur.y3 = caxpy<R>(sb, p.h * R(0.5), u);
+      cur.z3 = caxpy<R>(qb, p.h * R(0.5), gg);
+      cur.y4 = caxpy<R>(sb, p.h, cur.z3);
+      cur.pp = cadd<R>(sb, cscale<R>(cadd<R>(qb, cscale<R>(cadd<R>(u, cur.z3), R(2))), p.h / R(6)));
+      cur.qq = cadd<R>(qb, cscale<R>(caxpy<R>(f, R(2), gg), p.h / R(6)));
     }
-    const C qn = qb;  // injected pi at k+1
-    // pass-2 at k
-    if (writer) {
+    // ---- pass-2 at k = m-1 (inside the chunk)
+    const int k = m - 1;
+    if (k >= k0 && writer) {
       const size_t idx = (size_t)k * plane + ij;
-      C opsi = cur.pp, opi = cur.qq;
-      if (k > 0 && k < nz - 1) {
+      C opsi = prev.pp, opi = prev.qq;
+      if (k > 0) {  // k < nz - 1 here since m <= nz - 1
         const R g = p.gam[k];
-        const R w = p.m2 - vk * vk;
+        const R w = p.m2 - vprev * vprev;
         const C wzp = p.zc[2 * k], wzm = p.zc[2 * k + 1];
-        auto lap = [&](const C *L, C c, C zp, C zm) {
+        auto lap2 = [&](const C *L, C c, C zp, C zm) {
           const C lxx = cadd<R>(csub<R>(L[sl + 1], c), csub<R>(L[sl - 1], c));
           const C lyy = cadd<R>(csub<R>(L[sl + kWaveX], c), csub<R>(L[sl - kWaveX], c));
           const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
           return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
         };
-        const C k3 = cadd<R>(csub<R>(lap(l3, cur.y3, nxt.y3, y3m), cscale<R>(cur.y3, w)),
-                             kg2_damp<R>(vk, g, cur.z3));
+        const C qk = ra.q;  // injected pi at k = m-1
+        const C k3 = cadd<R>(csub<R>(lap2(lq3, prev.y3, cur.y3, y3mm), cscale<R>(prev.y3, w)),
+                             kg2_damp<R>(vprev, g, prev.z3));
         const C wv = caxpy<R>(qk, p.h, k3);
-        const C k4 = cadd<R>(csub<R>(lap(l4, cur.y4, nxt.y4, y4m), cscale<R>(cur.y4, w)),
-                             kg2_damp<R>(vk, g, wv));
-        opsi = cadd<R>(cur.pp, cscale<R>(wv, p.h / R(6)));
-        opi = cadd<R>(cur.qq, cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6)));
+        const C k4 = cadd<R>(csub<R>(lap2(lq4, prev.y4, cur.y4, y4mm), cscale<R>(prev.y4, w)),
+                             kg2_damp<R>(vprev, g, wv));
+        opsi = cadd<R>(prev.pp, cscale<R>(wv, p.h / R(6)));
+        opi = cadd<R>(prev.qq, cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6)));
       }
       p.out_psi[idx] = opsi;
       p.out_pi[idx] = opi;
     }
-    __syncthreads();  // every lane has read plane k from LDS
-    l3[sl] = nxt.y3;
-    l4[sl] = nxt.y4;
-    __syncthreads();
-    y3m = cur.y3;
-    y4m = cur.y4;
-    cur = nxt;
-    vk = vn;
-    qk = qn;
+    lq3b[wr][sl] = cur.y3;
+    lq4b[wr][sl] = cur.y4;
+    lssb[wr][sl] = rc.s;
+    lsqb[wr][sl] = rc.q;
+    __syncthreads();  // plane m+1 / m visible; every lane is done reading buffer rd
+    y3mm = prev.y3;
+    y4mm = prev.y4;
+    prev = cur;
+    vprev = rb.v;
+    ra = rb;
+    rb = rc;
+    inject_plane(m + 2, pre);
+    rc = pre;
+  }
+  // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
+  if (k1 == nz && writer) {
+    const size_t idx = (size_t)(nz - 1) * plane + ij;
+    p.out_psi[idx] = prev.pp;
+    p.out_pi[idx] = prev.qq;
   }
 }
 
