@@ -64,6 +64,11 @@ extern "C" {
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
 #define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state) */
 
+/* planes k = 0 and nz-1 (never stepped) */
+#define WAP_ENDS_FROZEN 0  /* stage inputs there = the injected state (fdtd_3d.cu:184-190) */
+#define WAP_ENDS_ZERO 1    /* stage inputs there = 0 (2D reference, fdtd.cu:189-198);
+                              Schrodinger, 4-pass schedule only */
+
 #define WAP_MAX_SRC_BAND 8
 
 /*
@@ -82,7 +87,10 @@ typedef struct wap_desc {
                                  KG_FUSED uses 4, 8 or 16 rows (auto 16) */
   int32_t tile_z;             /* planes per thread column; 0 = auto */
   int32_t schedule;           /* WAP_SCHED_*: how a KG step is split into passes */
-  double xmin, xmax, ymin, ymax, zmin, zmax; /* XMIN..ZMAX; spacing = extent/(n-1) */
+  int32_t end_planes;         /* WAP_ENDS_*: planes k = 0, nz-1 */
+  int32_t reserved0;
+  double xmin, xmax, ymin, ymax, zmin, zmax; /* XMIN..ZMAX; spacing = extent/(n-1);
+                                               ny = 1 is a 2D grid (no y term) */
   double dt;                  /* DT */
   double lap_coeff;           /* LAP_COEFF = A (Schrodinger) */
   double e_sim;               /* carrier frequency of the source phase (E_SIM) */

@@ -14,6 +14,12 @@ stage reads a stage input that no thread of the same stage writes, i.e. the
 intended classical RK4.  Planes k = 0 and nz-1 are frozen at the (injected)
 base value, as ``fdtd_3d.cu:184-190`` and ``:268`` make them.
 
+2D (``fdtd.cu``, the reference's ``RelativisticSolver``) is the same
+stepper on a grid with ny = 1 (no y term; the reference's y is z here) and
+``end_planes = "zero"``: the end rows of the stage outputs are 0
+(``fdtd.cu:189-198``) instead of the base value, while the state's end rows
+still keep b (``fdtd.cu:272``).  Atomic units: A = 0.5 (``fdtd.cu:250-252``).
+
 Klein-Gordon (not in the reference; the build's extension): the first-order
 system in (psi, pi = d psi / dt), hbar = 1, time in hbar/eV:
 
@@ -37,9 +43,10 @@ import numpy as np
 
 
 def spacing(p):
-    """``fdtd_3d.cu:31-33``: N-1 intervals on every axis (also periodic x, y)."""
+    """``fdtd_3d.cu:31-33``: N-1 intervals on every axis (also periodic x, y).
+    ny = 1 is the 2D grid of ``fdtd.cu`` (no y term)."""
     dx = (p["xmax"] - p["xmin"]) / (p["nx"] - 1.0)
-    dy = (p["ymax"] - p["ymin"]) / (p["ny"] - 1.0)
+    dy = (p["ymax"] - p["ymin"]) / (p["ny"] - 1.0) if p["ny"] > 1 else np.inf
     dz = (p["zmax"] - p["zmin"]) / (p["nz"] - 1.0)
     return dx, dy, dz
 
@@ -113,7 +120,8 @@ def laplacian(y, p, cp=None, cm=None):
     lap = np.zeros_like(y)
     yc = y[1:-1]
     lap[1:-1] = ((np.roll(yc, -1, axis=2) - yc) + (np.roll(yc, 1, axis=2) - yc)) / (dx * dx)
-    lap[1:-1] += ((np.roll(yc, -1, axis=1) - yc) + (np.roll(yc, 1, axis=1) - yc)) / (dy * dy)
+    if p["ny"] > 1:
+        lap[1:-1] += ((np.roll(yc, -1, axis=1) - yc) + (np.roll(yc, 1, axis=1) - yc)) / (dy * dy)
     lap[1:-1] += (cp[1:-1, None, None] * (y[2:] - yc)
                   - cm[1:-1, None, None] * (yc - y[:-2]))
     return lap
@@ -193,10 +201,16 @@ def inject(state, t, p):
     return (psi, pi)
 
 
-def _axpy(base, c, k):
+def _axpy(base, c, k, zero_ends=False):
+    """A stage input b + c k.  zero_ends: the 2D reference writes 0 into the
+    stage outputs on its end rows (``fdtd.cu:189-198``)."""
     if isinstance(base, tuple):
-        return tuple(b + c * kk for b, kk in zip(base, k))
-    return base + c * k
+        return tuple(_axpy(b, c, kk, zero_ends) for b, kk in zip(base, k))
+    y = base + c * k
+    if zero_ends:
+        y[0] = 0.0
+        y[-1] = 0.0
+    return y
 
 
 def rk4_step(state, V, t, p, cache=None):
@@ -209,10 +223,11 @@ def rk4_step(state, V, t, p, cache=None):
         cache = _cache(p)
     dt = p["dt"]
     b = inject(state, t, p)
+    ze = p.get("end_planes", "frozen") == "zero"
     k1 = rhs(b, V, p, cache)
-    k2 = rhs(_axpy(b, 0.5 * dt, k1), V, p, cache)
-    k3 = rhs(_axpy(b, 0.5 * dt, k2), V, p, cache)
-    k4 = rhs(_axpy(b, dt, k3), V, p, cache)
+    k2 = rhs(_axpy(b, 0.5 * dt, k1, ze), V, p, cache)
+    k3 = rhs(_axpy(b, 0.5 * dt, k2, ze), V, p, cache)
+    k4 = rhs(_axpy(b, dt, k3, ze), V, p, cache)
     if isinstance(b, tuple):
         return tuple(bb + (dt / 6.0) * (a1 + 2.0 * a2 + 2.0 * a3 + a4)
                      for bb, a1, a2, a3, a4 in zip(b, k1, k2, k3, k4))

@@ -185,6 +185,57 @@ def test_klein_gordon_step_equals_p4_of_sparse_system(rng):
     assert np.max(np.abs(got[1] - w_pi)) / np.max(np.abs(w_pi)) < 1e-12
 
 
+def test_2d_zero_end_rows_step_equals_projected_rk4_of_sparse_H(rng):
+    # fdtd.cu: stage outputs are 0 on the end rows (189-198), the final update
+    # skips them (272); a 2D grid is ny = 1 with A = 0.5
+    p = _params(6, 1, 9, src=True, pml_cells=3, k_src=2, A=0.5, end_planes="zero")
+    shape = (p["nz"], 1, p["nx"])
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    V = 3e4 * rng.standard_normal(shape)
+    H = -0.5 * _lap_matrix(p) + sp.diags(V.ravel() * _interior_mask(p))
+    M = -1j * H
+    Z = sp.diags(_interior_mask(p).astype(float))
+    t = 5 * p["dt"]
+    dt = p["dt"]
+    b = fdtd3d.inject(psi, t, p).ravel()
+    k1 = M @ b
+    k2 = M @ (Z @ (b + 0.5 * dt * k1))
+    k3 = M @ (Z @ (b + 0.5 * dt * k2))
+    k4 = M @ (Z @ (b + dt * k3))
+    want = (b + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)).reshape(shape)
+    got = fdtd3d.rk4_step(psi, V, t, p)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-13
+    assert np.array_equal(got[0], b.reshape(shape)[0])
+    # the end rows matter: the frozen-plane rule differs next to them by far
+    # more than the tolerance above (it enters at second order in dt)
+    frozen = fdtd3d.rk4_step(psi, V, t, dict(p, end_planes="frozen"))
+    assert np.max(np.abs(frozen[1] - got[1])) > 1e-9 * np.max(np.abs(got[1]))
+
+
+def test_2d_grid_is_the_y_uniform_3d_grid(rng):
+    # ny = 1 has no y term: a y-uniform state on ny = 2 steps identically
+    p1 = _params(6, 1, 10, k_src=3, A=0.5, end_planes="zero")
+    p2 = dict(p1, ny=2, ymin=-0.025, ymax=0.025)
+    psi = rng.standard_normal((10, 1, 6)) + 1j * rng.standard_normal((10, 1, 6))
+    V = 1e4 * rng.standard_normal((10, 1, 6))
+    a = fdtd3d.run(psi, V, 3, p1)
+    b = fdtd3d.run(np.repeat(psi, 2, axis=1), np.repeat(V, 2, axis=1), 3, p2)
+    assert np.allclose(b[:, 0], a[:, 0], rtol=0, atol=1e-12 * np.max(np.abs(a)))
+    assert np.allclose(b[:, 1], a[:, 0], rtol=0, atol=1e-12 * np.max(np.abs(a)))
+
+
+def test_zero_and_frozen_end_rules_agree_on_zero_end_planes(rng):
+    # with zero end planes and no source there, k and b are 0 on the ends, so
+    # both rules write the same stage outputs
+    p = _params(5, 1, 9, k_src=3, A=0.5)
+    psi = rng.standard_normal((9, 1, 5)) + 1j * rng.standard_normal((9, 1, 5))
+    psi[[0, -1]] = 0
+    V = 1e4 * rng.standard_normal((9, 1, 5))
+    a = fdtd3d.run(psi, V, 4, dict(p, end_planes="zero"))
+    b = fdtd3d.run(psi, V, 4, dict(p, end_planes="frozen"))
+    assert np.array_equal(a, b)
+
+
 def test_source_window_weights_and_phase():
     # fdtd_3d.cu:150-156: s = (k-K)/2 -> weights 0, 0.025, 0.05 on K, K+1, K+2
     p = _params(4, 4, 12, k_src=4)

@@ -33,6 +33,7 @@ class WapDesc(ctypes.Structure):
         ("equation", _c_i32), ("precision", _c_i32), ("time_mode", _c_i32),
         ("src_enabled", _c_i32), ("src_k", _c_i32), ("src_band", _c_i32),
         ("tile_y", _c_i32), ("tile_z", _c_i32), ("schedule", _c_i32),
+        ("end_planes", _c_i32), ("reserved0", _c_i32),
         ("xmin", _c_dbl), ("xmax", _c_dbl), ("ymin", _c_dbl), ("ymax", _c_dbl),
         ("zmin", _c_dbl), ("zmax", _c_dbl),
         ("dt", _c_dbl), ("lap_coeff", _c_dbl), ("e_sim", _c_dbl), ("k_beam", _c_dbl),
@@ -128,6 +129,7 @@ def make_desc(p):
     d.tile_y = int(p.tile_y)
     d.tile_z = int(p.tile_z)
     d.schedule = SCHED[p.schedule]
+    d.end_planes = {"frozen": 0, "zero": 1}[p.end_planes]
     (d.xmin, d.xmax, d.ymin, d.ymax, d.zmin, d.zmax) = [float(v) for v in p.extent]
     d.dt = float(p.dt)
     d.lap_coeff = float(p.lap_coeff)

@@ -1,8 +1,9 @@
-"""The ``fdtd_3d_bin`` command (the reference program's argv / file contract).
+"""The ``fdtd_3d_bin`` command (the reference programs' argv / file contract).
 
-``fdtd_3d_bin N_FRAMES STEPS_PER_FRAME SNAPSHOT_COUNT [-lambda] [PARAMS_FILE]``
-reads ./potential.bin and writes ./snapshots.bin like ``fdtd_3d.cu:275-403``;
-its source is ``csrc/fdtd_3d_main.cpp`` and it calls ``libwapfdtd.so``.
+``fdtd_3d_bin N_FRAMES STEPS_PER_FRAME SNAPSHOT_COUNT [E_KINETIC | -lambda] [PARAMS_FILE]``
+reads ./potential.bin and writes ./snapshots.bin like ``fdtd_3d.cu:275-403``
+(a PARAMS_FILE with an NZ line) or ``fdtd.cu:323-552`` (2D, no NZ line); its
+source is ``csrc/fdtd_3d_main.cpp`` and it calls ``libwapfdtd.so``.
 """
 import os
 import shutil

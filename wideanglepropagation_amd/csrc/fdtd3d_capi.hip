@@ -122,7 +122,7 @@ int grid_1d(size_t n) {
 int build_tables(wap_handle *h) {
   const wap_desc &d = h->d;
   const double dx = (d.xmax - d.xmin) / (d.nx - 1.0);
-  const double dy = (d.ymax - d.ymin) / (d.ny - 1.0);
+  const double dy = d.ny > 1 ? (d.ymax - d.ymin) / (d.ny - 1.0) : 0.0;
   const double dz = (d.zmax - d.zmin) / (d.nz - 1.0);
   const int nz = d.nz;
   std::vector<double> ramp(nz), z(nz);
@@ -143,7 +143,7 @@ int build_tables(wap_handle *h) {
   }
   const double scale = is_kg(h) ? d.hbar_c * d.hbar_c : -d.lap_coeff;
   h->hx = scale / (dx * dx);
-  h->hy = scale / (dy * dy);
+  h->hy = d.ny > 1 ? scale / (dy * dy) : 0.0;  // ny = 1: a 2D grid, no y term
   h->m2 = d.rest_energy * d.rest_energy;
   std::vector<std::complex<double>> zc(2 * (size_t)nz, std::complex<double>(0, 0));
   for (int k = 1; k + 1 < nz; ++k) {
@@ -248,6 +248,7 @@ wap::PassArgs<R> base_args(const wap_handle *h) {
   a.acc_pi = (C *)h->acc_pi;
   a.cmp_psi = (C *)h->cmp_psi;
   a.cmp_pi = (C *)h->cmp_pi;
+  a.zero_ends = h->d.end_planes == WAP_ENDS_ZERO ? 1 : 0;
   a.rec_psi = nullptr;
   a.rec_pi = nullptr;
   return a;
@@ -508,6 +509,7 @@ void choose_shape(wap_handle *h) {
   const wap_desc &d = h->d;
   int ty = d.tile_y > 0 ? d.tile_y : 4;
   if (ty > 8) ty = 8;
+  if (ty > d.ny) ty = d.ny;   // a 2D grid (ny = 1) runs one-wave workgroups
   h->tile_y = ty;
   h->ntx = (d.nx + wap::kWaveX - 1) / wap::kWaveX;
   h->nty = (d.ny + ty - 1) / ty;
@@ -527,8 +529,13 @@ void choose_shape(wap_handle *h) {
 
 int validate(const wap_desc *d) {
   if (!d) return fail(WAP_ERR_ARG, "desc is NULL");
-  if (d->nx < 2 || d->ny < 2 || d->nz < 3)
-    return fail(WAP_ERR_ARG, "grid must be at least 2 x 2 x 3 (nx, ny >= 2; nz >= 3)");
+  if (d->nx < 2 || d->ny < 1 || d->nz < 3)
+    return fail(WAP_ERR_ARG, "grid must be at least 2 x 1 x 3 (nx >= 2, ny >= 1, nz >= 3)");
+  if (d->end_planes != WAP_ENDS_FROZEN && d->end_planes != WAP_ENDS_ZERO)
+    return fail(WAP_ERR_ARG, "unknown end_planes");
+  if (d->end_planes == WAP_ENDS_ZERO &&
+      (d->equation != WAP_EQ_SCHRODINGER || d->precision == WAP_PREC_F32_KAHAN))
+    return fail(WAP_ERR_ARG, "zero end planes are for the f32/f64 Schrodinger stepper");
   if ((int64_t)d->nx * d->ny > (int64_t)1 << 30)
     return fail(WAP_ERR_ARG, "nx*ny exceeds the 32-bit in-plane index range");
   if (d->equation != WAP_EQ_SCHRODINGER && d->equation != WAP_EQ_KLEIN_GORDON)
@@ -537,7 +544,7 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "unknown precision");
   if (d->time_mode != WAP_TIME_EXACT && d->time_mode != WAP_TIME_F32)
     return fail(WAP_ERR_ARG, "unknown time_mode");
-  if (!(d->xmax > d->xmin) || !(d->ymax > d->ymin) || !(d->zmax > d->zmin))
+  if (!(d->xmax > d->xmin) || !(d->zmax > d->zmin) || (d->ny > 1 && !(d->ymax > d->ymin)))
     return fail(WAP_ERR_ARG, "extent must be increasing on every axis");
   if (!(d->dt > 0.0)) return fail(WAP_ERR_ARG, "dt must be > 0");
   if (!(d->pml_thick > 0.0)) return fail(WAP_ERR_ARG, "pml_thick must be > 0");
@@ -1359,6 +1366,8 @@ int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
   *out = nullptr;
   if (slots < 1) return fail(WAP_ERR_ARG, "slots must be >= 1");
   if (is_kahan(h)) return fail(WAP_ERR_ARG, "the adjoint supports f64 and f32 handles");
+  if (h->d.end_planes != WAP_ENDS_FROZEN)
+    return fail(WAP_ERR_ARG, "the adjoint supports frozen end planes");
   wap_adjoint *a = new (std::nothrow) wap_adjoint();
   if (!a) return fail(WAP_ERR_OOM, "host allocation failed");
   a->h = h;

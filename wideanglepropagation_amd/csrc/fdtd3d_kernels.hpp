@@ -90,6 +90,7 @@ struct PassArgs {
   R hx, hy;            // lateral weights
   R m2;                // KG: (m c^2)^2
   R w_acc, w_y;        // stage weights
+  int zero_ends;       // stage outputs y = 0 on k = 0, nz-1 (2D reference)
   int src_k0, src_n;   // source planes src_k0 .. src_k0+src_n-1
   R src_keep[kMaxBand];
   C src_coef[kMaxBand];     // a_b e^{i k (z-zK)}
@@ -250,20 +251,22 @@ __device__ __forceinline__ void rk4_column(const PassArgs<R> &p, int tile, cplx<
       if (EQ == 1) bq = inj<R>(p.st_pi[idx], sk, sk.cq, prof, ij);
     }
 
+    // 2D reference: the stage outputs on the end planes are 0 (the state keeps b)
+    const bool yzero = p.zero_ends && !interior;
     if (STAGE == 1) {
       if (p.rec_psi) {
         p.rec_psi[idx] = b;
         if (EQ == 1) p.rec_pi[idx] = bq;
       }
       p.acc_psi[idx] = KAHAN ? cscale<R>(kpsi, p.w_acc) : caxpy<R>(b, p.w_acc, kpsi);
-      p.out_psi[idx] = caxpy<R>(b, p.w_y, kpsi);
+      p.out_psi[idx] = yzero ? zero : caxpy<R>(b, p.w_y, kpsi);
       if (EQ == 1) {
         p.acc_pi[idx] = KAHAN ? cscale<R>(kpi, p.w_acc) : caxpy<R>(bq, p.w_acc, kpi);
         p.out_pi[idx] = caxpy<R>(bq, p.w_y, kpi);
       }
     } else if (STAGE == 2 || STAGE == 3) {
       p.acc_psi[idx] = caxpy<R>(p.acc_psi[idx], p.w_acc, kpsi);
-      p.out_psi[idx] = caxpy<R>(b, p.w_y, kpsi);
+      p.out_psi[idx] = yzero ? zero : caxpy<R>(b, p.w_y, kpsi);
       if (EQ == 1) {
         p.acc_pi[idx] = caxpy<R>(p.acc_pi[idx], p.w_acc, kpi);
         p.out_pi[idx] = caxpy<R>(bq, p.w_y, kpi);

@@ -48,10 +48,12 @@ from .physics import (SOL_LIGHT, MASS_PART, BOHR_RADIUS_ANGSTROM,  # noqa: F401
                       klein_gordon_params, FDTDParams, DT_DEFAULT)
 
 __all__ = ["SOL_LIGHT", "MASS_PART", "BOHR_RADIUS_ANGSTROM", "calculate_physics_params",
-           "RelativisticSolver3D"]
+           "RelativisticSolver", "RelativisticSolver3D"]
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 _PARAMS_TEMPLATE = os.path.join(_PKG_DIR, "csrc", "fdtd_3d_params.h")
+_PARAMS_TEMPLATE_2D = os.path.join(_PKG_DIR, "csrc", "fdtd_params.h")
+DT_2D = 1e-6   # fdtd.cu DT
 
 
 def _replace_define(content, name, val_str):
@@ -59,8 +61,279 @@ def _replace_define(content, name, val_str):
     return re.sub(fr"(#define\s+{name}\s+).*?$", fr"\g<1>{val_str}", content, flags=re.MULTILINE)
 
 
-class RelativisticSolver3D:
+def _au_constants_f32(wavelength_angstrom):
+    """P_INF, E_sim, k_beam of the 2D program in float (``fdtd.cu`` main, argv[4] < 0:
+    lambda_au = lambda / 0.5291772109f, p = 2 pi / lambda_au, E_sim = 0.5 p^2;
+    inject_source: k_beam = sqrtf(2 E_sim))."""
+    f = np.float32
+    lam_au = f(f(abs(wavelength_angstrom)) / f(0.5291772109))
+    p_inf = f(f(f(2.0) * f(3.14159265359)) / lam_au)
+    e_sim = f(f(f(0.5) * p_inf) * p_inf)
+    k_beam = f(np.sqrt(f(f(2.0) * e_sim)))
+    return float(p_inf), float(e_sim), float(k_beam)
+
+
+class RelativisticSolver:
+    """2D RK4 FDTD solver (``fdtd_solver.py:31-271``, ``fdtd.cu``) on MI355X.
+
+    The 2D grid (ny, nx) is stepped as a 3D grid with one lateral row: y of the
+    reference is the propagation axis (PML, end rows that zero the stage
+    outputs, ``fdtd.cu:189-198``), x is periodic; atomic units with
+    H = -0.5 Lap + V (``fdtd.cu:250-252``) and DT = 1e-6.
+    """
+    _RUN_LABEL = "Running"
+
+    def __init__(self, source_file=None, binary_file=None):
+        if source_file is None:
+            self._tmpdir = tempfile.mkdtemp(prefix="wap_fdtd2d_")
+            source_file = os.path.join(self._tmpdir, "fdtd_params.h")
+            shutil.copyfile(_PARAMS_TEMPLATE_2D, source_file)
+        self.source_file = os.path.abspath(source_file)
+        if binary_file is None:
+            self.binary_file = os.path.join(os.path.dirname(self.source_file),
+                                            "fdtd_open_boundaries")
+        else:
+            self.binary_file = os.path.abspath(binary_file)
+        self.working_dir = os.path.dirname(self.source_file) or "."
+        self.last_run = None
+
+    def _update_macros(self, nx, ny, xmin, xmax, ymin, ymax,
+                       pml_thick=0.5, pml_sigma=80.0, pml_theta=0.785, pml_power=3.0,
+                       j_source=120):
+        """``fdtd_solver.py:54-99``: rewrite the #define values; True if changed."""
+        if not os.path.exists(self.source_file):
+            raise FileNotFoundError(f"Source file not found: {self.source_file}")
+        with open(self.source_file, "r") as f:
+            content = f.read()
+        new = content
+        new = _replace_define(new, "NX", str(int(nx)))
+        new = _replace_define(new, "NY", str(int(ny)))
+        new = _replace_define(new, "XMIN", f"{xmin:.4f}f")
+        new = _replace_define(new, "XMAX", f"{xmax:.4f}f")
+        new = _replace_define(new, "YMIN", f"{ymin:.4f}f")
+        new = _replace_define(new, "YMAX", f"{ymax:.4f}f")
+        new = _replace_define(new, "PML_THICK", f"{pml_thick:.4f}f")
+        new = _replace_define(new, "PML_SIGMA0", f"{pml_sigma:.4f}f")
+        new = _replace_define(new, "PML_THETA", f"{pml_theta:.4f}f")
+        new = _replace_define(new, "PML_POWER", f"{pml_power:.1f}f")
+        new = _replace_define(new, "J_SOURCE", str(int(j_source)))
+        if new != content:
+            print(f"Configuration changed. Updating {os.path.basename(self.source_file)}...")
+            with open(self.source_file, "w") as f:
+                f.write(new)
+            return True
+        return False
+
+    def compile(self):
+        """Make the HIP solver library available (``fdtd_solver.py:101-126``)."""
+        from .build import LIB_PATH, build
+        try:
+            if not os.path.exists(LIB_PATH):
+                print("Compiling HIP solver...")
+                build()
+            from ._lib import lib
+            lib()
+        except Exception as e:  # noqa: BLE001
+            print("Compilation FAILED:")
+            print(e)
+            raise RuntimeError("Build failed")
+
+    def _params2d(self, shape, extent, wavelength_angstrom, pml_thick, pml_sigma, j_source, kw):
+        ny, nx = shape
+        xmin, xmax, ymin, ymax = extent
+        quant = kw.get("reference_quantize", True)
+        pml_theta = kw.get("pml_theta", 0.785)
+        pml_power = kw.get("pml_power", 3.0)
+        if quant:
+            p_inf, e_sim, k_beam = _au_constants_f32(wavelength_angstrom)
+            q = lambda v, fmt: float(np.float32(float(fmt % v)))  # noqa: E731
+            ext = (q(xmin, "%.4f"), q(xmax, "%.4f"), 0.0, 0.0, q(ymin, "%.4f"), q(ymax, "%.4f"))
+            base = dict(dt=float(np.float32(kw.get("dt") or DT_2D)), time_mode="f32",
+                        pml_thick=q(pml_thick, "%.4f"), pml_sigma0=q(pml_sigma, "%.4f"),
+                        pml_theta=q(pml_theta, "%.4f"), pml_power=q(pml_power, "%.1f"),
+                        src_alpha=float(np.float32(0.05)))
+        else:
+            _, p_inf, e_sim = calculate_physics_params(abs(wavelength_angstrom))
+            k_beam = float(np.sqrt(2.0 * e_sim))
+            ext = (xmin, xmax, 0.0, 0.0, ymin, ymax)
+            base = dict(dt=float(kw.get("dt") or DT_2D), pml_thick=pml_thick,
+                        pml_sigma0=pml_sigma, pml_theta=pml_theta, pml_power=pml_power)
+        return FDTDParams(nx=nx, ny=1, nz=ny, extent=ext, lap_coeff=0.5, e_sim=e_sim,
+                          k_beam=k_beam, equation="schrodinger",
+                          precision=kw.get("precision", "f32"), end_planes="zero",
+                          src_enabled=bool(kw.get("source", True)), src_k=int(j_source),
+                          **base)
+
+    def run(self, V_map, wavelength_angstrom, extent,
+            n_frames=600, steps_per_frame=100, snapshot_count=None,
+            pml_thick=0.5, pml_sigma=80.0, j_source=120, **kwargs):
+        """Run the 2D simulation (``fdtd_solver.py:128-242``).
+
+        V_map: (NY, NX) effective potential (Hartree); extent: (xmin, xmax, ymin, ymax)
+        in Bohr; y is the propagation axis.  Returns the reference's dict with float32
+        (ny, nx) phi / psi snapshots, or None on failure.
+        """
+        ny, nx = V_map.shape
+        xmin, xmax, ymin, ymax = extent
+        if snapshot_count is None:
+            snapshot_count = n_frames
+        if "abs_width" in kwargs:
+            pml_thick = kwargs["abs_width"]
+        if "eta_max" in kwargs and kwargs["eta_max"] < 5000:   # fdtd_solver.py:171-180
+            pml_sigma = kwargs["eta_max"]
+        self._update_macros(nx, ny, xmin, xmax, ymin, ymax, pml_thick=pml_thick,
+                            pml_sigma=pml_sigma, pml_theta=kwargs.get("pml_theta", 0.785),
+                            pml_power=kwargs.get("pml_power", 3.0), j_source=j_source)
+        if kwargs.get("backend", "hip") == "cli":
+            return self._run_cli(V_map, wavelength_angstrom, n_frames, steps_per_frame,
+                                 snapshot_count)
+        try:
+            self.compile()
+            params = self._params2d(V_map.shape, extent, wavelength_angstrom, pml_thick,
+                                    pml_sigma, j_source, kwargs)
+            V3 = np.asarray(V_map, dtype=np.float32).reshape(ny, 1, nx)
+            kw = dict(kwargs)
+            if kw.get("psi0") is not None:
+                kw["psi0"] = np.asarray(kw["psi0"]).reshape(ny, 1, nx)
+            out = self._run_inprocess(params, V3, n_frames, steps_per_frame, snapshot_count,
+                                      kw)
+        except RuntimeError as e:
+            print("Simulation Execution Failed:")
+            print(e)
+            return None
+        out["snapshots"] = [(t, phi.reshape(ny, nx), psi.reshape(ny, nx))
+                            for t, phi, psi in out["snapshots"]]
+        if "exit_wave" in out:
+            out["exit_wave"] = out["exit_wave"].reshape(nx)
+        if "state" in out:
+            st = out["state"]
+            out["state"] = (tuple(s.reshape(ny, nx) for s in st) if isinstance(st, tuple)
+                            else st.reshape(ny, nx))
+        return out
+
+    def _run_inprocess(self, params, V_map, n_frames, steps_per_frame, snapshot_count, kw):
+        import torch
+        from .engine import FDTD3D
+
+        V32 = np.ascontiguousarray(np.asarray(V_map).astype(np.float32))
+        sim = FDTD3D(params, V=V32)
+        try:
+            if kw.get("psi0") is not None:
+                sim.set_state(kw["psi0"], kw.get("pi0"))
+            if kw.get("source_profile") is not None:
+                sim.set_source_profile(kw["source_profile"])
+            M = int(snapshot_count)
+            save_start = max(0, n_frames - M)
+            saved = 0
+            snaps = []
+            exit_k = kw.get("exit_plane")
+            acc = None
+            if exit_k is not None:
+                acc = torch.zeros((params.ny, params.nx), dtype=torch.complex128, device="cuda")
+            quant = params.time_mode == "f32"
+            dt32 = np.float32(params.dt)
+            e_sim32 = np.float32(params.e_sim)
+            t_run = 0.0
+            for frame in range(n_frames):
+                torch.cuda.synchronize()
+                t_a = time.perf_counter()
+                sim.step(steps_per_frame)
+                sim.synchronize()
+                t_run += time.perf_counter() - t_a
+                if frame >= save_start and saved < M:
+                    if quant:
+                        t_snap = np.float32(np.float32(frame * steps_per_frame) * dt32)
+                    else:
+                        t_snap = np.float32((frame + 1) * steps_per_frame * params.dt)
+                    if acc is not None:
+                        sim.accumulate_plane(exit_k, float(e_sim32) * float(t_snap), acc)
+                    if not kw.get("no_snapshots", False):
+                        st = sim.get_state(np.complex64)
+                        if isinstance(st, tuple):
+                            st = st[0]
+                        phi = np.ascontiguousarray(st.real)
+                        psi = np.ascontiguousarray(st.imag)
+                        phi.setflags(write=False)
+                        psi.setflags(write=False)
+                        snaps.append((t_snap, phi, psi))
+                    saved += 1
+            out = {"snapshots": snaps, "dt": dt32, "E_sim": e_sim32, "success": True}
+            vox_steps = params.voxels * n_frames * steps_per_frame
+            out["elapsed_s"] = t_run
+            out["gvoxel_steps_per_s"] = vox_steps / t_run / 1e9 if t_run > 0 else float("nan")
+            if acc is not None:
+                out["exit_wave"] = (acc / max(1, saved)).cpu().numpy()
+            if kw.get("return_state", False):
+                out["state"] = sim.get_state(np.complex128)
+            print(f"Loaded {len(snaps)} snapshots.")
+            self.last_run = out
+            return out
+        finally:
+            sim.close()
+
+    def _run_cli(self, V_map, wavelength_angstrom, n_frames, steps_per_frame, snapshot_count):
+        """File-contract path (``fdtd_solver.py:186-242, 405-454``): potential.bin ->
+        the solver command -> snapshots.bin, run in ``working_dir``."""
+        from .cli import ensure_cli_binary
+        source_mtime = os.path.getmtime(self.source_file)
+        binary_mtime = os.path.getmtime(self.binary_file) if os.path.exists(self.binary_file) else 0
+        if not os.path.exists(self.binary_file) or source_mtime > binary_mtime:
+            self.compile()
+            ensure_cli_binary(self.binary_file)
+        self._write_potential(V_map)
+        cmd = [self.binary_file, str(n_frames), str(steps_per_frame), str(snapshot_count),
+               str(-abs(wavelength_angstrom)), self.source_file]
+        if os.path.exists(self.binary_file):
+            os.chmod(self.binary_file, 0o755)
+        print(f"{self._RUN_LABEL}: {' '.join([os.path.basename(c) for c in cmd])}")
+        try:
+            subprocess.run(cmd, check=True, capture_output=True, text=True, cwd=self.working_dir)
+        except subprocess.CalledProcessError as e:
+            print("Simulation Execution Failed:")
+            print(e.stdout)
+            print(e.stderr)
+            return None
+        return self._load_snapshots(os.path.join(self.working_dir, "snapshots.bin"))
+
+    def _write_potential(self, V_map):
+        """``potential.bin`` of the 2D program: int32 [nx, ny] + float32 V
+        (``fdtd_solver.py:198-201``)."""
+        ny, nx = V_map.shape
+        pot_file = os.path.join(self.working_dir, "potential.bin")
+        with open(pot_file, "wb") as f:
+            f.write(np.array([nx, ny], dtype=np.int32).tobytes())
+            f.write(np.asarray(V_map).astype(np.float32).tobytes())
+        return pot_file
+
+    def _load_snapshots(self, filepath):
+        """``fdtd_solver.py:244-271``: header int32 nx, ny, M, float32 dt, E_sim; M records
+        of float32 t, phi[ny,nx], psi[ny,nx]."""
+        if not os.path.exists(filepath):
+            print(f"Error: Output file {filepath} not found.")
+            return None
+        snapshots = []
+        with open(filepath, "rb") as f:
+            head = f.read(20)
+            if len(head) != 20:
+                raise ValueError(f"{filepath}: truncated header")
+            nx, ny, M = np.frombuffer(head[:12], dtype=np.int32)
+            dt, e_sim = np.frombuffer(head[12:], dtype=np.float32)
+            vol = int(nx) * int(ny)
+            for _ in range(int(M)):
+                rec = f.read(4 + 8 * vol)
+                if len(rec) != 4 + 8 * vol:
+                    raise ValueError(f"{filepath}: truncated snapshot record")
+                t = np.frombuffer(rec[:4], dtype=np.float32)[0]
+                phi = np.frombuffer(rec[4:4 + 4 * vol], dtype=np.float32).reshape((ny, nx))
+                psi = np.frombuffer(rec[4 + 4 * vol:], dtype=np.float32).reshape((ny, nx))
+                snapshots.append((t, phi, psi))
+        print(f"Loaded {len(snapshots)} snapshots.")
+        return {"snapshots": snapshots, "dt": dt, "E_sim": e_sim, "success": True}
+
+
+class RelativisticSolver3D(RelativisticSolver):
     """3D RK4 FDTD solver (``fdtd_solver.py:274-484``) on MI355X."""
+    _RUN_LABEL = "Running 3D"
 
     def __init__(self, source_file=None, binary_file=None):
         if source_file is None:
@@ -116,26 +389,6 @@ class RelativisticSolver3D:
                 f.write(new)
             return True
         return False
-
-    # ----------------------------------------------------------- compile
-    def compile(self):
-        """Make the HIP solver library available (``fdtd_solver.py:101-126``).
-
-        Builds ``libwapfdtd.so`` with hipcc if it is missing and loads it;
-        on failure prints the compiler output and raises
-        ``RuntimeError("Build failed")`` like the reference.
-        """
-        from .build import LIB_PATH, build
-        try:
-            if not os.path.exists(LIB_PATH):
-                print("Compiling HIP solver...")
-                build()
-            from ._lib import lib
-            lib()
-        except Exception as e:  # noqa: BLE001
-            print("Compilation FAILED:")
-            print(e)
-            raise RuntimeError("Build failed")
 
     # --------------------------------------------------------------- run
     def _params(self, shape, extent, wavelength_angstrom, pml_thick, pml_sigma, k_source,
@@ -219,66 +472,6 @@ class RelativisticSolver3D:
             print(e)
             return None
 
-    def _run_inprocess(self, params, V_map, n_frames, steps_per_frame, snapshot_count, kw):
-        import torch
-        from .engine import FDTD3D
-
-        V32 = np.ascontiguousarray(np.asarray(V_map).astype(np.float32))
-        sim = FDTD3D(params, V=V32)
-        try:
-            if kw.get("psi0") is not None:
-                sim.set_state(kw["psi0"], kw.get("pi0"))
-            if kw.get("source_profile") is not None:
-                sim.set_source_profile(kw["source_profile"])
-            M = int(snapshot_count)
-            save_start = max(0, n_frames - M)
-            saved = 0
-            snaps = []
-            exit_k = kw.get("exit_plane")
-            acc = None
-            if exit_k is not None:
-                acc = torch.zeros((params.ny, params.nx), dtype=torch.complex128, device="cuda")
-            quant = params.time_mode == "f32"
-            dt32 = np.float32(params.dt)
-            e_sim32 = np.float32(params.e_sim)
-            t_run = 0.0
-            for frame in range(n_frames):
-                torch.cuda.synchronize()
-                t_a = time.perf_counter()
-                sim.step(steps_per_frame)
-                sim.synchronize()
-                t_run += time.perf_counter() - t_a
-                if frame >= save_start and saved < M:
-                    if quant:
-                        t_snap = np.float32(np.float32(frame * steps_per_frame) * dt32)
-                    else:
-                        t_snap = np.float32((frame + 1) * steps_per_frame * params.dt)
-                    if acc is not None:
-                        sim.accumulate_plane(exit_k, float(e_sim32) * float(t_snap), acc)
-                    if not kw.get("no_snapshots", False):
-                        st = sim.get_state(np.complex64)
-                        if isinstance(st, tuple):
-                            st = st[0]
-                        phi = np.ascontiguousarray(st.real)
-                        psi = np.ascontiguousarray(st.imag)
-                        phi.setflags(write=False)
-                        psi.setflags(write=False)
-                        snaps.append((t_snap, phi, psi))
-                    saved += 1
-            out = {"snapshots": snaps, "dt": dt32, "E_sim": e_sim32, "success": True}
-            vox_steps = params.voxels * n_frames * steps_per_frame
-            out["elapsed_s"] = t_run
-            out["gvoxel_steps_per_s"] = vox_steps / t_run / 1e9 if t_run > 0 else float("nan")
-            if acc is not None:
-                out["exit_wave"] = (acc / max(1, saved)).cpu().numpy()
-            if kw.get("return_state", False):
-                out["state"] = sim.get_state(np.complex128)
-            print(f"Loaded {len(snaps)} snapshots.")
-            self.last_run = out
-            return out
-        finally:
-            sim.close()
-
     # ------------------------------------------------------------- files
     def _write_potential(self, V_map):
         """``potential.bin``: int32 [nx, ny, nz] + float32 V (``fdtd_solver.py:416-419``)."""
@@ -288,30 +481,6 @@ class RelativisticSolver3D:
             f.write(np.array([nx, ny, nz], dtype=np.int32).tobytes())
             f.write(np.asarray(V_map).astype(np.float32).tobytes())
         return pot_file
-
-    def _run_cli(self, V_map, wavelength_angstrom, n_frames, steps_per_frame, snapshot_count):
-        """File-contract path (``fdtd_solver.py:405-454``): potential.bin -> fdtd_3d_bin ->
-        snapshots.bin, run in ``working_dir``."""
-        from .cli import ensure_cli_binary
-        source_mtime = os.path.getmtime(self.source_file)
-        binary_mtime = os.path.getmtime(self.binary_file) if os.path.exists(self.binary_file) else 0
-        if not os.path.exists(self.binary_file) or source_mtime > binary_mtime:
-            self.compile()
-            ensure_cli_binary(self.binary_file)
-        self._write_potential(V_map)
-        cmd = [self.binary_file, str(n_frames), str(steps_per_frame), str(snapshot_count),
-               str(-abs(wavelength_angstrom)), self.source_file]
-        if os.path.exists(self.binary_file):
-            os.chmod(self.binary_file, 0o755)
-        print(f"Running 3D: {' '.join([os.path.basename(c) for c in cmd])}")
-        try:
-            subprocess.run(cmd, check=True, capture_output=True, text=True, cwd=self.working_dir)
-        except subprocess.CalledProcessError as e:
-            print("Simulation Execution Failed:")
-            print(e.stdout)
-            print(e.stderr)
-            return None
-        return self._load_snapshots(os.path.join(self.working_dir, "snapshots.bin"))
 
     def _load_snapshots(self, filepath):
         """Parse ``snapshots.bin`` (``fdtd_solver.py:456-484``).

@@ -99,6 +99,7 @@ class FDTDParams:
     tile_y: int = 0
     tile_z: int = 0
     schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused"
+    end_planes: str = "frozen"        # "frozen" (3D reference) | "zero" (2D reference)
 
     def __post_init__(self):
         if self.equation not in EQUATIONS:
@@ -107,6 +108,8 @@ class FDTDParams:
             raise ValueError(f"precision must be one of {PRECISIONS}")
         if self.schedule not in ("auto", "stages", "kg_2pass", "kg_fused"):
             raise ValueError("schedule must be 'auto', 'stages', 'kg_2pass' or 'kg_fused'")
+        if self.end_planes not in ("frozen", "zero"):
+            raise ValueError("end_planes must be 'frozen' or 'zero'")
         if self.time_mode not in ("exact", "f32"):
             raise ValueError("time_mode must be 'exact' or 'f32'")
         self.extent = tuple(float(v) for v in self.extent)
@@ -123,8 +126,8 @@ class FDTDParams:
 
     def spacing(self):
         x0, x1, y0, y1, z0, z1 = self.extent
-        return ((x1 - x0) / (self.nx - 1.0), (y1 - y0) / (self.ny - 1.0),
-                (z1 - z0) / (self.nz - 1.0))
+        dy = (y1 - y0) / (self.ny - 1.0) if self.ny > 1 else float("inf")
+        return ((x1 - x0) / (self.nx - 1.0), dy, (z1 - z0) / (self.nz - 1.0))
 
     def z_coords(self):
         z0, z1 = self.extent[4], self.extent[5]
@@ -157,7 +160,7 @@ class FDTDParams:
             "pml_power": self.pml_power, "kg_gamma_max": self.kg_gamma_max,
             "src_enabled": self.src_enabled, "src_k": self.src_k,
             "src_band": self.src_band, "src_alpha": self.src_alpha,
-            "src_profile": src_profile,
+            "src_profile": src_profile, "end_planes": self.end_planes,
         }
 
 
