@@ -53,6 +53,7 @@ struct wap_handle {
   void *zc = nullptr;
   void *gam = nullptr;
   void *prof = nullptr;
+  void *src_tab = nullptr;  // wap::SrcCoef<R>[kMaxBand]
   double *partial = nullptr;
   double *dsum = nullptr;
   void *staging = nullptr;
@@ -89,7 +90,8 @@ int dev_alloc(wap_handle *h, void **p, size_t bytes) {
 void free_all(wap_handle *h) {
   void **bufs[] = {&h->psi, &h->pi, &h->acc_psi, &h->acc_pi, &h->ya_psi, &h->ya_pi,
                    &h->yb_psi, &h->yb_pi, &h->cmp_psi, &h->cmp_pi, &h->V, &h->zc,
-                   &h->gam, &h->prof, (void **)&h->partial, (void **)&h->dsum, &h->staging};
+                   &h->gam, &h->prof, &h->src_tab, (void **)&h->partial, (void **)&h->dsum,
+                   &h->staging};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -119,6 +121,22 @@ int grid_1d(size_t n) {
 }
 
 // Per-plane coefficients, float64 on the host.
+template <typename R>
+int upload_src_tab(wap_handle *h, std::complex<double> mi_w) {
+  wap::SrcCoef<R> tab[wap::kMaxBand];
+  for (int b = 0; b < wap::kMaxBand; ++b) {
+    const bool on = b < h->nsrc;
+    const std::complex<double> c = on ? h->src_spatial[b] : 0.0;
+    const std::complex<double> cq = mi_w * c;
+    tab[b].keep = on ? (R)h->src_keep[b] : (R)1;
+    tab[b].pad = (R)0;
+    tab[b].c = wap::cplx<R>{(R)c.real(), (R)c.imag()};
+    tab[b].cq = wap::cplx<R>{(R)cq.real(), (R)cq.imag()};
+  }
+  WAP_HIP(hipMemcpy(h->src_tab, tab, sizeof(tab), hipMemcpyHostToDevice));
+  return WAP_OK;
+}
+
 int build_tables(wap_handle *h) {
   const wap_desc &d = h->d;
   const double dx = (d.xmax - d.xmin) / (d.nx - 1.0);
@@ -186,7 +204,9 @@ int build_tables(wap_handle *h) {
     }
     h->nsrc = d.src_band;
   }
-  return WAP_OK;
+  const std::complex<double> mi_w(0.0, -d.e_sim);
+  if (h->rs == 8) return upload_src_tab<double>(h, mi_w);
+  return upload_src_tab<float>(h, mi_w);
 }
 
 double step_time(const wap_handle *h, int64_t step0, int64_t n, double t0) {
@@ -204,20 +224,7 @@ void fill_source(const wap_handle *h, double t, A &a) {
   using R = decltype(a.hx);
   a.src_k0 = h->src_k0;
   a.src_n = h->nsrc;
-  const std::complex<double> mi_w(0.0, -h->d.e_sim);
-  for (int b = 0; b < wap::kMaxBand; ++b) {
-    if (b < h->nsrc) {
-      const std::complex<double> c = h->src_spatial[b];
-      const std::complex<double> cq = mi_w * c;
-      a.src_keep[b] = (R)h->src_keep[b];
-      a.src_coef[b] = wap::cplx<R>{(R)c.real(), (R)c.imag()};
-      a.src_coef_pi[b] = wap::cplx<R>{(R)cq.real(), (R)cq.imag()};
-    } else {
-      a.src_keep[b] = (R)1;
-      a.src_coef[b] = wap::cplx<R>{(R)0, (R)0};
-      a.src_coef_pi[b] = wap::cplx<R>{(R)0, (R)0};
-    }
-  }
+  a.src_tab = (const wap::SrcCoef<R> *)h->src_tab;
   const std::complex<double> ph = std::polar(1.0, -h->d.e_sim * t);
   a.src_ph = wap::cplx<R>{(R)ph.real(), (R)ph.imag()};
 }
@@ -320,10 +327,18 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
 }
 
 // the schedule a handle runs (WAP_SCHED_STAGES, _KG_2PASS or _KG_FUSED)
+// The fused pass addresses one xy plane of a field with a buffer descriptor and
+// 32-bit offsets, and skips lanes with an offset of 2^31: a plane must stay below that.
+bool fused_plane_fits(const wap_desc &d) {
+  const size_t cs = d.precision == WAP_PREC_F64 ? 16 : 8;
+  return (size_t)d.nx * d.ny * cs < ((size_t)1 << 31);
+}
+
 int sched_of(const wap_handle *h) {
   if (!is_kg(h)) return WAP_SCHED_STAGES;
   if (h->d.schedule == WAP_SCHED_AUTO)  // the fused pass spills registers in float64
-    return h->d.precision == WAP_PREC_F32 ? WAP_SCHED_KG_FUSED : WAP_SCHED_KG_2PASS;
+    return h->d.precision == WAP_PREC_F32 && fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED
+                                                                   : WAP_SCHED_KG_2PASS;
   return h->d.schedule;
 }
 
@@ -332,9 +347,16 @@ bool kg_two_pass(const wap_handle *h) {
   return s == WAP_SCHED_KG_2PASS || s == WAP_SCHED_KG_FUSED;
 }
 
-int kg1_rows(const wap_handle *h) {
+// Fused-pass workgroups: tile_y = 4, 8 or 16 waves (default 16), i.e. rows = waves - 1
+// rows of 64 columns (two of them halo rows) plus the side-column wave.
+int kg1_waves(const wap_handle *h) {
   const int ty = h->d.tile_y;
   return (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
+}
+int kg1_ntx(const wap_handle *h) { return (h->d.nx + wap::kWaveX - 1) / wap::kWaveX; }
+int kg1_nty(const wap_handle *h) {
+  const int out_rows = kg1_waves(h) - 3;
+  return (h->d.ny + out_rows - 1) / out_rows;
 }
 
 // The fused pass holds 16 waves per CU (4 per SIMD at ~120 VGPRs), i.e. 16 / rows
@@ -343,13 +365,14 @@ int kg1_rows(const wap_handle *h) {
 // chunks amortise the halo planes, but a last round that is mostly empty costs
 // a whole round.
 int kg1_tile_z(const wap_handle *h) {
-  if (h->d.tile_z > 0) return h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
-  const int rows = kg1_rows(h);
-  const long cols = (long)((h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2)) *
-                    ((h->d.ny + rows - 3) / (rows - 2));
-  const long slots = 256L * (16 / rows);
+  if (h->d.tile_z > 0) {
+    const int tz = h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
+    return tz < wap::kKG1MaxTileZ ? tz : wap::kKG1MaxTileZ;
+  }
+  const long cols = (long)kg1_ntx(h) * kg1_nty(h);
+  const long slots = 256L * (16 / kg1_waves(h));
   const int cand[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-  int best = h->d.nz;
+  int best = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
   double best_cost = 1e300;
   for (int t : cand) {
     const int tz = t < h->d.nz ? t : h->d.nz;
@@ -368,24 +391,26 @@ template <typename R>
 int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                   double *pass_ms) {
   using C = wap::cplx<R>;
-  const int rows = kg1_rows(h);
-  const int ntx = (h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2);
-  const int nty = (h->d.ny + rows - 3) / (rows - 2);
+  const int waves = kg1_waves(h);
+  const int ntx = kg1_ntx(h);
+  const int nty = kg1_nty(h);
   const int tz = kg1_tile_z(h);
   const int ntz = (h->d.nz + tz - 1) / tz;
-  const dim3 block(wap::kWaveX, rows);
+  const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)(ntx * nty * ntz));
 
   wap::KG1Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
   a.V = (const R *)h->V;
-  a.zc = (const C *)h->zc;
   a.gam = (const R *)h->gam;
   a.prof = (const C *)h->prof;
   a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
   a.tile_z = tz;
   a.ntx = ntx; a.nty = nty; a.ntz = ntz;
-  a.hx = (R)h->hx; a.hy = (R)h->hy; a.m2 = (R)h->m2; a.h = (R)h->d.dt;
+  const double dz = (h->d.zmax - h->d.zmin) / (h->d.nz - 1.0);
+  const double hz = h->d.hbar_c * h->d.hbar_c / (dz * dz);  // the zc table's KG weight
+  a.hx = (R)h->hx; a.hy = (R)h->hy; a.hz = (R)hz; a.hd = (R)(2.0 * (h->hx + h->hy + hz));
+  a.m2 = (R)h->m2; a.h = (R)h->d.dt;
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
     a.in_psi = (const C *)h->psi;
@@ -393,10 +418,10 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
     a.out_psi = (C *)h->ya_psi;
     a.out_pi = (C *)h->ya_pi;
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    switch (rows) {
-      case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 4>), grid, block, 0, h->stream, a); break;
-      case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 8>), grid, block, 0, h->stream, a); break;
-      default: hipLaunchKernelGGL((wap::kg1_pass<R, 16>), grid, block, 0, h->stream, a); break;
+    switch (waves) {
+      case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 3>), grid, block, 0, h->stream, a); break;
+      case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 7>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::kg1_pass<R, 15>), grid, block, 0, h->stream, a); break;
     }
     // the new state is in the other buffer
     std::swap(h->psi, h->ya_psi);
@@ -564,6 +589,8 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "the 2-pass and fused schedules are for the Klein-Gordon equation");
   if (d->schedule == WAP_SCHED_KG_FUSED && d->precision == WAP_PREC_F32_KAHAN)
     return fail(WAP_ERR_ARG, "the fused schedule runs f32 and f64 (f32_kahan uses 2 passes)");
+  if (d->schedule == WAP_SCHED_KG_FUSED && !fused_plane_fits(*d))
+    return fail(WAP_ERR_ARG, "the fused schedule needs one xy plane of a field under 2 GiB");
   return WAP_OK;
 }
 
@@ -605,6 +632,7 @@ int wap_create(const wap_desc *desc, wap_handle **out) {
   if (!rc) rc = dev_alloc(h, &h->V, h->n * h->rs);
   if (!rc) rc = dev_alloc(h, &h->zc, 2 * (size_t)desc->nz * h->cs);
   if (!rc) rc = dev_alloc(h, &h->gam, (size_t)desc->nz * h->rs);
+  if (!rc) rc = dev_alloc(h, &h->src_tab, wap::kMaxBand * sizeof(wap::SrcCoef<double>));
   h->n_partial = grid_1d(h->n) < 1024 ? grid_1d(h->n) : 1024;
   if (!rc) rc = dev_alloc(h, (void **)&h->partial, (size_t)h->n_partial * 8);
   if (!rc) rc = dev_alloc(h, (void **)&h->dsum, 8);
@@ -926,13 +954,10 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (sched_of(h) == WAP_SCHED_KG_FUSED) {
-    const int rows = kg1_rows(h);
     const int tz = kg1_tile_z(h);
-    if (tile_y) *tile_y = rows;
+    if (tile_y) *tile_y = kg1_waves(h);
     if (tile_z) *tile_z = tz;
-    if (groups)
-      *groups = (int64_t)((h->d.nx + wap::kWaveX - 3) / (wap::kWaveX - 2)) *
-                ((h->d.ny + rows - 3) / (rows - 2)) * ((h->d.nz + tz - 1) / tz);
+    if (groups) *groups = (int64_t)kg1_ntx(h) * kg1_nty(h) * ((h->d.nz + tz - 1) / tz);
     return WAP_OK;
   }
   if (tile_y) *tile_y = h->tile_y;
@@ -1334,7 +1359,7 @@ int adjoint_step_t(wap_adjoint *ad, int slot) {
   a.hx = (R)h->hx; a.hy = (R)h->hy; a.m2 = (R)h->m2; a.h = (R)h->d.dt;
   a.src_k0 = h->src_k0;
   a.src_n = h->nsrc;
-  for (int q = 0; q < wap::kMaxBand; ++q) a.src_keep[q] = q < h->nsrc ? (R)h->src_keep[q] : (R)1;
+  a.src_tab = (const wap::SrcCoef<R> *)h->src_tab;
   const int q = EQ == 1 ? 1 : 0;
   void *const *kin[4] = {nullptr, ad->kba, ad->kbb, ad->kba};   // by stage S = 4, 3, 2, 1
   void *const *kout[4] = {ad->kba, ad->kbb, ad->kba, nullptr};

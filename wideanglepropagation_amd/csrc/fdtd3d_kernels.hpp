@@ -64,6 +64,48 @@ __device__ __forceinline__ cplx<R> caxpy(cplx<R> a, R s, cplx<R> b) {
   return mkc<R>(a.x + s * b.x, a.y + s * b.y);
 }
 
+// Time-free coefficients of one source plane b (plane src_k0 + b), a small
+// device table filled once per handle: keep = 1 - a_b, c = a_b e^{i k (z_b - z_K)},
+// cq = (-i w) c.  The per-step phasor e^{-i w t} multiplies c and cq in the kernel.
+template <typename R>
+struct SrcCoef {
+  R keep, pad;
+  cplx<R> c, cq;
+};
+
+// Raw buffer access (MUBUF): a descriptor in scalar registers for one plane and a
+// 32-bit per-lane byte offset.  Offsets at or beyond the descriptor's size are
+// dropped by the hardware (loads return 0 and touch no memory, stores write
+// nothing), which replaces per-lane branches around loads and stores.
+constexpr uint32_t kBufSkip = 0x80000000u;
+constexpr int kBufFlags = 0x00020000;  // descriptor word 3 on gfx9-family parts: 32-bit data format
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kBufFlags);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  } else {
+    static_assert(sizeof(T) == 16, "4, 8 or 16 bytes");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
+  if constexpr (sizeof(T) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+  } else {
+    static_assert(sizeof(T) == 16, "8 or 16 bytes");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+  }
+}
+
 // Per-launch arguments of one pass.  Passed by value (kernarg segment).
 template <typename R>
 struct PassArgs {
@@ -92,9 +134,7 @@ struct PassArgs {
   R w_acc, w_y;        // stage weights
   int zero_ends;       // stage outputs y = 0 on k = 0, nz-1 (2D reference)
   int src_k0, src_n;   // source planes src_k0 .. src_k0+src_n-1
-  R src_keep[kMaxBand];
-  C src_coef[kMaxBand];     // a_b e^{i k (z-zK)}
-  C src_coef_pi[kMaxBand];  // (-i w) * src_coef   (KG)
+  const SrcCoef<R> *src_tab;  // src_n entries
   C src_ph;                 // e^{-i w t} of this step (single-grid launches)
 };
 
@@ -130,17 +170,12 @@ __device__ __forceinline__ SrcPlane<R> src_plane(const A &p, int k, cplx<R> ph) 
   s.cq = s.c;
   const int b = k - p.src_k0;
   if (b >= 0 && b < p.src_n) {
+    // k is uniform over the workgroup: one scalar load of the table entry
+    const SrcCoef<R> e = p.src_tab[b];
     s.band = true;
-#pragma unroll
-    for (int q = 0; q < kMaxBand; ++q) {
-      if (q == b) {
-        s.keep = p.src_keep[q];
-        s.c = p.src_coef[q];
-        s.cq = p.src_coef_pi[q];
-      }
-    }
-    s.c = cmul<R>(s.c, ph);
-    s.cq = cmul<R>(s.cq, ph);
+    s.keep = e.keep;
+    s.c = cmul<R>(e.c, ph);
+    s.cq = cmul<R>(e.cq, ph);
   }
   return s;
 }
@@ -374,9 +409,7 @@ struct KG2Args {
   int ntx, nty, ntz;
   R hx, hy, m2, h;
   int src_k0, src_n;
-  R src_keep[kMaxBand];
-  C src_coef[kMaxBand];
-  C src_coef_pi[kMaxBand];
+  const SrcCoef<R> *src_tab;
   C src_ph;
 };
 
@@ -550,18 +583,16 @@ __global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__rest
 //
 // The two stencil sweeps of the 2-pass schedule run in one pass over the
 // grid: each workgroup computes the pass-1 quantities (Y3, Y4, Z3, P, Q) of
-// one plane for its 64 x TY columns, stages Y3 / Y4 in LDS, and applies the
-// pass-2 stencil to the plane below it from LDS (lateral neighbours) and
-// registers (the +-z neighbours of its own column).  Workgroup tiles overlap
-// by one column / row on each side: the border lanes / rows compute pass-1
-// values for the halo and write nothing, the inner (64 - 2) x (TY - 2)
-// columns write psi', pi'.  The state ping-pongs between two buffers (a tile
-// still reads the old state of its neighbours' columns).  HBM bytes per voxel
-// (complex64, float32 V): read psi, pi, V (20) + write psi', pi' (16) = 36 B,
-// plus the halo columns / planes re-read through L2.
+// one plane for its columns, stages Y3 / Y4 in LDS, and applies the pass-2
+// stencil to the plane below it from LDS (lateral neighbours) and registers
+// (the +-z neighbours of its own column).  Workgroup tiles overlap by one
+// column / row on each side (pass-1 values of the halo, nothing written
+// there).  The state ping-pongs between two buffers (a tile still reads the
+// old state of its neighbours' columns).  HBM bytes per voxel (complex64,
+// float32 V): read psi, pi, V (20) + write psi', pi' (16) = 36 B, plus the
+// halo columns / planes re-read through L2.
 constexpr int kKG1MaxRows = 16;
-
-
+constexpr int kKG1MaxTileZ = 256;  // planes per workgroup chunk (the sponge table lives in LDS)
 
 template <typename R>
 struct KG1Args {
@@ -569,222 +600,272 @@ struct KG1Args {
   const C *in_psi, *in_pi;  // state n
   C *out_psi, *out_pi;      // state n + 1 (another buffer)
   const R *V;
-  const C *zc;
   const R *gam;
   const C *prof;
   int nx, ny, nz;
   int tile_z;
   int ntx, nty, ntz;
-  R hx, hy, m2, h;
+  // Laplacian weights (hbar c)^2 / d^2 per axis (the KG operator has no
+  // coordinate stretch, so the z weight is one number) and hd = 2 (hx + hy + hz)
+  R hx, hy, hz, hd, m2, h;
   int src_k0, src_n;
-  R src_keep[kMaxBand];
-  C src_coef[kMaxBand];
-  C src_coef_pi[kMaxBand];
+  const SrcCoef<R> *src_tab;
   C src_ph;
 };
 
 template <typename R>
-struct KG1Plane {  // pass-1 results of one voxel
+struct KG1Plane {  // pass-1 results of one voxel and the operator coefficients there
   cplx<R> y3, y4, z3, pp, qq;
+  R dd;  // hd + m^2 - V^2: A c = hx sx + hy sy + hz sz - dd c
+  R dv;  // -2 V: the pi coefficient is -2 gamma - 2 i V
 };
 
+// A c = c^2 Lap c - (m^2 - V^2) c with sx, sy, sz the sums of the two neighbours
+// of c along each axis.
+template <typename R>
+__device__ __forceinline__ cplx<R> kg1_apply(const KG1Args<R> &p, R dd, cplx<R> c, cplx<R> sx,
+                                             cplx<R> sy, cplx<R> sz) {
+  return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz * sz.x - dd * c.x,
+                p.hx * sx.y + p.hy * sy.y + p.hz * sz.y - dd * c.y);
+}
+
+// (dg + i dv) x = -2 gamma x - 2 i V x
+template <typename R>
+__device__ __forceinline__ cplx<R> kg1_damp(R dg, R dv, cplx<R> x) {
+  return mkc<R>(dg * x.x - dv * x.y, dg * x.y + dv * x.x);
+}
+
+// Workgroup layout: ROWS waves of 64 lanes, one row of 64 columns each (rows 0
+// and ROWS-1 are halo rows: pass-1 only), plus one side wave whose first
+// 2 ROWS lanes take the column just left and just right of the tile in every row
+// (pass-1 only).  The inner 64 x (ROWS-2) columns write psi', pi'.  Tiles are 64
+// columns wide, so a power-of-two nx needs no partial tile.  The haloed LDS
+// planes have 4 extra columns (the side columns and the outer neighbours the
+// side lanes read) and 2 extra rows (the outer neighbours the halo rows read).
+//
 // Loop over the pass-1 planes m of the chunk (plus one below and one above):
-//   top of iteration m: LDS S = injected state of plane m (every lane's column),
-//                       LDS Y = Y3 / Y4 of plane m - 1;
-//   compute pass-1 at m (lateral state from S; the tile's border lanes read
-//   the one neighbour outside the tile from global memory) and pass-2 at m - 1
-//   (lateral Y3 / Y4 from Y); write Y <- plane m, S <- plane m + 1 into the
-//   other buffer of each pair; barrier.
-// Each lane loads its own column one plane ahead (state, V, border neighbours),
-// so global latency overlaps the compute of the current plane.
-template <typename R, int TY>
-__global__ __launch_bounds__(kWaveX * TY) void kg1_pass(const KG1Args<R> p) {
+//   top of iteration m: LDS S = injected state of plane m, LDS Y = Y3 / Y4 of
+//                       plane m - 1;
+//   compute pass-1 at m (lateral state from S) and pass-2 at m - 1 (lateral
+//   Y3 / Y4 from Y); write Y <- plane m, S <- plane m + 1 into the other buffer
+//   of each pair; barrier.
+// Memory: every lane issues the same five buffer loads per plane, two planes
+// ahead (own column psi, pi, V and one outer neighbour psi, pi, skipped by an
+// out-of-range offset on lanes that have none), and the same two buffer
+// stores (skipped off the writer lanes).  With no branch around any vector
+// memory instruction the in-order load counter waits exactly for the plane a
+// step needs, never for the prefetch behind it.  The loop is unrolled by four,
+// so the ring of four plane slots rotates by renaming.
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass(
+    const KG1Args<R> p) {
   using C = cplx<R>;
+  static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
   // double-buffered planes: iteration m reads buffer m & 1 and writes buffer (m + 1) & 1,
   // so one barrier per plane separates every write from the reads that follow it.
-  // Y3 [0, 2 npl), Y4 [2 npl, 4 npl), state psi [4 npl, 6 npl), state pi [6 npl, 8 npl)
-  constexpr int npl = kWaveX * TY;
-  __shared__ C lq3b[2][npl];
-  __shared__ C lq4b[2][npl];
-  __shared__ C lssb[2][npl];
-  __shared__ C lsqb[2][npl];
+  constexpr int pw = kWaveX + 4;  // LDS row pitch: columns -2 .. 65 of the tile
+  constexpr int nps = pw * (ROWS + 2);
+  __shared__ C lq3b[2][nps];  // Y3 (only the tile's columns and side columns are used)
+  __shared__ C lq4b[2][nps];  // Y4
+  __shared__ C lssb[2][nps];  // state psi
+  __shared__ C lsqb[2][nps];  // state pi
+  // sponge rates of the chunk's planes and the source table: uniform reads from LDS
+  // (a uniform global load would be a vector load, and its wait a full drain)
+  __shared__ R lgam[kKG1MaxTileZ + 2];
+  __shared__ SrcCoef<R> lsrc[kMaxBand];
   const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
   const int tz = rest / p.nty;
-  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
+  const bool side = wv == ROWS;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const int gi = tx * (kWaveX - 2) - 1 + lx;
-  const int gj = tyi * (TY - 2) - 1 + ly;
+  // this lane's column: (row r, LDS column c, grid gi, gj); the side wave's lanes
+  // past 2 ROWS repeat the first ones (same loads, same values written)
+  const int sl = lx % (2 * ROWS);
+  const int r = side ? sl % ROWS : wv;
+  const bool right = sl >= ROWS;
+  const int c = side ? (right ? kWaveX + 2 : 1) : lx + 2;
+  const int gi = side ? (right ? tx * kWaveX + kWaveX : tx * kWaveX - 1) : tx * kWaveX + lx;
+  const int gj = tyi * (ROWS - 2) - 1 + r;
   const int i = ((gi % nx) + nx) % nx;
   const int j = ((gj % ny) + ny) % ny;
-  const bool writer = lx >= 1 && lx <= kWaveX - 2 && ly >= 1 && ly <= TY - 2 && gi < nx && gj < ny;
+  const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;  // uniform
+  const bool writer = wave_writes && gi < nx && gj < ny;
   const size_t plane = (size_t)nx * ny;
+  const uint32_t pbc = (uint32_t)(plane * sizeof(C)), pbr = (uint32_t)(plane * sizeof(R));
   const int ij = j * nx + i;
-  // the one neighbour of a border lane that lies outside the tile
-  const bool bxm = lx == 0, bxp = lx == kWaveX - 1, bym = ly == 0, byp = ly == TY - 1;
-  const int ox = bxm ? j * nx + ((i == 0) ? nx - 1 : i - 1)
-                     : j * nx + ((i + 1 == nx) ? 0 : i + 1);
-  const int oy = bym ? ((j == 0) ? ny - 1 : j - 1) * nx + i
-                     : ((j + 1 == ny) ? 0 : j + 1) * nx + i;
-  const bool bx = bxm || bxp, by = bym || byp;
-  const int sl = ly * kWaveX + lx;
+  const int ss = (r + 1) * pw + c;  // LDS slot
+  // the outer neighbour this lane loads for the halo of S: x for side lanes,
+  // y for the halo rows, none for the rest
+  int oo = -1, hs = ss;
+  if (side) {
+    oo = j * nx + (right ? ((i + 1 == nx) ? 0 : i + 1) : ((i == 0) ? nx - 1 : i - 1));
+    hs = right ? ss + 1 : ss - 1;
+  } else if (wv == 0) {
+    oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+    hs = ss - pw;
+  } else if (wv == ROWS - 1) {
+    oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+    hs = ss + pw;
+  }
+  const bool has_o = oo >= 0;
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t ow = writer ? oc : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
   const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
+  const R h = p.h, h2 = p.h * R(0.5), h6 = p.h / R(6);
 
-  // raw loads of one plane for this lane (own column, border neighbours, V)
-  struct Raw {
-    C s, q, xs, xq, ys, yq;
+  struct Raw {  // one plane of this lane: own column and its outer neighbour (0 if none)
+    C s, q, os, oq;
     R v;
   };
-  auto load_plane = [&](int m, Raw &r) {
-    r.s = r.q = r.xs = r.xq = r.ys = r.yq = zero;
-    r.v = R(0);
-    if (m < 0 || m >= nz) return;
-    const size_t b = (size_t)m * plane;
-    r.s = p.in_psi[b + ij];
-    r.q = p.in_pi[b + ij];
-    r.v = p.V[b + ij];
-    if (bx) {
-      r.xs = p.in_psi[b + ox];
-      r.xq = p.in_pi[b + ox];
-    }
-    if (by) {
-      r.ys = p.in_psi[b + oy];
-      r.yq = p.in_pi[b + oy];
+  // Planes outside the grid are never used in arithmetic (the end planes are
+  // frozen), so a valid plane is loaded in their place.
+  auto load_plane = [&](int m, Raw &w) {
+    const size_t b = (size_t)min(max(m, 0), nz - 1) * plane;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in_psi + b, pbc);
+    const __amdgpu_buffer_rsrc_t rq = buf_rsrc(p.in_pi + b, pbc);
+    w.s = buf_ld<C>(rs, oc);
+    w.q = buf_ld<C>(rq, oc);
+    w.v = buf_ld<R>(buf_rsrc(p.V + b, pbr), ov);
+    w.os = buf_ld<C>(rs, ooc);
+    w.oq = buf_ld<C>(rq, ooc);
+  };
+  // the soft source on a band plane (uniform test; the table is in LDS)
+  auto inject_plane = [&](int m, Raw &w) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = lsrc[bnd].keep;
+    sp.c = cmul<R>(lsrc[bnd].c, ph);
+    sp.cq = cmul<R>(lsrc[bnd].cq, ph);
+    w.s = inj<R>(w.s, sp, sp.c, prof, ij);
+    w.q = inj<R>(w.q, sp, sp.cq, prof, ij);
+    if (has_o) {
+      w.os = inj<R>(w.os, sp, sp.c, prof, oo);
+      w.oq = inj<R>(w.oq, sp, sp.cq, prof, oo);
     }
   };
-  auto inject_plane = [&](int m, Raw &r) {
-    if (m < 0 || m >= nz) return;
-    const SrcPlane<R> sp = src_plane<R>(p, m, ph);
-    if (!sp.band) return;
-    r.s = inj<R>(r.s, sp, sp.c, prof, ij);
-    r.q = inj<R>(r.q, sp, sp.cq, prof, ij);
-    if (bx) {
-      r.xs = inj<R>(r.xs, sp, sp.c, prof, ox);
-      r.xq = inj<R>(r.xq, sp, sp.cq, prof, ox);
-    }
-    if (by) {
-      r.ys = inj<R>(r.ys, sp, sp.c, prof, oy);
-      r.yq = inj<R>(r.yq, sp, sp.cq, prof, oy);
+  auto store_state = [&](int b, const Raw &w) {
+    lssb[b][ss] = w.s;
+    lsqb[b][ss] = w.q;
+    if (has_o) {
+      lssb[b][hs] = w.os;
+      lsqb[b][hs] = w.oq;
     }
   };
 
   const int m_start = max(k0 - 1, 0);
   const int m_end = (k1 == nz) ? nz - 1 : k1;  // inclusive
-  // column queue: a = m-1, b = m, c = m+1 (injected); pre = raw m+2
-  Raw ra, rb, rc, pre;
-  load_plane(m_start - 1, ra);
-  load_plane(m_start, rb);
-  load_plane(m_start + 1, rc);
-  inject_plane(m_start - 1, ra);
-  inject_plane(m_start, rb);
-  inject_plane(m_start + 1, rc);
-  lssb[m_start & 1][sl] = rb.s;
-  lsqb[m_start & 1][sl] = rb.q;
+  {
+    const int tid = wv * kWaveX + lx;
+    for (int t = tid; t <= m_end - m_start; t += kWaveX * (ROWS + 1)) lgam[t] = p.gam[m_start + t];
+    if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
+  }
+  Raw r0, r1, r2, r3;
+  load_plane(m_start - 1, r0);
+  load_plane(m_start, r1);
+  load_plane(m_start + 1, r2);
+  __syncthreads();  // lsrc
+  inject_plane(m_start - 1, r0);
+  inject_plane(m_start, r1);
+  store_state(m_start & 1, r1);
   __syncthreads();
 
-  KG1Plane<R> prev;                 // pass-1 results at m-1
+  KG1Plane<R> prev;  // pass-1 results at m-1
   prev.y3 = prev.y4 = prev.z3 = prev.pp = prev.qq = zero;
-  C y3mm = zero, y4mm = zero;       // Y3 / Y4 at m-2
-  R vprev = R(0);
+  prev.dd = prev.dv = R(0);
+  C y3mm = zero, y4mm = zero;  // Y3 / Y4 at m-2
 
-  for (int m = m_start; m <= m_end; ++m) {
+  // one plane: ra, rb hold planes m-1, m (injected), rc plane m+1 (raw), pre receives m+2
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
     const int rd = m & 1, wr = rd ^ 1;
-    const C *lss = lssb[rd], *lsq = lsqb[rd];     // state of plane m
-    const C *lq3 = lq3b[rd], *lq4 = lq4b[rd];     // Y3 / Y4 of plane m - 1
-    load_plane(m + 2, pre);         // prefetch for the next iteration
-    // ---- pass-1 at m
+    const C *lss = lssb[rd], *lsq = lsqb[rd];  // state of plane m
+    const C *lq3 = lq3b[rd], *lq4 = lq4b[rd];  // Y3 / Y4 of plane m - 1
+    load_plane(m + 2, pre);
+    inject_plane(m + 1, rc);
+    // ---- pass-1 at m.  On the frozen end planes the step factors are 0, which
+    // gives y3 = y4 = pp = psi, z3 = qq = pi exactly (all operands are finite).
     KG1Plane<R> cur;
     const C sb = rb.s, qb = rb.q;
-    if (m == 0 || m == nz - 1) {
-      cur.y3 = sb;
-      cur.y4 = sb;
-      cur.z3 = qb;
-      cur.pp = sb;
-      cur.qq = qb;
-    } else {
-      // LDS neighbours; border lanes take the global one (their LDS index is clamped to
-      // their own slot so no read leaves the array)
-      const int sxp = bxp ? sl : sl + 1, sxm = bxm ? sl : sl - 1;
-      const int syp = byp ? sl : sl + kWaveX, sym = bym ? sl : sl - kWaveX;
-      const C axp = bxp ? rb.xs : lss[sxp], bxp_ = bxp ? rb.xq : lsq[sxp];
-      const C axm = bxm ? rb.xs : lss[sxm], bxm_ = bxm ? rb.xq : lsq[sxm];
-      const C ayp = byp ? rb.ys : lss[syp], byp_ = byp ? rb.yq : lsq[syp];
-      const C aym = bym ? rb.ys : lss[sym], bym_ = bym ? rb.yq : lsq[sym];
-      const C qhm = (m - 1 == 0) ? zero : ra.q;   // q^ : 0 on the frozen planes
+    cur.dd = p.hd + (p.m2 - rb.v * rb.v);
+    cur.dv = R(-2) * rb.v;
+    {
+      const bool frozen = m == 0 || m == nz - 1;
+      const R f1 = frozen ? R(0) : h, f2 = frozen ? R(0) : h2, f6 = frozen ? R(0) : h6;
+      const C sx = cadd<R>(lss[ss + 1], lss[ss - 1]), sy = cadd<R>(lss[ss + pw], lss[ss - pw]);
+      const C qx = cadd<R>(lsq[ss + 1], lsq[ss - 1]), qy = cadd<R>(lsq[ss + pw], lsq[ss - pw]);
+      const C qhm = (m - 1 == 0) ? zero : ra.q;  // q^ : 0 on the frozen planes
       const C qhp = (m + 1 == nz - 1) ? zero : rc.q;
-      const R v = rb.v;
-      const R g = p.gam[m];
-      const R w = p.m2 - v * v;
-      const C wzp = p.zc[2 * m], wzm = p.zc[2 * m + 1];
-      auto lap = [&](C c, C xp, C xm, C yp, C ym, C zp, C zm) {
-        const C lxx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
-        const C lyy = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
-        const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
-        return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
-      };
-      const C As = csub<R>(lap(sb, axp, axm, ayp, aym, rc.s, ra.s), cscale<R>(sb, w));
-      const C Aq = csub<R>(lap(qb, bxp_, bxm_, byp_, bym_, qhp, qhm), cscale<R>(qb, w));
-      const C f = cadd<R>(As, kg2_damp<R>(v, g, qb));
-      const C u = caxpy<R>(qb, p.h * R(0.5), f);
-      const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
-      cur.y3 = caxpy<R>(sb, p.h * R(0.5), u);
-      cur.z3 = caxpy<R>(qb, p.h * R(0.5), gg);
-      cur.y4 = caxpy<R>(sb, p.h, cur.z3);
-      cur.pp = cadd<R>(sb, cscale<R>(cadd<R>(qb, cscale<R>(cadd<R>(u, cur.z3), R(2))), p.h / R(6)));
-      cur.qq = cadd<R>(qb, cscale<R>(caxpy<R>(f, R(2), gg), p.h / R(6)));
+      const R dg = R(-2) * lgam[m - m_start];
+      const C As = kg1_apply<R>(p, cur.dd, sb, sx, sy, cadd<R>(rc.s, ra.s));
+      const C Aq = kg1_apply<R>(p, cur.dd, qb, qx, qy, cadd<R>(qhp, qhm));
+      const C f = cadd<R>(As, kg1_damp<R>(dg, cur.dv, qb));
+      const C u = caxpy<R>(qb, f2, f);
+      const C gg = cadd<R>(caxpy<R>(As, f2, Aq), kg1_damp<R>(dg, cur.dv, u));
+      cur.y3 = caxpy<R>(sb, f2, u);
+      cur.z3 = caxpy<R>(qb, f2, gg);
+      cur.y4 = caxpy<R>(sb, f1, cur.z3);
+      cur.pp = caxpy<R>(sb, f6, caxpy<R>(qb, R(2), cadd<R>(u, cur.z3)));
+      cur.qq = caxpy<R>(qb, f6, caxpy<R>(f, R(2), gg));
     }
-    // ---- pass-2 at k = m-1 (inside the chunk)
+    // ---- pass-2 at k = m-1 (stored when inside the chunk); k = 0 is frozen (the
+    // factor 0 leaves psi, pi), and k <= nz - 2 since m <= nz - 1
     const int k = m - 1;
-    if (k >= k0 && writer) {
-      const size_t idx = (size_t)k * plane + ij;
-      C opsi = prev.pp, opi = prev.qq;
-      if (k > 0) {  // k < nz - 1 here since m <= nz - 1
-        const R g = p.gam[k];
-        const R w = p.m2 - vprev * vprev;
-        const C wzp = p.zc[2 * k], wzm = p.zc[2 * k + 1];
-        auto lap2 = [&](const C *L, C c, C zp, C zm) {
-          const C lxx = cadd<R>(csub<R>(L[sl + 1], c), csub<R>(L[sl - 1], c));
-          const C lyy = cadd<R>(csub<R>(L[sl + kWaveX], c), csub<R>(L[sl - kWaveX], c));
-          const C lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
-          return cadd<R>(cadd<R>(cscale<R>(lxx, p.hx), cscale<R>(lyy, p.hy)), lz);
-        };
-        const C qk = ra.q;  // injected pi at k = m-1
-        const C k3 = cadd<R>(csub<R>(lap2(lq3, prev.y3, cur.y3, y3mm), cscale<R>(prev.y3, w)),
-                             kg2_damp<R>(vprev, g, prev.z3));
-        const C wv = caxpy<R>(qk, p.h, k3);
-        const C k4 = cadd<R>(csub<R>(lap2(lq4, prev.y4, cur.y4, y4mm), cscale<R>(prev.y4, w)),
-                             kg2_damp<R>(vprev, g, wv));
-        opsi = cadd<R>(prev.pp, cscale<R>(wv, p.h / R(6)));
-        opi = cadd<R>(prev.qq, cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6)));
-      }
-      p.out_psi[idx] = opsi;
-      p.out_pi[idx] = opi;
+    C opsi = prev.pp, opi = prev.qq;
+    if (wave_writes) {
+      const R g6 = (k == 0) ? R(0) : h6;
+      const R dg = R(-2) * lgam[max(k, m_start) - m_start];
+      const C sx3 = cadd<R>(lq3[ss + 1], lq3[ss - 1]), sy3 = cadd<R>(lq3[ss + pw], lq3[ss - pw]);
+      const C sx4 = cadd<R>(lq4[ss + 1], lq4[ss - 1]), sy4 = cadd<R>(lq4[ss + pw], lq4[ss - pw]);
+      const C qk = ra.q;  // injected pi at k = m-1
+      const C k3 = cadd<R>(kg1_apply<R>(p, prev.dd, prev.y3, sx3, sy3, cadd<R>(cur.y3, y3mm)),
+                           kg1_damp<R>(dg, prev.dv, prev.z3));
+      const C wv3 = caxpy<R>(qk, h, k3);
+      const C k4 = cadd<R>(kg1_apply<R>(p, prev.dd, prev.y4, sx4, sy4, cadd<R>(cur.y4, y4mm)),
+                           kg1_damp<R>(dg, prev.dv, wv3));
+      opsi = caxpy<R>(prev.pp, g6, wv3);
+      opi = caxpy<R>(prev.qq, g6, caxpy<R>(k4, R(2), k3));
     }
-    lq3b[wr][sl] = cur.y3;
-    lq4b[wr][sl] = cur.y4;
-    lssb[wr][sl] = rc.s;
-    lsqb[wr][sl] = rc.q;
+    {
+      const size_t b = (size_t)max(k, 0) * plane;
+      const uint32_t o = (k >= k0) ? ow : kBufSkip;
+      buf_st<C>(buf_rsrc(p.out_psi + b, pbc), o, opsi);
+      buf_st<C>(buf_rsrc(p.out_pi + b, pbc), o, opi);
+    }
+    lq3b[wr][ss] = cur.y3;
+    lq4b[wr][ss] = cur.y4;
+    store_state(wr, rc);
     __syncthreads();  // plane m+1 / m visible; every lane is done reading buffer rd
     y3mm = prev.y3;
     y4mm = prev.y4;
     prev = cur;
-    vprev = rb.v;
-    ra = rb;
-    rb = rc;
-    inject_plane(m + 2, pre);
-    rc = pre;
+  };
+
+  int m = m_start;
+  for (; m + 3 <= m_end; m += 4) {
+    plane_step(m, r0, r1, r2, r3);
+    plane_step(m + 1, r1, r2, r3, r0);
+    plane_step(m + 2, r2, r3, r0, r1);
+    plane_step(m + 3, r3, r0, r1, r2);
   }
+  // the last 0-3 planes continue the same rotation (m is uniform: no divergence)
+  if (m <= m_end) plane_step(m, r0, r1, r2, r3);
+  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0);
+  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1);
   // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
-  if (k1 == nz && writer) {
-    const size_t idx = (size_t)(nz - 1) * plane + ij;
-    p.out_psi[idx] = prev.pp;
-    p.out_pi[idx] = prev.qq;
+  if (k1 == nz) {
+    const size_t b = (size_t)(nz - 1) * plane;
+    buf_st<C>(buf_rsrc(p.out_psi + b, pbc), ow, prev.pp);
+    buf_st<C>(buf_rsrc(p.out_pi + b, pbc), ow, prev.qq);
   }
 }
 
@@ -821,7 +902,7 @@ struct AdjArgs {
   int ntx, nty, ntz;
   R hx, hy, m2, h;
   int src_k0, src_n;
-  R src_keep[kMaxBand];
+  const SrcCoef<R> *src_tab;
 };
 
 template <typename R, int EQ, int S>
@@ -955,11 +1036,7 @@ __global__ __launch_bounds__(512) void adj_pass(const AdjArgs<R> p) {
     } else {
       R keep = R(1);
       const int bnd = k - p.src_k0;
-      if (bnd >= 0 && bnd < p.src_n) {
-#pragma unroll
-        for (int q = 0; q < kMaxBand; ++q)
-          if (q == bnd) keep = p.src_keep[q];
-      }
+      if (bnd >= 0 && bnd < p.src_n) keep = p.src_tab[bnd].keep;
       p.lam_psi[idx] = cscale<R>(cadd<R>(p.acc_psi[idx], yb_s), keep);
       if (EQ == 1) p.lam_pi[idx] = cscale<R>(cadd<R>(p.acc_pi[idx], yb_q), keep);
     }

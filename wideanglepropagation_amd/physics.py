@@ -138,7 +138,9 @@ class FDTDParams:
         if self.equation != "klein_gordon":
             return "stages"
         if self.schedule == "auto":
-            return "kg_fused" if self.precision == "f32" else "kg_2pass"
+            # the fused pass addresses one xy plane of a field with 32-bit offsets
+            fits = self.nx * self.ny * 8 < 2 ** 31
+            return "kg_fused" if self.precision == "f32" and fits else "kg_2pass"
         return self.schedule
 
     def with_(self, **kw):
