@@ -334,6 +334,19 @@ bool fused_plane_fits(const wap_desc &d) {
   return (size_t)d.nx * d.ny * cs < ((size_t)1 << 31);
 }
 
+// (hbar c)^2 / dz^2: the uniform z weight of the Klein-Gordon operator (no stretch)
+double kg_hz(const wap_handle *h) {
+  const double dz = (h->d.zmax - h->d.zmin) / (h->d.nz - 1.0);
+  return h->d.hbar_c * h->d.hbar_c / (dz * dz);
+}
+
+// x = hi + lo with hi the nearest R and lo the nearest R to the remainder
+template <typename R>
+void split_hi_lo(double x, R &hi, R &lo) {
+  hi = (R)x;
+  lo = (R)(x - (double)hi);
+}
+
 int sched_of(const wap_handle *h) {
   if (!is_kg(h)) return WAP_SCHED_STAGES;
   if (h->d.schedule == WAP_SCHED_AUTO)  // the fused pass spills registers in float64
@@ -407,10 +420,11 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
   a.tile_z = tz;
   a.ntx = ntx; a.nty = nty; a.ntz = ntz;
-  const double dz = (h->d.zmax - h->d.zmin) / (h->d.nz - 1.0);
-  const double hz = h->d.hbar_c * h->d.hbar_c / (dz * dz);  // the zc table's KG weight
-  a.hx = (R)h->hx; a.hy = (R)h->hy; a.hz = (R)hz; a.hd = (R)(2.0 * (h->hx + h->hy + hz));
-  a.m2 = (R)h->m2; a.h = (R)h->d.dt;
+  const double hz = kg_hz(h);
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  split_hi_lo(hz, a.hz_hi, a.hz_lo);
+  split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + hz) + h->m2, a.d0_hi, a.d0_lo);
+  a.h = (R)h->d.dt;
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
     a.in_psi = (const C *)h->psi;
@@ -468,7 +482,8 @@ wap::KG2Args<R> kg2_args(const wap_handle *h, int ty = 0) {
   a.ntz = h->ntz;
   a.hx = (R)h->hx;
   a.hy = (R)h->hy;
-  a.m2 = (R)h->m2;
+  split_hi_lo(kg_hz(h), a.hz_hi, a.hz_lo);
+  split_hi_lo(h->m2, a.m2_hi, a.m2_lo);
   a.h = (R)h->d.dt;
   if (ty > 0) {
     a.tile_y = ty;
@@ -1490,7 +1505,7 @@ extern "C" int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, d
   if (bytes < (1 << 20) || reps < 1 || !copy_gbs || !read_gbs)
     return fail(WAP_ERR_ARG, "need bytes >= 1 MiB, reps >= 1 and output pointers");
   const size_t n = (size_t)bytes / 16;
-  float4 *a = nullptr, *b = nullptr;
+  wap::f32x4 *a = nullptr, *b = nullptr;
   float *sink = nullptr;
   WAP_HIP(hipMalloc((void **)&a, n * 16));
   hipError_t e = hipMalloc((void **)&b, n * 16);
@@ -1508,7 +1523,7 @@ extern "C" int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, d
       rc = fail(WAP_ERR_HIP, "wap_stream_bandwidth setup failed");
       break;
     }
-    const dim3 grid(256 * 8 * 8), block(256);
+    const dim3 grid((unsigned)((n + wap::kStreamChunk - 1) / wap::kStreamChunk)), block(256);
     float ms = 0.f;
     hipLaunchKernelGGL(wap::stream_copy, grid, block, 0, nullptr, a, b, n);  // warm-up
     (void)hipEventRecord(e0, nullptr);

@@ -407,7 +407,10 @@ struct KG2Args {
   int nx, ny, nz;
   int tile_y, tile_z;
   int ntx, nty, ntz;
-  R hx, hy, m2, h;
+  // lateral weights; the z weight (uniform: the KG operator has no coordinate
+  // stretch) and m^2 as unevaluated sums hi + lo, so that their float rounding
+  // does not shift the dispersion of every voxel alike
+  R hx, hy, hz_hi, hz_lo, m2_hi, m2_lo, h;
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
@@ -418,12 +421,19 @@ template <typename R>
 __device__ __forceinline__ cplx<R> kg2_lap(const KG2Args<R> &p, int k, cplx<R> c, cplx<R> xp,
                                            cplx<R> xm, cplx<R> yp, cplx<R> ym, cplx<R> zp,
                                            cplx<R> zm) {
-  const cplx<R> wzp = p.zc[2 * k];
-  const cplx<R> wzm = p.zc[2 * k + 1];
+  (void)k;
   const cplx<R> lx = cadd<R>(csub<R>(xp, c), csub<R>(xm, c));
   const cplx<R> ly = cadd<R>(csub<R>(yp, c), csub<R>(ym, c));
-  const cplx<R> lz = csub<R>(cmul<R>(wzp, csub<R>(zp, c)), cmul<R>(wzm, csub<R>(c, zm)));
-  return cadd<R>(cadd<R>(cscale<R>(lx, p.hx), cscale<R>(ly, p.hy)), lz);
+  const cplx<R> lz = cadd<R>(csub<R>(zp, c), csub<R>(zm, c));
+  return cadd<R>(cadd<R>(cscale<R>(lx, p.hx), cscale<R>(ly, p.hy)),
+                 cadd<R>(cscale<R>(lz, p.hz_lo), cscale<R>(lz, p.hz_hi)));
+}
+
+// (m^2 - V^2) c with m^2 = m2_hi + m2_lo
+template <typename R>
+__device__ __forceinline__ cplx<R> kg2_mass(const KG2Args<R> &p, R v, cplx<R> c) {
+  const R w = p.m2_hi - v * v;
+  return mkc<R>(w * c.x + p.m2_lo * c.x, w * c.y + p.m2_lo * c.y);
 }
 
 // D z = (-2 i V - 2 g) z
@@ -484,7 +494,6 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
     const bool interior = (k > 0) && (k < nz - 1);
     const R v = p.V[idx];
     const R g = p.gam[k];
-    const R w = p.m2 - v * v;
     C ax0, ax1, bx0, bx1, ay0, ay1, by0, by1;
     if (interior) {
       fields(k, o_xp, sk, ax0, bx0);
@@ -497,8 +506,8 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
       // q^ == q on interior planes
       const C q = interior ? bc : inj<R>(p.st_pi[idx], sk, sk.cq, prof, ij);
       if (interior) {
-        const C As = csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am), cscale<R>(ac, w));
-        const C Aq = csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm), cscale<R>(bc, w));
+        const C As = csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am), kg2_mass<R>(p, v, ac));
+        const C Aq = csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm), kg2_mass<R>(p, v, bc));
         const C f = cadd<R>(As, kg2_damp<R>(v, g, q));
         const C u = caxpy<R>(q, p.h * R(0.5), f);
         const C gg = cadd<R>(caxpy<R>(As, p.h * R(0.5), Aq), kg2_damp<R>(v, g, u));
@@ -525,11 +534,11 @@ __device__ __forceinline__ void kg2_column(const KG2Args<R> &p, int tile, cplx<R
       if (interior) {
         const C z3 = p.z3[idx];
         const C k3 = cadd<R>(csub<R>(kg2_lap<R>(p, k, ac, ax0, ax1, ay0, ay1, ap, am),
-                                     cscale<R>(ac, w)),
+                                     kg2_mass<R>(p, v, ac)),
                              kg2_damp<R>(v, g, z3));
         const C wv = caxpy<R>(q, p.h, k3);
         const C k4 = cadd<R>(csub<R>(kg2_lap<R>(p, k, bc, bx0, bx1, by0, by1, bp, bm),
-                                     cscale<R>(bc, w)),
+                                     kg2_mass<R>(p, v, bc)),
                              kg2_damp<R>(v, g, wv));
         dpsi = cscale<R>(wv, p.h / R(6));
         dpi = cscale<R>(caxpy<R>(k4, R(2), k3), p.h / R(6));
@@ -606,8 +615,13 @@ struct KG1Args {
   int tile_z;
   int ntx, nty, ntz;
   // Laplacian weights (hbar c)^2 / d^2 per axis (the KG operator has no
-  // coordinate stretch, so the z weight is one number) and hd = 2 (hx + hy + hz)
-  R hx, hy, hz, hd, m2, h;
+  // coordinate stretch, so the z weight is one number) and the constant part of
+  // the diagonal, d0 = 2 (hx + hy + hz) + m^2.  The z weight and d0 are unevaluated
+  // sums hi + lo: at k dz ~ 1 the z term and m^2 are ~1e11 eV^2, and rounding either
+  // coefficient to one float shifts the dispersion of every voxel alike (a
+  // systematic phase error that grows with the path length).  hx, hy stay single
+  // floats: the lateral eigenvalues are small at the lateral wavenumbers.
+  R hx, hy, hz_hi, hz_lo, d0_hi, d0_lo, h;
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
@@ -616,17 +630,19 @@ struct KG1Args {
 template <typename R>
 struct KG1Plane {  // pass-1 results of one voxel and the operator coefficients there
   cplx<R> y3, y4, z3, pp, qq;
-  R dd;  // hd + m^2 - V^2: A c = hx sx + hy sy + hz sz - dd c
+  R dl;  // d0_lo - V^2: A c = hx sx + hy sy + hz sz - d0_hi c - dl c
   R dv;  // -2 V: the pi coefficient is -2 gamma - 2 i V
 };
 
 // A c = c^2 Lap c - (m^2 - V^2) c with sx, sy, sz the sums of the two neighbours
 // of c along each axis.
 template <typename R>
-__device__ __forceinline__ cplx<R> kg1_apply(const KG1Args<R> &p, R dd, cplx<R> c, cplx<R> sx,
+__device__ __forceinline__ cplx<R> kg1_apply(const KG1Args<R> &p, R dl, cplx<R> c, cplx<R> sx,
                                              cplx<R> sy, cplx<R> sz) {
-  return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz * sz.x - dd * c.x,
-                p.hx * sx.y + p.hy * sy.y + p.hz * sz.y - dd * c.y);
+  return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz_lo * sz.x + p.hz_hi * sz.x - p.d0_hi * c.x -
+                    dl * c.x,
+                p.hx * sx.y + p.hy * sy.y + p.hz_lo * sz.y + p.hz_hi * sz.y - p.d0_hi * c.y -
+                    dl * c.y);
 }
 
 // (dg + i dv) x = -2 gamma x - 2 i V x
@@ -782,7 +798,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 
   KG1Plane<R> prev;  // pass-1 results at m-1
   prev.y3 = prev.y4 = prev.z3 = prev.pp = prev.qq = zero;
-  prev.dd = prev.dv = R(0);
+  prev.dl = prev.dv = R(0);
   C y3mm = zero, y4mm = zero;  // Y3 / Y4 at m-2
 
   // one plane: ra, rb hold planes m-1, m (injected), rc plane m+1 (raw), pre receives m+2
@@ -796,7 +812,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     // gives y3 = y4 = pp = psi, z3 = qq = pi exactly (all operands are finite).
     KG1Plane<R> cur;
     const C sb = rb.s, qb = rb.q;
-    cur.dd = p.hd + (p.m2 - rb.v * rb.v);
+    cur.dl = p.d0_lo - rb.v * rb.v;
     cur.dv = R(-2) * rb.v;
     {
       const bool frozen = m == 0 || m == nz - 1;
@@ -806,8 +822,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       const C qhm = (m - 1 == 0) ? zero : ra.q;  // q^ : 0 on the frozen planes
       const C qhp = (m + 1 == nz - 1) ? zero : rc.q;
       const R dg = R(-2) * lgam[m - m_start];
-      const C As = kg1_apply<R>(p, cur.dd, sb, sx, sy, cadd<R>(rc.s, ra.s));
-      const C Aq = kg1_apply<R>(p, cur.dd, qb, qx, qy, cadd<R>(qhp, qhm));
+      const C As = kg1_apply<R>(p, cur.dl, sb, sx, sy, cadd<R>(rc.s, ra.s));
+      const C Aq = kg1_apply<R>(p, cur.dl, qb, qx, qy, cadd<R>(qhp, qhm));
       const C f = cadd<R>(As, kg1_damp<R>(dg, cur.dv, qb));
       const C u = caxpy<R>(qb, f2, f);
       const C gg = cadd<R>(caxpy<R>(As, f2, Aq), kg1_damp<R>(dg, cur.dv, u));
@@ -827,10 +843,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       const C sx3 = cadd<R>(lq3[ss + 1], lq3[ss - 1]), sy3 = cadd<R>(lq3[ss + pw], lq3[ss - pw]);
       const C sx4 = cadd<R>(lq4[ss + 1], lq4[ss - 1]), sy4 = cadd<R>(lq4[ss + pw], lq4[ss - pw]);
       const C qk = ra.q;  // injected pi at k = m-1
-      const C k3 = cadd<R>(kg1_apply<R>(p, prev.dd, prev.y3, sx3, sy3, cadd<R>(cur.y3, y3mm)),
+      const C k3 = cadd<R>(kg1_apply<R>(p, prev.dl, prev.y3, sx3, sy3, cadd<R>(cur.y3, y3mm)),
                            kg1_damp<R>(dg, prev.dv, prev.z3));
       const C wv3 = caxpy<R>(qk, h, k3);
-      const C k4 = cadd<R>(kg1_apply<R>(p, prev.dd, prev.y4, sx4, sy4, cadd<R>(cur.y4, y4mm)),
+      const C k4 = cadd<R>(kg1_apply<R>(p, prev.dl, prev.y4, sx4, sy4, cadd<R>(cur.y4, y4mm)),
                            kg1_damp<R>(dg, prev.dv, wv3));
       opsi = caxpy<R>(prev.pp, g6, wv3);
       opi = caxpy<R>(prev.qq, g6, caxpy<R>(k4, R(2), k3));
@@ -1209,23 +1225,44 @@ __global__ __launch_bounds__(256) void atom_potential_kernel(
 }
 
 // STREAM-style copy / read of n 16-byte words (bandwidth reference for the roofline)
-__global__ __launch_bounds__(256) void stream_copy(const float4 *__restrict__ a,
-                                                   float4 *__restrict__ b, size_t n) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; t < n; t += stride) b[t] = a[t];
+// Reference points of the roofline: the best plain copy / read this device
+// sustains.  Each workgroup streams one contiguous chunk of kStreamChunk float4,
+// four loads in flight per lane, with non-temporal hints (the data is used once).
+constexpr int kStreamChunk = 4096;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void stream_copy(const f32x4 *__restrict__ a,
+                                                   f32x4 *__restrict__ b, size_t n) {
+  const size_t beg = (size_t)blockIdx.x * kStreamChunk;
+  const size_t end = beg + kStreamChunk < n ? beg + kStreamChunk : n;
+  for (size_t t = beg + threadIdx.x; t < end; t += 4 * 256) {
+    f32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t q = t + u * 256;
+      v[u] = q < end ? __builtin_nontemporal_load(a + q) : f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t q = t + u * 256;
+      if (q < end) __builtin_nontemporal_store(v[u], b + q);
+    }
+  }
 }
 
-__global__ __launch_bounds__(256) void stream_read(const float4 *__restrict__ a, size_t n,
+__global__ __launch_bounds__(256) void stream_read(const f32x4 *__restrict__ a, size_t n,
                                                    float *__restrict__ sink) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  float acc = 0.f;
-  for (; t < n; t += stride) {
-    const float4 v = a[t];
-    acc += v.x + v.y + v.z + v.w;
+  const size_t beg = (size_t)blockIdx.x * kStreamChunk;
+  const size_t end = beg + kStreamChunk < n ? beg + kStreamChunk : n;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (size_t t = beg + threadIdx.x; t < end; t += 4 * 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t q = t + u * 256;
+      if (q < end) acc += __builtin_nontemporal_load(a + q);
+    }
   }
-  if (acc == 12345.678f) sink[0] = acc;  // keeps the loads; never true for zero input
+  if (acc.x + acc.y + acc.z + acc.w == 12345.678f) sink[0] = 1.f;  // keeps the loads; never true for zero input
 }
 
 }  // namespace wap
