@@ -113,9 +113,10 @@ def copy_calibration(gib, reps=3):
                       "copy_GBs": 2 * n * 4 / el / 1e9}), flush=True)
 
 
-def cpu_baseline(eq, shape, nsteps=1):
+def cpu_baseline(eq, shape, budget_s=10.0, max_steps=64):
     """FP64 NumPy oracle on a bounded slab of the workload: the same lateral size, 1/8 of
-    the planes, nsteps steps, single-threaded NumPy."""
+    the planes, as many steps as fit in about budget_s seconds (at least one),
+    single-threaded NumPy."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import fdtd3d
     from wideanglepropagation_amd.workloads import kg_slab, schrodinger_gaussian_64
@@ -130,7 +131,10 @@ def cpu_baseline(eq, shape, nsteps=1):
         sub = p.shape
     d = p.oracle_dict()
     t = time.perf_counter()
-    fdtd3d.run(st, V, nsteps, d)
+    nsteps = 0
+    while nsteps < max_steps and (nsteps == 0 or time.perf_counter() - t < budget_s):
+        st = fdtd3d.run(st, V, 1, d, step0=nsteps)
+        nsteps += 1
     el = time.perf_counter() - t
     vox = sub[0] * sub[1] * sub[2]
     try:
