@@ -327,11 +327,11 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
 }
 
 // the schedule a handle runs (WAP_SCHED_STAGES, _KG_2PASS or _KG_FUSED)
-// The fused pass addresses one xy plane of a field with a buffer descriptor and
-// 32-bit offsets, and skips lanes with an offset of 2^31: a plane must stay below that.
+// The fused passes address a field with a buffer descriptor and 32-bit offsets, and
+// skip lanes with an offset of 2^31: a field must stay below 2 GiB.
 bool fused_plane_fits(const wap_desc &d) {
   const size_t cs = d.precision == WAP_PREC_F64 ? 16 : 8;
-  return (size_t)d.nx * d.ny * cs < ((size_t)1 << 31);
+  return (size_t)d.nx * d.ny * d.nz * cs < ((size_t)1 << 31);
 }
 
 // (hbar c)^2 / dz^2: the uniform z weight of the Klein-Gordon operator (no stretch)
@@ -367,38 +367,37 @@ int kg1_waves(const wap_handle *h) {
   return (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
 }
 int kg1_ntx(const wap_handle *h) { return (h->d.nx + wap::kWaveX - 1) / wap::kWaveX; }
-int kg1_nty(const wap_handle *h) {
-  const int out_rows = kg1_waves(h) - 3;
+int kg1_nty(const wap_handle *h, int waves) {
+  const int out_rows = waves - 3;
   return (h->d.ny + out_rows - 1) / out_rows;
 }
+int kg1_nty(const wap_handle *h) { return kg1_nty(h, kg1_waves(h)); }
 
-// The fused pass holds 16 waves per CU (4 per SIMD at ~120 VGPRs), i.e. 16 / rows
-// workgroups per CU on 256 CUs.  Pick the z chunk that minimises
-// (full rounds of workgroups) x (planes per chunk + 2 halo planes): few, long
-// chunks amortise the halo planes, but a last round that is mostly empty costs
-// a whole round.
-int kg1_tile_z(const wap_handle *h) {
+// The fused passes hold per_cu workgroups per CU on 256 CUs.  Pick the z chunk
+// that minimises (full rounds of workgroups) x (planes per chunk + 2 halo planes):
+// few, long chunks amortise the halo planes, but a last round that is mostly empty
+// costs a whole round.
+int kg1_tile_z(const wap_handle *h, int waves, int per_cu) {
   if (h->d.tile_z > 0) {
     const int tz = h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
     return tz < wap::kKG1MaxTileZ ? tz : wap::kKG1MaxTileZ;
   }
-  const long cols = (long)kg1_ntx(h) * kg1_nty(h);
-  const long slots = 256L * (16 / kg1_waves(h));
-  const int cand[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-  int best = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
+  const long cols = (long)kg1_ntx(h) * kg1_nty(h, waves);
+  const long slots = 256L * per_cu;
+  const int tmax = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
+  int best = tmax;
   double best_cost = 1e300;
-  for (int t : cand) {
-    const int tz = t < h->d.nz ? t : h->d.nz;
-    const long groups = cols * ((h->d.nz + tz - 1) / tz);
-    const double cost = (double)((groups + slots - 1) / slots) * (tz + 2);
-    if (cost < best_cost) {
+  for (int tz = 8 < tmax ? 8 : tmax; tz <= tmax; ++tz) {
+    const long chunks = (h->d.nz + tz - 1) / tz;
+    const double cost = (double)((cols * chunks + slots - 1) / slots) * (tz + 2);
+    if (cost < best_cost) {  // ties keep the shorter chunk (more workgroups in flight)
       best_cost = cost;
       best = tz;
     }
-    if (t >= h->d.nz) break;
   }
   return best;
 }
+int kg1_tile_z(const wap_handle *h) { return kg1_tile_z(h, kg1_waves(h), 16 / kg1_waves(h)); }
 
 template <typename R>
 int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
@@ -605,7 +604,7 @@ int validate(const wap_desc *d) {
   if (d->schedule == WAP_SCHED_KG_FUSED && d->precision == WAP_PREC_F32_KAHAN)
     return fail(WAP_ERR_ARG, "the fused schedule runs f32 and f64 (f32_kahan uses 2 passes)");
   if (d->schedule == WAP_SCHED_KG_FUSED && !fused_plane_fits(*d))
-    return fail(WAP_ERR_ARG, "the fused schedule needs one xy plane of a field under 2 GiB");
+    return fail(WAP_ERR_ARG, "the fused schedule needs each field under 2 GiB");
   return WAP_OK;
 }
 
@@ -1294,12 +1293,21 @@ struct wap_adjoint {
   wap_handle *h = nullptr;
   int slots = 0;
   int ncomp = 1;
+  // Klein-Gordon: one fused reverse pass per step from the step's input state
+  bool fused = false;
+  std::vector<void *> st;        // [slot][comp]: the input state u_n of the recorded step
+  std::vector<int64_t> st_step;  // its step index
+  std::vector<double> st_t0;
+  int cur = 0;                   // lam[cur] holds dL/du of the latest step
+  // Schrodinger: the 4-stage records and the stage-by-stage reverse passes
   std::vector<void *> rec;   // [slot][stage 0..3][comp]
-  void *lam[2] = {nullptr, nullptr};
+  void *lam2[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][comp]
   void *acc[2] = {nullptr, nullptr};
   void *kba[2] = {nullptr, nullptr};
   void *kbb[2] = {nullptr, nullptr};
   double *vbar = nullptr;
+  void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type
+  void *lam(int c) const { return lam2[cur][c]; }
 };
 
 namespace {
@@ -1307,8 +1315,11 @@ namespace {
 void adjoint_free(wap_adjoint *a) {
   for (void *&p : a->rec)
     if (p) { (void)hipFree(p); p = nullptr; }
-  void **bufs[] = {&a->lam[0], &a->lam[1], &a->acc[0], &a->acc[1], &a->kba[0], &a->kba[1],
-                   &a->kbb[0], &a->kbb[1], (void **)&a->vbar};
+  for (void *&p : a->st)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  void **bufs[] = {&a->lam2[0][0], &a->lam2[0][1], &a->lam2[1][0], &a->lam2[1][1],
+                   &a->acc[0], &a->acc[1], &a->kba[0], &a->kba[1],
+                   &a->kbb[0], &a->kbb[1], (void **)&a->vbar, &a->vacc};
   for (void **b : bufs)
     if (*b) { (void)hipFree(*b); *b = nullptr; }
 }
@@ -1360,8 +1371,8 @@ int adjoint_step_t(wap_adjoint *ad, int slot) {
   const dim3 grid((unsigned)(h->ntx * h->nty * h->ntz));
   wap::AdjArgs<R> a;
   std::memset((void *)&a, 0, sizeof(a));
-  a.lam_psi = (C *)ad->lam[0];
-  a.lam_pi = (C *)ad->lam[1];
+  a.lam_psi = (C *)ad->lam(0);
+  a.lam_pi = (C *)ad->lam(1);
   a.acc_psi = (C *)ad->acc[0];
   a.acc_pi = (C *)ad->acc[1];
   a.vbar = ad->vbar;
@@ -1397,6 +1408,77 @@ int adjoint_step_t(wap_adjoint *ad, int slot) {
   return WAP_OK;
 }
 
+// Fused Klein-Gordon adjoint workgroups: the handle's tile_y (16 or 8 waves; 16 by
+// default), at most 8 for double (its LDS planes are twice as large)
+template <typename R>
+int kg1_adj_waves(const wap_handle *h) {
+  const int w = kg1_waves(h) >= 16 ? 16 : 8;
+  return sizeof(R) == 4 ? w : 8;
+}
+
+// Record step `step`: keep its input state in `slot`, then advance the handle.  The
+// fused forward pass ping-pongs its state buffers, so the input state is already in
+// the handle's spare buffer afterwards and changes owner with the slot's (no copy).
+int adjoint_record_fused(wap_adjoint *ad, int64_t step, double t0, int slot) {
+  wap_handle *h = ad->h;
+  void **sp = &ad->st[(size_t)slot * 2];
+  int rc;
+  if (sched_of(h) == WAP_SCHED_KG_FUSED) {
+    rc = is_f64(h) ? run_steps_kg1<double>(h, 1, step, t0, nullptr, nullptr)
+                   : run_steps_kg1<float>(h, 1, step, t0, nullptr, nullptr);
+    if (rc) return rc;
+    std::swap(h->ya_psi, sp[0]);
+    std::swap(h->ya_pi, sp[1]);
+  } else {
+    const size_t fb = h->n * h->cs;
+    WAP_HIP(hipMemcpyAsync(sp[0], h->psi, fb, hipMemcpyDeviceToDevice, h->stream));
+    WAP_HIP(hipMemcpyAsync(sp[1], h->pi, fb, hipMemcpyDeviceToDevice, h->stream));
+    rc = dispatch_steps(h, 1, step, t0, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  ad->st_step[slot] = step;
+  ad->st_t0[slot] = t0;
+  return WAP_OK;
+}
+
+template <typename R>
+int adjoint_step_fused(wap_adjoint *ad, int slot) {
+  using C = wap::cplx<R>;
+  wap_handle *h = ad->h;
+  const int waves = kg1_adj_waves<R>(h);
+  const int ntx = kg1_ntx(h), nty = kg1_nty(h, waves);
+  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  const int ntz = (h->d.nz + tz - 1) / tz;
+  wap::KG1AdjArgs<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.u_psi = (const C *)ad->st[(size_t)slot * 2];
+  a.u_pi = (const C *)ad->st[(size_t)slot * 2 + 1];
+  a.lam_in_psi = (const C *)ad->lam2[ad->cur][0];
+  a.lam_in_pi = (const C *)ad->lam2[ad->cur][1];
+  a.lam_out_psi = (C *)ad->lam2[ad->cur ^ 1][0];
+  a.lam_out_pi = (C *)ad->lam2[ad->cur ^ 1][1];
+  a.vbar = (R *)ad->vacc;
+  a.V = (const R *)h->V;
+  a.gam = (const R *)h->gam;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = ntx; a.nty = nty; a.ntz = ntz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy; a.hz = (R)kg_hz(h);
+  split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + (double)a.hz) + h->m2, a.d0_hi, a.d0_lo);
+  a.h = (R)h->d.dt;
+  fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
+  const dim3 block(wap::kWaveX, waves);
+  const dim3 grid((unsigned)(ntx * nty * ntz));
+  if (sizeof(R) == 4 && waves == 16)
+    hipLaunchKernelGGL((wap::kg1_adj_pass<R, sizeof(R) == 4 ? 15 : 7>), grid, block, 0, h->stream, a);
+  else
+    hipLaunchKernelGGL((wap::kg1_adj_pass<R, 7>), grid, block, 0, h->stream, a);
+  WAP_HIP(hipGetLastError());
+  ad->cur ^= 1;
+  return WAP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1413,18 +1495,35 @@ int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
   a->h = h;
   a->slots = slots;
   a->ncomp = is_kg(h) ? 2 : 1;
+  a->fused = is_kg(h) && fused_plane_fits(h->d);
   const size_t fb = h->n * h->cs;
-  a->rec.assign((size_t)slots * 4 * a->ncomp, nullptr);
   int rc = WAP_OK;
-  for (size_t t = 0; t < a->rec.size() && !rc; ++t) {
-    hipError_t e = hipMalloc(&a->rec[t], fb);
-    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint records: ") + hipGetErrorString(e));
-  }
-  for (int c = 0; c < a->ncomp && !rc; ++c) {
-    void **bufs[] = {&a->lam[c], &a->acc[c], &a->kba[c], &a->kbb[c]};
-    for (void **b : bufs) {
-      hipError_t e = hipMalloc(b, fb);
-      if (e != hipSuccess) { rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e)); break; }
+  auto alloc = [&](void **b) {
+    if (rc) return;
+    hipError_t e = hipMalloc(b, fb);
+    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
+  };
+  if (a->fused) {
+    a->st.assign((size_t)slots * 2, nullptr);
+    a->st_step.assign(slots, 0);
+    a->st_t0.assign(slots, 0.0);
+    for (void *&b : a->st) alloc(&b);
+    for (int c = 0; c < 2; ++c) {
+      alloc(&a->lam2[0][c]);
+      alloc(&a->lam2[1][c]);
+    }
+    if (!rc) {
+      hipError_t e = hipMalloc(&a->vacc, h->n * h->rs);
+      if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
+    }
+  } else {
+    a->rec.assign((size_t)slots * 4 * a->ncomp, nullptr);
+    for (void *&b : a->rec) alloc(&b);
+    for (int c = 0; c < a->ncomp; ++c) {
+      alloc(&a->lam2[0][c]);
+      alloc(&a->acc[c]);
+      alloc(&a->kba[c]);
+      alloc(&a->kbb[c]);
     }
   }
   if (!rc) {
@@ -1451,8 +1550,9 @@ int wap_adjoint_destroy(wap_adjoint *a) {
 int wap_adjoint_reset(wap_adjoint *a) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   const size_t fb = a->h->n * a->h->cs;
-  for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam[c], 0, fb, a->h->stream));
+  for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam(c), 0, fb, a->h->stream));
   WAP_HIP(hipMemsetAsync(a->vbar, 0, a->h->n * sizeof(double), a->h->stream));
+  if (a->vacc) WAP_HIP(hipMemsetAsync(a->vacc, 0, a->h->n * a->h->rs, a->h->stream));
   return WAP_OK;
 }
 
@@ -1460,6 +1560,7 @@ int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
   if (!a->h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  if (a->fused) return adjoint_record_fused(a, step, t0, slot);
   const bool kg = is_kg(a->h);
   if (is_f64(a->h))
     return kg ? adjoint_record_t<double, 1>(a, step, t0, slot) : adjoint_record_t<double, 0>(a, step, t0, slot);
@@ -1469,6 +1570,7 @@ int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
 int wap_adjoint_step(wap_adjoint *a, int slot) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
+  if (a->fused) return is_f64(a->h) ? adjoint_step_fused<double>(a, slot) : adjoint_step_fused<float>(a, slot);
   const bool kg = is_kg(a->h);
   if (is_f64(a->h)) return kg ? adjoint_step_t<double, 1>(a, slot) : adjoint_step_t<double, 0>(a, slot);
   return kg ? adjoint_step_t<float, 1>(a, slot) : adjoint_step_t<float, 0>(a, slot);
@@ -1482,10 +1584,10 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
   const dim3 grid((n + 255) / 256), block(256);
   const size_t off = (size_t)k * h->plane;
   if (is_f64(h))
-    hipLaunchKernelGGL(wap::seed_plane<double>, grid, block, 0, h->stream, (double2 *)a->lam[0],
+    hipLaunchKernelGGL(wap::seed_plane<double>, grid, block, 0, h->stream, (double2 *)a->lam(0),
                        off, n, (const double2 *)g, c_re, c_im);
   else
-    hipLaunchKernelGGL(wap::seed_plane<float>, grid, block, 0, h->stream, (float2 *)a->lam[0], off,
+    hipLaunchKernelGGL(wap::seed_plane<float>, grid, block, 0, h->stream, (float2 *)a->lam(0), off,
                        n, (const double2 *)g, c_re, c_im);
   WAP_HIP(hipGetLastError());
   return WAP_OK;
@@ -1493,6 +1595,17 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
 
 int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device) {
   if (!a || !out) return fail(WAP_ERR_ARG, "NULL argument");
+  if (a->vacc) {  // the fused path accumulates in the handle's real type
+    const size_t n = a->h->n;
+    const dim3 grid(grid_1d(n)), block(256);
+    if (is_f64(a->h))
+      hipLaunchKernelGGL((wap::import_real<double, double>), grid, block, 0, a->h->stream,
+                         (const double *)a->vacc, a->vbar, n);
+    else
+      hipLaunchKernelGGL((wap::import_real<float, double>), grid, block, 0, a->h->stream,
+                         (const float *)a->vacc, a->vbar, n);
+    WAP_HIP(hipGetLastError());
+  }
   WAP_HIP(hipMemcpyAsync(out, a->vbar, a->h->n * sizeof(double),
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->h->stream));
   WAP_HIP(hipStreamSynchronize(a->h->stream));

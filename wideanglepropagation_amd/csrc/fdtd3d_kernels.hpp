@@ -73,36 +73,41 @@ struct SrcCoef {
   cplx<R> c, cq;
 };
 
-// Raw buffer access (MUBUF): a descriptor in scalar registers for one plane and a
-// 32-bit per-lane byte offset.  Offsets at or beyond the descriptor's size are
-// dropped by the hardware (loads return 0 and touch no memory, stores write
-// nothing), which replaces per-lane branches around loads and stores.
+// Raw buffer access (MUBUF): a descriptor in scalar registers for one whole field,
+// a uniform byte offset (the plane, in a scalar register) and a 32-bit per-lane byte
+// offset.  A lane offset of kBufSkip lies beyond the descriptor's range, and the
+// hardware drops such an access (a load returns 0 and touches no memory, a store
+// writes nothing), which replaces per-lane branches around loads and stores.  The
+// host runs these kernels only for fields under 2 GiB, so every real offset is in range.
 constexpr uint32_t kBufSkip = 0x80000000u;
+constexpr int kBufRange = 0x7fffffff;
 constexpr int kBufFlags = 0x00020000;  // descriptor word 3 on gfx9-family parts: 32-bit data format
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kBufFlags);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, kBufRange, kBufFlags);
 }
 template <typename T>
-__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
   if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
   } else if constexpr (sizeof(T) == 8) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
   } else {
     static_assert(sizeof(T) == 16, "4, 8 or 16 bytes");
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
   }
 }
 template <typename T>
-__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
-  if constexpr (sizeof(T) == 8) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, T v) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, soff, 0);
+  } else if constexpr (sizeof(T) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, soff, 0);
   } else {
-    static_assert(sizeof(T) == 16, "8 or 16 bytes");
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+    static_assert(sizeof(T) == 16, "4, 8 or 16 bytes");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, soff, 0);
   }
 }
 
@@ -710,8 +715,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int j = ((gj % ny) + ny) % ny;
   const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;  // uniform
   const bool writer = wave_writes && gi < nx && gj < ny;
-  const size_t plane = (size_t)nx * ny;
-  const uint32_t pbc = (uint32_t)(plane * sizeof(C)), pbr = (uint32_t)(plane * sizeof(R));
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rsp = buf_rsrc(p.in_psi), rsq = buf_rsrc(p.in_pi);
+  const __amdgpu_buffer_rsrc_t rsv = buf_rsrc(p.V);
+  const __amdgpu_buffer_rsrc_t rop = buf_rsrc(p.out_psi), roq = buf_rsrc(p.out_pi);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;  // LDS slot
   // the outer neighbour this lane loads for the halo of S: x for side lanes,
@@ -745,14 +752,13 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // Planes outside the grid are never used in arithmetic (the end planes are
   // frozen), so a valid plane is loaded in their place.
   auto load_plane = [&](int m, Raw &w) {
-    const size_t b = (size_t)min(max(m, 0), nz - 1) * plane;
-    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in_psi + b, pbc);
-    const __amdgpu_buffer_rsrc_t rq = buf_rsrc(p.in_pi + b, pbc);
-    w.s = buf_ld<C>(rs, oc);
-    w.q = buf_ld<C>(rq, oc);
-    w.v = buf_ld<R>(buf_rsrc(p.V + b, pbr), ov);
-    w.os = buf_ld<C>(rs, ooc);
-    w.oq = buf_ld<C>(rq, ooc);
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    w.s = buf_ld<C>(rsp, oc, bc);
+    w.q = buf_ld<C>(rsq, oc, bc);
+    w.v = buf_ld<R>(rsv, ov, b * (uint32_t)sizeof(R));
+    w.os = buf_ld<C>(rsp, ooc, bc);
+    w.oq = buf_ld<C>(rsq, ooc, bc);
   };
   // the soft source on a band plane (uniform test; the table is in LDS)
   auto inject_plane = [&](int m, Raw &w) {
@@ -852,10 +858,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       opi = caxpy<R>(prev.qq, g6, caxpy<R>(k4, R(2), k3));
     }
     {
-      const size_t b = (size_t)max(k, 0) * plane;
+      const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
-      buf_st<C>(buf_rsrc(p.out_psi + b, pbc), o, opsi);
-      buf_st<C>(buf_rsrc(p.out_pi + b, pbc), o, opi);
+      buf_st<C>(rop, o, bc, opsi);
+      buf_st<C>(roq, o, bc, opi);
     }
     lq3b[wr][ss] = cur.y3;
     lq4b[wr][ss] = cur.y4;
@@ -879,9 +885,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1);
   // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
   if (k1 == nz) {
-    const size_t b = (size_t)(nz - 1) * plane;
-    buf_st<C>(buf_rsrc(p.out_psi + b, pbc), ow, prev.pp);
-    buf_st<C>(buf_rsrc(p.out_pi + b, pbc), ow, prev.qq);
+    const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
+    buf_st<C>(rop, ow, bc, prev.pp);
+    buf_st<C>(roq, ow, bc, prev.qq);
   }
 }
 
@@ -1060,6 +1066,377 @@ __global__ __launch_bounds__(512) void adj_pass(const AdjArgs<R> p) {
     sc = sp;
     xs_c = xs_p;
     xq_c = xq_p;
+  }
+}
+
+// ------------------------------------- Klein-Gordon adjoint step in 1 fused pass
+//
+// Reverse of one KG RK4 step n: lam = dL/du_{n+1} in, dL/du_n out, dL/dV += the
+// step's term, given the step's input state u_n (gradients of real L packed as
+// d/dRe + i d/dIm).  M(x_s, x_q) = (x_q, A x_s + D x_q) on interior rows and 0 on the
+// frozen end planes, A = c^2 Lap - (m^2 - V^2) (real, symmetric between interior
+// voxels), D = -2 gamma - 2 i V, x^ = x restricted to interior planes,
+// M^H x = (A^T x_q^, x_s^ + D* x_q^).  The stage adjoints of the RK4 step
+//   kb4 = h/6 lam, kb3 = h/3 lam + h M^H kb4, kb2 = h/3 lam + h/2 M^H kb3,
+//   kb1 = h/6 lam + h/2 M^H kb2,  lam_n = K (lam + M^H (kb4 + kb3 + kb2 + kb1))
+// (K: the soft source's keep factor) need A^T at most twice, like the forward step:
+//   pointwise  a4 = kb4_q^ = (h/6 lam_q)^,  a3 = kb3_q^ = (h/3 lam_q + h Yb4_q)^,
+//              Yb4_q = h/6 (lam_s + D* lam_q)
+//   sweep 1    T4 = A^T a4, T3 = A^T a3,
+//              Yb3_q = h/3 lam_s + h T4 + D* a3,   a2 = kb2_q^ = (h/3 lam_q + h/2 Yb3_q)^,
+//              Yb2_q = h/3 lam_s + h/2 T3 + D* a2, a1 = kb1_q^ = (h/6 lam_q + h/2 Yb2_q)^
+//   sweep 2    T2 = A^T a2, T1 = A^T a1,
+//              lam_n,s = K (lam_s + T4 + T3 + T2 + T1),
+//              lam_n,q = K (lam_q + Yb4_q + Yb3_q + Yb2_q + D* a1 + h/6 lam_s + h/2 T2)
+//   (on a frozen plane only lam_q and the T terms remain).
+// dL/dV += sum_S Re(conj(kb_S,q) (2 V Y_S,s - 2 i Y_S,q)) on interior voxels, with the
+// forward stage inputs Y_1..Y_4 of step n recomputed from u_n as in kg1_pass (Y_4,q =
+// b_q + h (A y3 + D z3) needs the forward's second sweep).  On a frozen plane A^T x^
+// is hz times x at the adjacent interior plane, which the shared stencil gives since
+// x^ is 0 on frozen planes (and on the planes loaded in place of out-of-grid ones).
+// The layout, loads and barriers follow kg1_pass; the last chunk runs one extra,
+// empty plane m = nz so that the top frozen plane gets its sweep-2 values.
+template <typename R>
+struct KG1AdjArgs {
+  using C = cplx<R>;
+  const C *u_psi, *u_pi;          // the step's input state u_n
+  const C *lam_in_psi, *lam_in_pi;  // dL/du_{n+1}
+  C *lam_out_psi, *lam_out_pi;      // dL/du_n (another buffer)
+  R *vbar;                          // dL/dV accumulator (the handle's real type)
+  const R *V;
+  const R *gam;
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  // hz: the z weight rounded once (the gradient does not propagate a systematic
+  // dispersion shift); d0 = 2 (hx + hy + hz) + m^2 as d0_hi + d0_lo
+  R hx, hy, hz, d0_hi, d0_lo, h;
+  int src_k0, src_n;
+  const SrcCoef<R> *src_tab;
+  C src_ph;
+};
+
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_adj_pass(
+    const KG1AdjArgs<R> p) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
+  constexpr int pw = kWaveX + 4;
+  constexpr int nps = pw * (ROWS + 2);
+  __shared__ C lssb[2][nps];  // forward state psi (injected)
+  __shared__ C lsqb[2][nps];  // forward state pi
+  __shared__ C la4b[2][nps];  // a4
+  __shared__ C la3b[2][nps];  // a3
+  __shared__ C ly3b[2][nps];  // forward Y3 (tile and side columns)
+  __shared__ C la2b[2][nps];  // a2
+  __shared__ C la1b[2][nps];  // a1
+  __shared__ R lgam[kKG1MaxTileZ + 4];
+  __shared__ SrcCoef<R> lsrc[kMaxBand];
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const bool side = wv == ROWS;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const int sl = lx % (2 * ROWS);
+  const int r = side ? sl % ROWS : wv;
+  const bool right = sl >= ROWS;
+  const int c = side ? (right ? kWaveX + 2 : 1) : lx + 2;
+  const int gi = side ? (right ? tx * kWaveX + kWaveX : tx * kWaveX - 1) : tx * kWaveX + lx;
+  const int gj = tyi * (ROWS - 2) - 1 + r;
+  const int i = ((gi % nx) + nx) % nx;
+  const int j = ((gj % ny) + ny) % ny;
+  const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
+  const bool writer = wave_writes && gi < nx && gj < ny;
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rup = buf_rsrc(p.u_psi), ruq = buf_rsrc(p.u_pi);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V);
+  const __amdgpu_buffer_rsrc_t rlp = buf_rsrc(p.lam_in_psi), rlq = buf_rsrc(p.lam_in_pi);
+  const __amdgpu_buffer_rsrc_t rop = buf_rsrc(p.lam_out_psi), roq = buf_rsrc(p.lam_out_pi);
+  const __amdgpu_buffer_rsrc_t rvb = buf_rsrc(p.vbar);
+  const int ij = j * nx + i;
+  const int ss = (r + 1) * pw + c;
+  int oo = -1, hs = ss;
+  if (side) {
+    oo = j * nx + (right ? ((i + 1 == nx) ? 0 : i + 1) : ((i == 0) ? nx - 1 : i - 1));
+    hs = right ? ss + 1 : ss - 1;
+  } else if (wv == 0) {
+    oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+    hs = ss - pw;
+  } else if (wv == ROWS - 1) {
+    oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+    hs = ss + pw;
+  }
+  const bool has_o = oo >= 0;
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t oor = has_o ? (uint32_t)oo * sizeof(R) : kBufSkip;
+  const uint32_t ow = writer ? oc : kBufSkip;
+  const uint32_t owd = writer ? (uint32_t)ij * sizeof(R) : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C ph = p.src_ph;
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+  const R h = p.h, h2 = p.h * R(0.5), h3 = p.h / R(3), h6 = p.h / R(6);
+
+  const int m_start = max(k0 - 1, 0);
+  const int m_end = (k1 == nz) ? nz : k1;  // inclusive; m = nz is the empty plane above the grid
+  const int ngam = m_end + 2 - m_start;   // planes m_start .. m_end + 1
+  {
+    const int tid = wv * kWaveX + lx;
+    for (int t = tid; t < ngam; t += kWaveX * (ROWS + 1)) lgam[t] = p.gam[min(m_start + t, nz - 1)];
+    if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
+  }
+  auto interior = [&](int m) { return m > 0 && m < nz - 1; };
+  auto gam_of = [&](int m) { return lgam[min(max(m - m_start, 0), ngam - 1)]; };
+  auto keep_of = [&](int m) {
+    const int bnd = m - p.src_k0;
+    return (bnd >= 0 && bnd < p.src_n && m < nz) ? lsrc[bnd].keep : R(1);
+  };
+
+  struct Raw {  // own column of one plane: state (injected once prepared), V, lam
+    C s, q, ls, lq;
+    R v;
+  };
+  struct Halo {  // a border lane's outer neighbour of one plane
+    C s, q, ls, lq;
+    R v;
+  };
+  auto load_plane = [&](int m, Raw &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    w.s = buf_ld<C>(rup, oc, bc);
+    w.q = buf_ld<C>(ruq, oc, bc);
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    w.ls = buf_ld<C>(rlp, oc, bc);
+    w.lq = buf_ld<C>(rlq, oc, bc);
+  };
+  auto load_halo = [&](int m, Halo &o) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    o.s = buf_ld<C>(rup, ooc, bc);
+    o.q = buf_ld<C>(ruq, ooc, bc);
+    o.v = buf_ld<R>(rv, oor, b * (uint32_t)sizeof(R));
+    o.ls = buf_ld<C>(rlp, ooc, bc);
+    o.lq = buf_ld<C>(rlq, ooc, bc);
+  };
+  // the soft source of step n on a band plane (forward state only)
+  auto inject = [&](int m, C &xs, C &xq, int at) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = lsrc[bnd].keep;
+    sp.c = cmul<R>(lsrc[bnd].c, ph);
+    sp.cq = cmul<R>(lsrc[bnd].cq, ph);
+    xs = inj<R>(xs, sp, sp.c, prof, at);
+    xq = inj<R>(xq, sp, sp.cq, prof, at);
+  };
+  // pointwise a4, a3 of plane m (0 off the interior)
+  auto point_a = [&](int m, C ls, C lq, R v, C &a4, C &a3) {
+    if (!interior(m)) {
+      a4 = a3 = zero;
+      return;
+    }
+    const R dg = R(-2) * gam_of(m), dv = R(2) * v;  // D* = -2 gamma + 2 i V
+    const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp<R>(dg, dv, lq)), h6);
+    a4 = cscale<R>(lq, h6);
+    a3 = caxpy<R>(cscale<R>(lq, h3), h, yb4q);
+  };
+  // plane m into LDS buffer b: own column (prepared) and the outer neighbour
+  auto store_plane = [&](int b, int m, C s_, C q_, C a4, C a3, Halo &o) {
+    lssb[b][ss] = s_;
+    lsqb[b][ss] = q_;
+    la4b[b][ss] = a4;
+    la3b[b][ss] = a3;
+    if (has_o) {
+      inject(m, o.s, o.q, oo);
+      C oa4, oa3;
+      point_a(m, o.ls, o.lq, o.v, oa4, oa3);
+      lssb[b][hs] = o.s;
+      lsqb[b][hs] = o.q;
+      la4b[b][hs] = oa4;
+      la3b[b][hs] = oa3;
+    }
+  };
+  // A x with the operator coefficient dl = d0_lo - V^2 of the center voxel
+  auto apply = [&](R dl, C x, C sx, C sy, C sz) {
+    return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz * sz.x - p.d0_hi * x.x - dl * x.x,
+                  p.hx * sx.y + p.hy * sy.y + p.hz * sz.y - p.d0_hi * x.y - dl * x.y);
+  };
+  auto lat = [&](const C *L, C &sx, C &sy) {
+    sx = cadd<R>(L[ss + 1], L[ss - 1]);
+    sy = cadd<R>(L[ss + pw], L[ss - pw]);
+  };
+  // sum_S Re(conj(k_S) ys_S) and sum_S Re(conj(k_S) (-i) yq_S): the gradient term is
+  // 2 V (first) + 2 (second)
+  auto dot = [](C k, C y, R acc) { return acc + k.x * y.x + k.y * y.y; };
+  auto crs = [](C k, C y, R acc) { return acc + k.x * y.y - k.y * y.x; };
+
+  Raw r0, r1, r2, r3;
+  Halo hn;
+  load_plane(m_start, r1);
+  load_plane(m_start + 1, r2);
+  load_halo(m_start, hn);
+  __syncthreads();  // lgam, lsrc
+  // planes m_start - 1 (for the z- neighbours of the first pass-1) and m_start in LDS;
+  // the plane below is only read at this lane's own slot
+  {
+    Raw w;
+    load_plane(m_start - 1, w);
+    inject(m_start - 1, w.s, w.q, ij);
+    C a4, a3;
+    point_a(m_start - 1, w.ls, w.lq, w.v, a4, a3);
+    const int b = (m_start - 1) & 1;
+    lssb[b][ss] = w.s;
+    lsqb[b][ss] = w.q;
+    la4b[b][ss] = a4;
+    la3b[b][ss] = a3;
+    // y3, a2, a1 of plane m_start - 1 (the Y values of plane p live in buffer (p + 1) & 1;
+    // read as the z- neighbours of the first stored pass-2 when m_start = 0: the plane
+    // below the grid, 0)
+    ly3b[m_start & 1][ss] = zero;
+    la2b[m_start & 1][ss] = zero;
+    la1b[m_start & 1][ss] = zero;
+  }
+  inject(m_start, r1.s, r1.q, ij);
+  {
+    C a4, a3;
+    point_a(m_start, r1.ls, r1.lq, r1.v, a4, a3);
+    store_plane(m_start & 1, m_start, r1.s, r1.q, a4, a3, hn);
+  }
+  __syncthreads();
+
+  struct Pass1 {  // pass-1 results of one voxel that pass-2 of the same plane needs
+    C z3, ps, pq;   // (its y3, a2, a1, q and a4 are read back from the LDS planes)
+    R dl, dv, g1;
+  };
+  Pass1 prev;
+  prev.z3 = prev.ps = prev.pq = zero;
+  prev.dl = prev.dv = prev.g1 = R(0);
+  R vb = R(0);  // vbar at the plane of the next pass-2
+
+  // rb holds plane m (lam, V used), rc plane m+1 (raw), pre receives plane m+2.  The
+  // values of planes m and m-1 (and the Y planes m-1, m-2) that this lane needs at its
+  // own column are read back from its own LDS slots: buffer rd holds plane m (state,
+  // a4, a3) and m-1 (y3, a2, a1); buffer wr still holds plane m-1 (state, a4, a3) and
+  // m-2 (y3, a2, a1) until this lane overwrites its slot at the end of the iteration.
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
+    (void)ra;
+    const int rd = m & 1, wr = rd ^ 1;
+    // loads: vbar for the next iteration's pass-2 plane, the outer neighbours of plane
+    // m+1, then the prefetch of plane m+2 (older loads are the ones waited for)
+    const R vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
+                                (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+    load_halo(m + 1, hn);
+    load_plane(m + 2, pre);
+    inject(m + 1, rc.s, rc.q, ij);
+    C ca4, ca3;
+    point_a(m + 1, rc.ls, rc.lq, rc.v, ca4, ca3);
+    // ---- pass-1 at m (forward sweep 1 and adjoint sweep 1)
+    Pass1 cur;
+    C cy3, ca2, ca1;  // this plane's y3, a2, a1 (LDS and the z+ neighbours of pass-2)
+    const bool in_m = interior(m);
+    const R v = rb.v;
+    cur.dl = p.d0_lo - v * v;
+    cur.dv = R(-2) * v;
+    const C qm = lsqb[wr][ss], a4m = la4b[wr][ss];  // plane m-1 (also pass-2's center)
+    {
+      const C s = lssb[rd][ss], q = lsqb[rd][ss];
+      const C a4 = la4b[rd][ss], a3 = la3b[rd][ss];
+      const C sm = lssb[wr][ss], a3m = la3b[wr][ss];
+      const R f1 = in_m ? h : R(0), f2 = in_m ? h2 : R(0);
+      const R dg = R(-2) * gam_of(m);
+      C sx, sy, qx, qy;
+      lat(lssb[rd], sx, sy);
+      lat(lsqb[rd], qx, qy);
+      const C qhm = (m - 1 == 0) ? zero : qm;
+      const C qhp = (m + 1 == nz - 1) ? zero : rc.q;
+      // forward
+      const C As = apply(cur.dl, s, sx, sy, cadd<R>(rc.s, sm));
+      const C Aq = apply(cur.dl, q, qx, qy, cadd<R>(qhp, qhm));
+      const C f = cadd<R>(As, kg1_damp<R>(dg, cur.dv, q));
+      const C u = caxpy<R>(q, f2, f);
+      const C gg = cadd<R>(caxpy<R>(As, f2, Aq), kg1_damp<R>(dg, cur.dv, u));
+      cy3 = caxpy<R>(s, f2, u);
+      cur.z3 = caxpy<R>(q, f2, gg);
+      const C y4 = caxpy<R>(s, f1, cur.z3);
+      // adjoint
+      C bx, by;
+      lat(la4b[rd], bx, by);
+      const C T4 = apply(cur.dl, a4, bx, by, cadd<R>(ca4, a4m));
+      lat(la3b[rd], bx, by);
+      const C T3 = apply(cur.dl, a3, bx, by, cadd<R>(ca3, a3m));
+      const R dvs = R(2) * v;  // D* = -2 gamma + 2 i V
+      const C ls = rb.ls, lq = rb.lq;
+      if (in_m) {
+        const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp<R>(dg, dvs, lq)), h6);
+        const C yb3q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h, T4), kg1_damp<R>(dg, dvs, a3));
+        ca2 = caxpy<R>(cscale<R>(lq, h3), h2, yb3q);
+        const C yb2q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h2, T3), kg1_damp<R>(dg, dvs, ca2));
+        ca1 = caxpy<R>(cscale<R>(lq, h6), h2, yb2q);
+        cur.ps = cadd<R>(ls, cadd<R>(T4, T3));
+        cur.pq = cadd<R>(cadd<R>(lq, yb4q), cadd<R>(cadd<R>(yb3q, yb2q),
+                                                     caxpy<R>(kg1_damp<R>(dg, dvs, ca1), h6, ls)));
+        const C y2s = caxpy<R>(s, h2, q);
+        const R sd = dot(a4, y4, dot(a3, cy3, dot(ca2, y2s, dot(ca1, s, R(0)))));
+        const R sc = crs(a3, cur.z3, crs(ca2, u, crs(ca1, q, R(0))));
+        cur.g1 = R(2) * (v * sd + sc);
+      } else {
+        ca2 = ca1 = zero;
+        cur.ps = cadd<R>(ls, cadd<R>(T4, T3));
+        cur.pq = lq;
+        cur.g1 = R(0);
+        if (m >= nz) cy3 = zero;  // the empty plane above the grid
+      }
+    }
+    // ---- pass-2 at k = m-1 (stored when inside the chunk)
+    const int k = m - 1;
+    C lns = prev.ps, lnq = prev.pq;
+    R g = R(0);
+    if (wave_writes) {
+      const bool in_k = interior(k);
+      const R dg = R(-2) * gam_of(k);
+      C sx, sy;
+      lat(ly3b[rd], sx, sy);
+      const C k3 = cadd<R>(apply(prev.dl, ly3b[rd][ss], sx, sy, cadd<R>(cy3, ly3b[wr][ss])),
+                           kg1_damp<R>(dg, prev.dv, prev.z3));
+      const C wv4 = caxpy<R>(qm, h, k3);
+      lat(la2b[rd], sx, sy);
+      const C T2 = apply(prev.dl, la2b[rd][ss], sx, sy, cadd<R>(ca2, la2b[wr][ss]));
+      lat(la1b[rd], sx, sy);
+      const C T1 = apply(prev.dl, la1b[rd][ss], sx, sy, cadd<R>(ca1, la1b[wr][ss]));
+      const R kp = keep_of(k);
+      lns = cscale<R>(cadd<R>(prev.ps, cadd<R>(T2, T1)), kp);
+      lnq = cscale<R>(caxpy<R>(prev.pq, in_k ? h2 : R(0), T2), kp);
+      if (in_k) g = prev.g1 + R(2) * (a4m.x * wv4.y - a4m.y * wv4.x);
+    }
+    {
+      const uint32_t b = (uint32_t)max(k, 0) * plane;
+      const uint32_t o = (k >= k0) ? ow : kBufSkip;
+      buf_st<C>(rop, o, b * (uint32_t)sizeof(C), lns);
+      buf_st<C>(roq, o, b * (uint32_t)sizeof(C), lnq);
+      buf_st<R>(rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);
+    }
+    vb = vb_next;
+    ly3b[wr][ss] = cy3;
+    la2b[wr][ss] = ca2;
+    la1b[wr][ss] = ca1;
+    store_plane(wr, m + 1, rc.s, rc.q, ca4, ca3, hn);
+    __syncthreads();
+    prev = cur;
+  };
+
+  // a plain loop: the 4-slot unrolled ring of the forward pass spills here
+  for (int m = m_start; m <= m_end; ++m) {
+    plane_step(m, r0, r1, r2, r3);
+    r1 = r2;
+    r2 = r3;
   }
 }
 

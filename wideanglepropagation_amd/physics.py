@@ -138,8 +138,8 @@ class FDTDParams:
         if self.equation != "klein_gordon":
             return "stages"
         if self.schedule == "auto":
-            # the fused pass addresses one xy plane of a field with 32-bit offsets
-            fits = self.nx * self.ny * 8 < 2 ** 31
+            # the fused pass addresses a field with 32-bit offsets
+            fits = self.nx * self.ny * self.nz * 8 < 2 ** 31
             return "kg_fused" if self.precision == "f32" and fits else "kg_2pass"
         return self.schedule
 

@@ -199,8 +199,7 @@ def adjoint_study(quick=False, n_fd=32):
     half = ext[5] - 20 * spacing_z
     pos, Zs, scr = random_slab(150, ((ext[0], ext[1]), (ext[2], ext[3])), (-half, half), 2)
     pert = pos + np.random.default_rng(3).normal(0.0, 0.05, pos.shape)
-    p32, V = kg_slab(shape, precision="f32", device="cuda", positions=(pos, Zs, scr),
-                     schedule="kg_2pass")
+    p32, V = kg_slab(shape, precision="f32", device="cuda", positions=(pos, Zs, scr))
     # dt = 1e-6 (stable: max |w| dt ~ 0.9): 1500 steps carry the wave front across the
     # 1.1 A between the source and the exit plane (at 5e-7 it would not yet arrive)
     p32 = p32.with_(dt=1e-6)
