@@ -63,6 +63,7 @@ struct wap_handle {
   int tile_y = 4, tile_z = 8;
   int ntx = 1, nty = 1, ntz = 1;
   double hx = 0, hy = 0, m2 = 0;
+  int gz_lo = 1, gz_hi = 0;  // the longest run of planes with gamma = 0 (empty if lo > hi)
   int nsrc = 0;
   int src_k0 = 0;
   double src_keep[wap::kMaxBand] = {};
@@ -172,6 +173,15 @@ int build_tables(wap_handle *h) {
   if (is_kg(h))
     for (int k = 0; k < nz; ++k)
       if (ramp[k] > 0.0) gam[k] = d.kg_gamma_max * std::pow(ramp[k], d.pml_power);
+  h->gz_lo = 1;
+  h->gz_hi = 0;
+  for (int k = 0, run = 0; k < nz; ++k) {
+    run = gam[k] == 0.0 ? run + 1 : 0;
+    if (run > h->gz_hi - h->gz_lo + 1) {
+      h->gz_lo = k - run + 1;
+      h->gz_hi = k;
+    }
+  }
 
   // upload in the stepper's precision
   if (is_f64(h)) {
@@ -1467,6 +1477,8 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   a.hx = (R)h->hx; a.hy = (R)h->hy; a.hz = (R)kg_hz(h);
   split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + (double)a.hz) + h->m2, a.d0_hi, a.d0_lo);
   a.h = (R)h->d.dt;
+  a.gz_lo = h->gz_lo;
+  a.gz_hi = h->gz_hi;
   fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)(ntx * nty * ntz));

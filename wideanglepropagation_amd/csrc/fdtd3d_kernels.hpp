@@ -30,6 +30,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
+
 namespace wap {
 
 constexpr int kMaxBand = 8;
@@ -655,6 +658,12 @@ template <typename R>
 __device__ __forceinline__ cplx<R> kg1_damp(R dg, R dv, cplx<R> x) {
   return mkc<R>(dg * x.x - dv * x.y, dg * x.y + dv * x.x);
 }
+// the same with dg known to be 0 when DG is false (planes outside the sponge)
+template <bool DG, typename R>
+__device__ __forceinline__ cplx<R> kg1_damp_t(R dg, R dv, cplx<R> x) {
+  if constexpr (DG) return kg1_damp<R>(dg, dv, x);
+  return mkc<R>(-dv * x.y, dv * x.x);
+}
 
 // Workgroup layout: ROWS waves of 64 lanes, one row of 64 columns each (rows 0
 // and ROWS-1 are halo rows: pass-1 only), plus one side wave whose first
@@ -1112,6 +1121,7 @@ struct KG1AdjArgs {
   // hz: the z weight rounded once (the gradient does not propagate a systematic
   // dispersion shift); d0 = 2 (hx + hy + hz) + m^2 as d0_hi + d0_lo
   R hx, hy, hz, d0_hi, d0_lo, h;
+  int gz_lo, gz_hi;  // gamma is 0 on planes gz_lo..gz_hi (none if gz_lo > gz_hi)
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
@@ -1133,6 +1143,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   __shared__ C la1b[2][nps];  // a1
   __shared__ R lgam[kKG1MaxTileZ + 4];
   __shared__ SrcCoef<R> lsrc[kMaxBand];
+  // V and lam of the plane of the next pass-1, each lane's own slot (written and read
+  // by the same lane only, so one buffer and no barrier)
+  __shared__ C lols[(ROWS + 1) * kWaveX], lolq[(ROWS + 1) * kWaveX];
+  __shared__ R lov[(ROWS + 1) * kWaveX];
   const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
@@ -1140,6 +1154,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int tz = rest / p.nty;
   const int lx = threadIdx.x;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int own = wv * kWaveX + lx;
   const bool side = wv == ROWS;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const int sl = lx % (2 * ROWS);
@@ -1237,19 +1252,20 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     xs = inj<R>(xs, sp, sp.c, prof, at);
     xq = inj<R>(xq, sp, sp.cq, prof, at);
   };
-  // pointwise a4, a3 of plane m (0 off the interior)
-  auto point_a = [&](int m, C ls, C lq, R v, C &a4, C &a3) {
+  // pointwise a4, a3 of plane m (0 off the interior); DG: gamma may be nonzero
+  auto point_a = [&](auto dgt, int m, C ls, C lq, R v, C &a4, C &a3) {
+    constexpr bool DG = decltype(dgt)::value;
     if (!interior(m)) {
       a4 = a3 = zero;
       return;
     }
-    const R dg = R(-2) * gam_of(m), dv = R(2) * v;  // D* = -2 gamma + 2 i V
-    const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp<R>(dg, dv, lq)), h6);
+    const R dg = DG ? R(-2) * gam_of(m) : R(0), dv = R(2) * v;  // D* = -2 gamma + 2 i V
+    const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp_t<DG, R>(dg, dv, lq)), h6);
     a4 = cscale<R>(lq, h6);
     a3 = caxpy<R>(cscale<R>(lq, h3), h, yb4q);
   };
   // plane m into LDS buffer b: own column (prepared) and the outer neighbour
-  auto store_plane = [&](int b, int m, C s_, C q_, C a4, C a3, Halo &o) {
+  auto store_plane = [&](auto dgt, int b, int m, C s_, C q_, C a4, C a3, Halo &o) {
     lssb[b][ss] = s_;
     lsqb[b][ss] = q_;
     la4b[b][ss] = a4;
@@ -1257,7 +1273,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     if (has_o) {
       inject(m, o.s, o.q, oo);
       C oa4, oa3;
-      point_a(m, o.ls, o.lq, o.v, oa4, oa3);
+      point_a(dgt, m, o.ls, o.lq, o.v, oa4, oa3);
       lssb[b][hs] = o.s;
       lsqb[b][hs] = o.q;
       la4b[b][hs] = oa4;
@@ -1278,7 +1294,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   auto dot = [](C k, C y, R acc) { return acc + k.x * y.x + k.y * y.y; };
   auto crs = [](C k, C y, R acc) { return acc + k.x * y.y - k.y * y.x; };
 
-  Raw r0, r1, r2, r3;
+  Raw r1, r2, r3;
   Halo hn;
   load_plane(m_start, r1);
   load_plane(m_start + 1, r2);
@@ -1291,7 +1307,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     load_plane(m_start - 1, w);
     inject(m_start - 1, w.s, w.q, ij);
     C a4, a3;
-    point_a(m_start - 1, w.ls, w.lq, w.v, a4, a3);
+    point_a(std::true_type{}, m_start - 1, w.ls, w.lq, w.v, a4, a3);
     const int b = (m_start - 1) & 1;
     lssb[b][ss] = w.s;
     lsqb[b][ss] = w.q;
@@ -1307,8 +1323,11 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   inject(m_start, r1.s, r1.q, ij);
   {
     C a4, a3;
-    point_a(m_start, r1.ls, r1.lq, r1.v, a4, a3);
-    store_plane(m_start & 1, m_start, r1.s, r1.q, a4, a3, hn);
+    point_a(std::true_type{}, m_start, r1.ls, r1.lq, r1.v, a4, a3);
+    store_plane(std::true_type{}, m_start & 1, m_start, r1.s, r1.q, a4, a3, hn);
+    lols[own] = r1.ls;
+    lolq[own] = r1.lq;
+    lov[own] = r1.v;
   }
   __syncthreads();
 
@@ -1316,33 +1335,35 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     C z3, ps, pq;   // (its y3, a2, a1, q and a4 are read back from the LDS planes)
     R dl, dv, g1;
   };
-  Pass1 prev;
-  prev.z3 = prev.ps = prev.pq = zero;
-  prev.dl = prev.dv = prev.g1 = R(0);
-  R vb = R(0);  // vbar at the plane of the next pass-2
+  Pass1 p0, p1;
+  p0.z3 = p0.ps = p0.pq = zero;
+  p0.dl = p0.dv = p0.g1 = R(0);
+  R vb0 = R(0), vb1 = R(0);  // vbar at the plane of the next pass-2
 
-  // rb holds plane m (lam, V used), rc plane m+1 (raw), pre receives plane m+2.  The
-  // values of planes m and m-1 (and the Y planes m-1, m-2) that this lane needs at its
-  // own column are read back from its own LDS slots: buffer rd holds plane m (state,
-  // a4, a3) and m-1 (y3, a2, a1); buffer wr still holds plane m-1 (state, a4, a3) and
-  // m-2 (y3, a2, a1) until this lane overwrites its slot at the end of the iteration.
-  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
-    (void)ra;
+  // rc holds plane m+1 (raw), pre receives plane m+2; V and lam of plane m come from the
+  // own LDS slot.  The values of planes m and m-1 (and the Y planes m-1, m-2) that this
+  // lane needs at its own column are read back from its own LDS slots: buffer rd holds
+  // plane m (state, a4, a3) and m-1 (y3, a2, a1); buffer wr still holds plane m-1
+  // (state, a4, a3) and m-2 (y3, a2, a1) until this lane overwrites its slot at the end
+  // of the iteration.  prev/cur and vb/vb_next alternate between two copies.
+  // DG: gamma may be nonzero on planes m-1..m+1 (else the damping is the V term alone)
+  auto plane_step = [&](auto dgt, const int m, Raw &rc, Raw &pre, const Pass1 &prev, Pass1 &cur,
+                        const R vb, R &vb_next) {
+    constexpr bool DG = decltype(dgt)::value;
     const int rd = m & 1, wr = rd ^ 1;
     // loads: vbar for the next iteration's pass-2 plane, the outer neighbours of plane
     // m+1, then the prefetch of plane m+2 (older loads are the ones waited for)
-    const R vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
-                                (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+    vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
+                        (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
     load_halo(m + 1, hn);
     load_plane(m + 2, pre);
     inject(m + 1, rc.s, rc.q, ij);
     C ca4, ca3;
-    point_a(m + 1, rc.ls, rc.lq, rc.v, ca4, ca3);
+    point_a(dgt, m + 1, rc.ls, rc.lq, rc.v, ca4, ca3);
     // ---- pass-1 at m (forward sweep 1 and adjoint sweep 1)
-    Pass1 cur;
     C cy3, ca2, ca1;  // this plane's y3, a2, a1 (LDS and the z+ neighbours of pass-2)
     const bool in_m = interior(m);
-    const R v = rb.v;
+    const R v = lov[own];
     cur.dl = p.d0_lo - v * v;
     cur.dv = R(-2) * v;
     const C qm = lsqb[wr][ss], a4m = la4b[wr][ss];  // plane m-1 (also pass-2's center)
@@ -1351,7 +1372,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       const C a4 = la4b[rd][ss], a3 = la3b[rd][ss];
       const C sm = lssb[wr][ss], a3m = la3b[wr][ss];
       const R f1 = in_m ? h : R(0), f2 = in_m ? h2 : R(0);
-      const R dg = R(-2) * gam_of(m);
+      const R dg = DG ? R(-2) * gam_of(m) : R(0);
       C sx, sy, qx, qy;
       lat(lssb[rd], sx, sy);
       lat(lsqb[rd], qx, qy);
@@ -1360,9 +1381,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       // forward
       const C As = apply(cur.dl, s, sx, sy, cadd<R>(rc.s, sm));
       const C Aq = apply(cur.dl, q, qx, qy, cadd<R>(qhp, qhm));
-      const C f = cadd<R>(As, kg1_damp<R>(dg, cur.dv, q));
+      const C f = cadd<R>(As, kg1_damp_t<DG, R>(dg, cur.dv, q));
       const C u = caxpy<R>(q, f2, f);
-      const C gg = cadd<R>(caxpy<R>(As, f2, Aq), kg1_damp<R>(dg, cur.dv, u));
+      const C gg = cadd<R>(caxpy<R>(As, f2, Aq), kg1_damp_t<DG, R>(dg, cur.dv, u));
       cy3 = caxpy<R>(s, f2, u);
       cur.z3 = caxpy<R>(q, f2, gg);
       const C y4 = caxpy<R>(s, f1, cur.z3);
@@ -1373,16 +1394,16 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       lat(la3b[rd], bx, by);
       const C T3 = apply(cur.dl, a3, bx, by, cadd<R>(ca3, a3m));
       const R dvs = R(2) * v;  // D* = -2 gamma + 2 i V
-      const C ls = rb.ls, lq = rb.lq;
+      const C ls = lols[own], lq = lolq[own];
       if (in_m) {
-        const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp<R>(dg, dvs, lq)), h6);
-        const C yb3q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h, T4), kg1_damp<R>(dg, dvs, a3));
+        const C yb4q = cscale<R>(cadd<R>(ls, kg1_damp_t<DG, R>(dg, dvs, lq)), h6);
+        const C yb3q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h, T4), kg1_damp_t<DG, R>(dg, dvs, a3));
         ca2 = caxpy<R>(cscale<R>(lq, h3), h2, yb3q);
-        const C yb2q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h2, T3), kg1_damp<R>(dg, dvs, ca2));
+        const C yb2q = cadd<R>(caxpy<R>(cscale<R>(ls, h3), h2, T3), kg1_damp_t<DG, R>(dg, dvs, ca2));
         ca1 = caxpy<R>(cscale<R>(lq, h6), h2, yb2q);
         cur.ps = cadd<R>(ls, cadd<R>(T4, T3));
         cur.pq = cadd<R>(cadd<R>(lq, yb4q), cadd<R>(cadd<R>(yb3q, yb2q),
-                                                     caxpy<R>(kg1_damp<R>(dg, dvs, ca1), h6, ls)));
+                                                     caxpy<R>(kg1_damp_t<DG, R>(dg, dvs, ca1), h6, ls)));
         const C y2s = caxpy<R>(s, h2, q);
         const R sd = dot(a4, y4, dot(a3, cy3, dot(ca2, y2s, dot(ca1, s, R(0)))));
         const R sc = crs(a3, cur.z3, crs(ca2, u, crs(ca1, q, R(0))));
@@ -1401,11 +1422,11 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     R g = R(0);
     if (wave_writes) {
       const bool in_k = interior(k);
-      const R dg = R(-2) * gam_of(k);
+      const R dg = DG ? R(-2) * gam_of(k) : R(0);
       C sx, sy;
       lat(ly3b[rd], sx, sy);
       const C k3 = cadd<R>(apply(prev.dl, ly3b[rd][ss], sx, sy, cadd<R>(cy3, ly3b[wr][ss])),
-                           kg1_damp<R>(dg, prev.dv, prev.z3));
+                           kg1_damp_t<DG, R>(dg, prev.dv, prev.z3));
       const C wv4 = caxpy<R>(qm, h, k3);
       lat(la2b[rd], sx, sy);
       const C T2 = apply(prev.dl, la2b[rd][ss], sx, sy, cadd<R>(ca2, la2b[wr][ss]));
@@ -1423,20 +1444,26 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       buf_st<C>(roq, o, b * (uint32_t)sizeof(C), lnq);
       buf_st<R>(rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);
     }
-    vb = vb_next;
     ly3b[wr][ss] = cy3;
     la2b[wr][ss] = ca2;
     la1b[wr][ss] = ca1;
-    store_plane(wr, m + 1, rc.s, rc.q, ca4, ca3, hn);
+    lols[own] = rc.ls;
+    lolq[own] = rc.lq;
+    lov[own] = rc.v;
+    store_plane(dgt, wr, m + 1, rc.s, rc.q, ca4, ca3, hn);
     __syncthreads();
-    prev = cur;
   };
 
-  // a plain loop: the 4-slot unrolled ring of the forward pass spills here
+  // a plain loop with copies (unrolling the register rings spills at 128 VGPRs; so does
+  // a variant specialised to interior planes, which then runs 2x slower)
   for (int m = m_start; m <= m_end; ++m) {
-    plane_step(m, r0, r1, r2, r3);
-    r1 = r2;
+    if (m - 1 >= p.gz_lo && m + 1 <= p.gz_hi)  // no sponge on planes m-1..m+1
+      plane_step(std::false_type{}, m, r2, r3, p0, p1, vb0, vb1);
+    else
+      plane_step(std::true_type{}, m, r2, r3, p0, p1, vb0, vb1);
     r2 = r3;
+    p0 = p1;
+    vb0 = vb1;
   }
 }
 
