@@ -230,6 +230,17 @@ int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
                        const double *extent, double r_soft, double cutoff, void *out, int dtype,
                        void *stream);
 
+/* Phase grating of one potential slice of a spectral multislice: psi *= exp(i 2 pi
+ * (n(v) - 1) dz / lambda), n the relativistic electron refractive index at the
+ * electrostatic potential v [V] for kinetic energy energy_ev (rest energy
+ * rest_energy_ev).  Replaces electron_refractive_index and the phase grating of
+ * simulate_fresnel_as (propagation.py:66-81, 330-335).  psi: n complex of dtype
+ * WAP_DT_C64 / C128, v: n reals of dtype WAP_DT_F32 / F64, both device memory;
+ * asynchronous on `stream` (NULL: the null stream). */
+int wap_slice_transmit(void *psi, int psi_dtype, const void *v, int v_dtype, int64_t n,
+                       double energy_ev, double rest_energy_ev, double slice_thickness,
+                       double wavelength, void *stream);
+
 /* Measured HBM bandwidth reference (no reference counterpart): a STREAM copy
  * (read + write) and a read-only sweep of `bytes`-sized device buffers, reps
  * launches each, timed with HIP events on the null stream; GB/s = 1e9 B/s. */

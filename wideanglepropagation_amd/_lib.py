@@ -80,6 +80,8 @@ SIGNATURES = {
     "wap_adjoint_seed": (_c_int, [_vp, _c_int, _vp, _c_dbl, _c_dbl]),
     "wap_adjoint_get_grad": (_c_int, [_vp, _vp, _c_int]),
     "wap_stream_bandwidth": (_c_int, [_c_i64, _c_int, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_dbl)]),
+    "wap_slice_transmit": (_c_int, [_vp, _c_int, _vp, _c_int, _c_i64, _c_dbl, _c_dbl, _c_dbl,
+                                    _c_dbl, _vp]),
     "wap_atom_potential": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_dbl, _c_dbl, _vp,
                                     _c_int, _vp]),
 }

@@ -1710,3 +1710,36 @@ extern "C" int wap_atom_potential(const double *atoms, int n_atoms, int nx, int 
   if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("wap_atom_potential: ") + hipGetErrorString(e));
   return WAP_OK;
 }
+
+extern "C" int wap_slice_transmit(void *psi, int psi_dtype, const void *v, int v_dtype, int64_t n,
+                                  double energy_ev, double rest_energy_ev, double slice_thickness,
+                                  double wavelength, void *stream) {
+  if (n < 0) return fail(WAP_ERR_ARG, "n must be >= 0");
+  if (n > 0 && (!psi || !v)) return fail(WAP_ERR_ARG, "NULL argument");
+  if (psi_dtype != WAP_DT_C64 && psi_dtype != WAP_DT_C128)
+    return fail(WAP_ERR_ARG, "psi dtype must be WAP_DT_C64 or WAP_DT_C128");
+  if (v_dtype != WAP_DT_F32 && v_dtype != WAP_DT_F64)
+    return fail(WAP_ERR_ARG, "V dtype must be WAP_DT_F32 or WAP_DT_F64");
+  if (!(energy_ev > 0.0) || !(rest_energy_ev > 0.0) || !(wavelength > 0.0))
+    return fail(WAP_ERR_ARG, "energy, rest energy and wavelength must be > 0");
+  if (n == 0) return WAP_OK;
+  const double kdz = 2.0 * M_PI * slice_thickness / wavelength;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 16384);
+  const dim3 grid((unsigned)blocks), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  const bool c128 = psi_dtype == WAP_DT_C128, f64 = v_dtype == WAP_DT_F64;
+  if (c128 && f64)
+    hipLaunchKernelGGL((wap::slice_transmit_kernel<double, double>), grid, block, 0, st,
+                       (wap::cplx<double> *)psi, (const double *)v, n, energy_ev, rest_energy_ev, kdz);
+  else if (c128)
+    hipLaunchKernelGGL((wap::slice_transmit_kernel<double, float>), grid, block, 0, st,
+                       (wap::cplx<double> *)psi, (const float *)v, n, energy_ev, rest_energy_ev, kdz);
+  else if (f64)
+    hipLaunchKernelGGL((wap::slice_transmit_kernel<float, double>), grid, block, 0, st,
+                       (wap::cplx<float> *)psi, (const double *)v, n, energy_ev, rest_energy_ev, kdz);
+  else
+    hipLaunchKernelGGL((wap::slice_transmit_kernel<float, float>), grid, block, 0, st,
+                       (wap::cplx<float> *)psi, (const float *)v, n, energy_ev, rest_energy_ev, kdz);
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}

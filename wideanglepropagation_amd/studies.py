@@ -1,6 +1,6 @@
-"""Workload drivers: precision study, batched probes, adjoint gradient.
+"""Workload drivers: precision study, batched probes, adjoint gradient, spectral comparison.
 
-``python -m wideanglepropagation_amd.studies precision|batched|adjoint [--quick]``
+``python -m wideanglepropagation_amd.studies precision|batched|adjoint|spectral [--quick]``
 prints one JSON object per study.  Sizes follow the solver's standard
 workloads:
 
@@ -22,7 +22,15 @@ adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
            150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
            (seed 3).  sqrt(T) checkpointing.  Checks dL/dV against central
            differences of the f64 forward at 32 random voxels (seed 4) and reports
-           backward / forward time.
+           backward / forward time; the same loss and step count are also timed
+           on the 512 x 512 x 256 grid of the precision study (timing only).
+spectral   The plane-wave comparison of ``fdtd_3D_baseline_comparison.ipynb``:
+           lambda = 0.05 A, 64 x 64 over [-1, 1]^2 A, one Z = 79 atom (screened
+           Coulomb, 0.3 A) in the ROI z in [-1, 1] A, FDTD (Schrodinger, drop-in
+           RelativisticSolver3D) with a 0.4 A PML and 0.2 A source buffer, 30000
+           steps, at the notebook's dz = 2/127 A and at dz = 0.005 A; the exit wave
+           against the Fresnel and angular spectrum multislices of the same
+           potential.  Also the GPU multislice's slices/s at 512 x 512 x 256.
 """
 import argparse
 import json
@@ -260,6 +268,7 @@ def adjoint_study(quick=False, n_fd=32):
             print(f"[adjoint] {len(rows)}/{len(idx)} finite differences", file=sys.stderr,
                   flush=True)
     gmax = float(np.abs(g64).max())
+    big = None if quick else _adjoint_timing((256, 512, 512), nsteps, samples)
     err64 = [abs(r[2] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
     err32 = [abs(r[3] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
     return {"study": "adjoint", "grid_nz_ny_nx": list(shape), "steps": nsteps,
@@ -272,16 +281,195 @@ def adjoint_study(quick=False, n_fd=32):
             "max_rel_err_f32_adjoint_vs_fd": float(max(err32)),
             "median_rel_err_f32_adjoint_vs_fd": float(np.median(err32)),
             "rel_l2_f32_vs_f64_gradient": rel_l2(g32, g64),
+            "timing_512x512x256": big,
             "samples": [{"voxel": list(r[0]), "fd": r[1], "adj_f64": r[2], "adj_f32": r[3]}
                         for r in rows[:6]]}
 
 
+def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
+    """Forward and forward + backward time of the adjoint study's loss on another grid
+    (random target: timing only)."""
+    from .adjoint import exit_wave, far_field_intensity
+    p, V = kg_slab(shape, precision="f32", device="cuda", n_atoms=n_atoms, seed=2)
+    p = p.with_(dt=1e-6)
+    V32 = V.to(torch.float32)
+    del V
+    k_exit = p.nz - 16 - 3
+    It = torch.rand((p.ny, p.nx), dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(5))
+
+    def loss_of(Vx):
+        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples)
+        return torch.sum((far_field_intensity(E) - It) ** 2)
+
+    with torch.no_grad():  # warm-up (allocations, library load)
+        loss_of(V32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        loss_of(V32)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    Vg = V32.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_of(Vg).backward()
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    print(f"[adjoint] {shape}: forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s",
+          file=sys.stderr, flush=True)
+    return {"grid_nz_ny_nx": list(shape), "steps": nsteps,
+            "checkpoint_every": math.ceil(math.sqrt(nsteps)), "t_forward_s": t_fwd,
+            "t_forward_backward_s": t_fb, "backward_over_forward": (t_fb - t_fwd) / t_fwd}
+
+
+def _notebook_grid(dz_roi):
+    """The plane-wave comparison grid of ``fdtd_3D_baseline_comparison.ipynb`` (cell 3):
+    64 x 64 over [-1, 1]^2 A, the ROI z in [-1, 1] at spacing dz_roi, padded by a 0.4 A
+    PML and a 0.2 A source buffer on each side."""
+    nx = ny = 64
+    roi_nz = int(round(2.0 / dz_roi)) + 1
+    z = np.linspace(-1.0, 1.0, roi_nz)
+    dz = z[1] - z[0]
+    pml, buf = 0.4, 0.2
+    pad = pml + buf
+    ext_zmin = -1.0 - pad
+    ext_nz = int(np.round((1.0 + pad - ext_zmin) / dz))
+    z_ext = np.linspace(ext_zmin, ext_zmin + (ext_nz - 1) * dz, ext_nz)
+    return {"nx": nx, "ny": ny, "roi_nz": roi_nz, "dz": dz, "z": z, "z_ext": z_ext,
+            "pml": pml, "buf": buf, "pad": pad,
+            "extent_roi": (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0),
+            "extent_ext": (-1.0, 1.0, -1.0, 1.0, ext_zmin, float(z_ext[-1]))}
+
+
+def fdtd_vs_multislice(dz_roi, steps, wavelength=0.05, Z=79.0, screening=0.3):
+    """FDTD exit wave of a plane wave through one atom against the Fresnel and angular
+    spectrum multislices of the same potential, following the notebook's recipe
+    (``fdtd_3D_baseline_comparison.ipynb`` cells 3-16): the potential fills the ROI
+    planes of the padded FDTD grid (zero elsewhere); the FDTD runs through the drop-in
+    ``RelativisticSolver3D.run``; its exit plane (z = 1 A) is demodulated and normalised
+    by the incident intensity two planes before the ROI; the multislices start from a
+    plane wave pre-propagated (angular spectrum) over the padding."""
+    from . import spectral as sp
+    from .fdtd_solver import RelativisticSolver3D
+    g = _notebook_grid(dz_roi)
+    nx, ny, roi_nz, dz = g["nx"], g["ny"], g["roi_nz"], g["dz"]
+    z_ext = g["z_ext"]
+    ext_nz = len(z_ext)
+    # the atom's potential energy on the ROI planes (eV, negative); the electrostatic
+    # potential the multislice takes is its negative
+    V_roi = atom_potential((roi_nz, ny, nx), g["extent_roi"], [(0.0, 0.0, 0.0)], [Z],
+                           [screening], r_soft=2.0 / (nx - 1), device="cuda",
+                           dtype=torch.float64)
+    k0 = int(np.argmin(np.abs(z_ext - (-1.0))))
+    k0 = min(k0, ext_nz - roi_nz)
+    V_ext = np.zeros((ext_nz, ny, nx), dtype=np.float64)
+    V_ext[k0:k0 + roi_nz] = V_roi.cpu().numpy()
+    k_source = int((g["pml"] + 0.5 * g["buf"]) / dz)
+    k_source = max(1, min(max(k_source, int(np.ceil(g["pml"] / dz)) + 2), ext_nz - 2))
+    t0 = time.perf_counter()
+    res = RelativisticSolver3D().run(V_ext, wavelength_angstrom=wavelength,
+                                     extent=g["extent_ext"], n_frames=1, steps_per_frame=steps,
+                                     snapshot_count=1, pml_thick=g["pml"], k_source=k_source,
+                                     use_angstrom_units=True)
+    t_fdtd = time.perf_counter() - t0
+    if not res:
+        raise RuntimeError("FDTD run failed")
+    k_exit = int(np.argmin(np.abs(z_ext - 1.0)))
+    k_mon = max(0, int(np.argmin(np.abs(z_ext - (-1.0)))) - 2)
+    ex = np.zeros((ny, nx), dtype=np.complex128)
+    mon = np.zeros((ny, nx), dtype=np.complex128)
+    for t, phi, psi in res["snapshots"]:
+        f = (phi.astype(np.float64) + 1j * psi.astype(np.float64)) * np.exp(1j * float(res["E_sim"]) * float(t))
+        ex += f[k_exit]
+        mon += f[k_mon]
+    n = max(1, len(res["snapshots"]))
+    ex, mon = ex / n, mon / n
+    i_inc = float(np.mean(np.abs(mon) ** 2))
+    if i_inc > 1e-9:
+        ex = ex / math.sqrt(i_inc)
+    # spectral side (energy of the beam from the wavelength with the spectral constants)
+    pc = sp.HPLANCK * sp.C_LIGHT / (wavelength * 1e-10) / sp.E_CHARGE
+    e0 = sp.electron_rest_energy()
+    energy = math.sqrt(pc * pc + e0 * e0) - e0
+    ps = (2.0 / (ny - 1), 2.0 / (nx - 1))
+    probe = torch.ones((ny, nx), dtype=torch.complex128, device="cuda")
+    probe = sp.Propagator(probe, sp.angular_spectrum_propagation_kernel(ny, nx, ps, g["pad"], energy))
+    phi_roi = -V_roi
+    out = {"grid_ext_nz_ny_nx": [ext_nz, ny, nx], "roi_planes": roi_nz, "dz": dz,
+           "wavelength_A": wavelength, "energy_eV": energy, "fdtd_steps": steps,
+           "k_source": k_source, "k_exit": k_exit, "fdtd_s": t_fdtd,
+           "fdtd_gvoxel_steps_per_s": res.get("gvoxel_steps_per_s")}
+    exits = {}
+    for name, kern in (("fresnel", sp.fresnel_propagation_kernel),
+                       ("angular_spectrum", sp.angular_spectrum_propagation_kernel)):
+        H = kern(ny, nx, ps, dz, energy)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e_ms, _, _ = sp.simulate_fresnel_as(phi_roi, probe, H, dz, energy, keep_wavefronts=False)
+        torch.cuda.synchronize()
+        out[f"{name}_s"] = time.perf_counter() - t0
+        exits[name] = e_ms
+    fd = torch.tensor(ex, device="cuda")
+    out["fdtd_vs_fresnel"] = sp.compare_exit_waves(fd, exits["fresnel"])
+    out["fdtd_vs_angular_spectrum"] = sp.compare_exit_waves(fd, exits["angular_spectrum"])
+    out["fresnel_vs_angular_spectrum"] = sp.compare_exit_waves(exits["fresnel"],
+                                                               exits["angular_spectrum"])
+    return out
+
+
+def multislice_throughput(n=512, slices=256, dtype=torch.complex64, reps=3):
+    """Slices per second of the GPU multislice (HIP phase grating + rocFFT) on an n x n
+    plane wave through a random potential, and the phase-grating kernel's own rate."""
+    from . import spectral as sp
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    rdt = torch.float32 if dtype == torch.complex64 else torch.float64
+    pot = 50.0 * torch.rand((slices, n, n), generator=gen, device="cuda", dtype=rdt)
+    probe = torch.ones((n, n), dtype=dtype, device="cuda")
+    e, dz, ps = 200e3, 0.05, (0.05, 0.05)
+    H = sp.angular_spectrum_propagation_kernel(n, n, ps, dz, e)
+    sp.simulate_fresnel_as(pot, probe, H, dz, e, keep_wavefronts=False)
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sp.simulate_fresnel_as(pot, probe, H, dz, e, keep_wavefronts=False)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    w = probe.clone()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for i in range(slices):
+        sp.transmit(w, pot[i], dz, e)
+    b.record()
+    torch.cuda.synchronize()
+    t_tr = a.elapsed_time(b) / 1e3 / slices
+    nbytes = n * n * (2 * w.element_size() + pot.element_size())
+    return {"n": n, "slices": slices, "dtype": str(dtype).replace("torch.", ""),
+            "slices_per_s": slices / best, "ms_per_slice": best / slices * 1e3,
+            "transmit_us": t_tr * 1e6, "transmit_GBs": nbytes / t_tr / 1e9}
+
+
+def spectral_study(quick=False):
+    """FDTD against spectral multislice exit waves, and the multislice's throughput."""
+    rows = [fdtd_vs_multislice(2.0 / 127, 8000 if quick else 30000)]
+    if not quick:
+        rows.append(fdtd_vs_multislice(0.005, 30000))
+    thr = [multislice_throughput(n=128 if quick else 512, slices=32 if quick else 256,
+                                 dtype=d) for d in (torch.complex64, torch.complex128)]
+    for r in rows:
+        print(f"[spectral] dz {r['dz']:.4f}: fdtd vs AS {r['fdtd_vs_angular_spectrum']}",
+              file=sys.stderr, flush=True)
+    return {"study": "spectral", "comparisons": rows, "multislice_throughput": thr}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["precision", "batched", "adjoint"])
+    ap.add_argument("study", choices=["precision", "batched", "adjoint", "spectral"])
     ap.add_argument("--quick", action="store_true")
     a = ap.parse_args(argv)
-    fn = {"precision": precision_study, "batched": batched_study, "adjoint": adjoint_study}
+    fn = {"precision": precision_study, "batched": batched_study, "adjoint": adjoint_study,
+          "spectral": spectral_study}
     out = fn[a.study](quick=a.quick)
     print(json.dumps(out), flush=True)
 
