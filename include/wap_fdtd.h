@@ -202,16 +202,22 @@ int wap_batch_synchronize(wap_batch *b);
  * wap_adjoint_record each step of the segment into slots 0..m-1 (this also
  * advances the handle), and walk the segment backwards with wap_adjoint_step
  * (slot m-1 .. 0), adding exit-plane seeds with wap_adjoint_seed.  f64 and f32
- * handles; the records use the 4-pass stage schedule.
+ * handles with frozen end planes.  Klein-Gordon grids within the fused pass's field
+ * limit record each step's input state (the record step is the fused forward step,
+ * whose output buffer is swapped with the slot) and reverse a step in one fused
+ * pass; Schrodinger records the 4 stage inputs and reverses stage by stage.  Destroy
+ * the adjoint before its handle.
  */
 typedef struct wap_adjoint wap_adjoint;
 
-/* Adjoint workspace with `slots` per-step records (4 stage inputs per record). */
+/* Adjoint workspace with `slots` per-step records (KG: the input state, 2 fields;
+ * Schrodinger: 4 stage inputs). */
 int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out);
 int wap_adjoint_destroy(wap_adjoint *a);
 /* lambda = 0, dL/dV = 0. */
 int wap_adjoint_reset(wap_adjoint *a);
-/* One forward step (its time as wap_step(h, 1, step, t0)), recording its stage inputs. */
+/* One forward step (its time as wap_step(h, 1, step, t0)), recording what its
+ * reverse step needs into `slot`. */
 int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot);
 /* lambda <- adjoint of the recorded step; dL/dV += its contribution. */
 int wap_adjoint_step(wap_adjoint *a, int slot);
