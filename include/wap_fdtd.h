@@ -213,6 +213,16 @@ typedef struct wap_adjoint wap_adjoint;
 /* Adjoint workspace with `slots` per-step records (KG: the input state, 2 fields;
  * Schrodinger: 4 stage inputs). */
 int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out);
+/* Device bytes wap_adjoint_create_in needs for `slots` records on this handle. */
+int wap_adjoint_workspace_bytes(wap_handle *h, int slots, int64_t *bytes);
+/* As wap_adjoint_create, with every device buffer carved from the caller's `workspace`
+ * (device memory, 256-byte aligned, at least wap_adjoint_workspace_bytes): for callers
+ * with a caching allocator, so that repeated backward passes neither allocate nor free
+ * gigabytes of records.  The workspace must outlive the adjoint. */
+int wap_adjoint_create_in(wap_handle *h, int slots, void *workspace, int64_t bytes,
+                          wap_adjoint **out);
+/* Gives the handle back its own buffers (copying the state if a record swap left it in
+ * a record buffer), synchronises the handle's stream and releases the adjoint. */
 int wap_adjoint_destroy(wap_adjoint *a);
 /* lambda = 0, dL/dV = 0. */
 int wap_adjoint_reset(wap_adjoint *a);

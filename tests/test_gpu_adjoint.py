@@ -88,3 +88,111 @@ def test_f32_adjoint_close_to_f64(rng):
         grads.append(V.grad.detach().double().cpu().numpy())
     rel = np.linalg.norm(grads[1] - grads[0]) / np.linalg.norm(grads[0])
     assert rel < 2e-3, rel
+
+
+def _kg32(rng):
+    p64, Vn = _setup("klein_gordon", "f64", rng)
+    p = p64.with_(precision="f32")
+    assert p.resolved_schedule() == "kg_fused"   # records swap buffers on this schedule
+    return p, Vn.astype(np.float32), p64.init
+
+
+@pytest.mark.parametrize("nrec", [4, 5])
+def test_adjoint_leaves_handle_intact(rng, nrec):
+    # records advance the handle by swapping its output buffer into the slots; destroying
+    # the adjoint must hand the handle its own buffers back with the state intact (an odd
+    # record count leaves the state in a record buffer, which is then copied)
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print(f"case adjoint handle restore nrec={nrec}", flush=True)
+    from wideanglepropagation_amd.engine import FDTD3D
+    from wideanglepropagation_amd.adjoint import _Adjoint
+    p, V, (psi0, pi0) = _kg32(rng)
+    a, b = FDTD3D(p, V=V), FDTD3D(p, V=V)
+    try:
+        for s in (a, b):
+            s.set_state(psi0, pi0)
+        adj = _Adjoint(a, nrec)
+        ws_bytes = adj._ws.numel()
+        for m in range(nrec):
+            adj.record(m, m)
+        for m in reversed(range(nrec)):
+            adj.step(m)
+        adj.close()
+        a.step_index += nrec   # records advance the handle but not the engine's counter
+        # reuse the freed workspace memory with garbage: the handle must not point into it
+        junk = torch.full((ws_bytes,), 0x7f, dtype=torch.uint8, device="cuda")
+        b.step(nrec)
+        sa, sb = a.get_state(np.complex64), b.get_state(np.complex64)
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y)
+        a.step(3)
+        b.step(3)
+        junk.fill_(0x3c)
+        sa, sb = a.get_state(np.complex64), b.get_state(np.complex64)
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y)
+        del junk
+    finally:
+        a.close()
+        b.close()
+
+
+def test_adjoint_create_paths_agree(rng):
+    # the library-allocated and the caller-workspace adjoints give the same gradient
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print("case adjoint create vs create_in", flush=True)
+    import ctypes
+    from wideanglepropagation_amd.engine import FDTD3D
+    from wideanglepropagation_amd.adjoint import _Adjoint
+    from wideanglepropagation_amd._lib import lib, check
+    p, V, (psi0, pi0) = _kg32(rng)
+    g = torch.tensor(rng.standard_normal((p.ny, p.nx)) + 1j * rng.standard_normal((p.ny, p.nx)),
+                     dtype=torch.complex128, device="cuda")
+    grads = []
+    for mode in ("create", "create_in"):
+        sim = FDTD3D(p, V=V)
+        sim.set_state(psi0, pi0)
+        if mode == "create_in":
+            adj = _Adjoint(sim, 6)
+        else:
+            adj = _Adjoint.__new__(_Adjoint)
+            adj._L, adj.sim, adj._ws = lib(), sim, None
+            h = ctypes.c_void_p()
+            check(adj._L.wap_adjoint_create(sim.handle, 6, ctypes.byref(h)), "create")
+            adj._a = h
+        for m in range(6):
+            adj.record(m, m)
+        adj.seed(p.nz - 5, g, 0.5 + 0.25j)
+        for m in reversed(range(6)):
+            adj.step(m)
+        grads.append(adj.grad().cpu().numpy())
+        adj.close()
+        sim.close()
+    assert np.abs(grads[0]).max() > 0
+    assert np.array_equal(grads[0], grads[1])
+
+
+def test_adjoint_workspace_too_small(rng):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import ctypes
+    from wideanglepropagation_amd.engine import FDTD3D
+    from wideanglepropagation_amd._lib import lib
+    p, V, _ = _kg32(rng)
+    sim = FDTD3D(p, V=V)
+    try:
+        L = lib()
+        nb = ctypes.c_int64()
+        assert L.wap_adjoint_workspace_bytes(sim.handle, 3, ctypes.byref(nb)) == 0
+        assert nb.value >= 3 * 2 * p.voxels * 8
+        ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+        h = ctypes.c_void_p()
+        assert L.wap_adjoint_create_in(sim.handle, 3, ctypes.c_void_p(ws.data_ptr()),
+                                       nb.value - 1, ctypes.byref(h)) != 0
+        assert b"workspace" in L.wap_last_error()
+        assert L.wap_adjoint_create_in(sim.handle, 3, ctypes.c_void_p(ws.data_ptr() + 8),
+                                       nb.value - 256, ctypes.byref(h)) != 0
+    finally:
+        sim.close()

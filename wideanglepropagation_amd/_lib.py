@@ -73,6 +73,8 @@ SIGNATURES = {
     "wap_batch_replay": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_dbl)]),
     "wap_batch_synchronize": (_c_int, [_vp]),
     "wap_adjoint_create": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp)]),
+    "wap_adjoint_workspace_bytes": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_i64)]),
+    "wap_adjoint_create_in": (_c_int, [_vp, _c_int, _vp, _c_i64, ctypes.POINTER(_vp)]),
     "wap_adjoint_destroy": (_c_int, [_vp]),
     "wap_adjoint_reset": (_c_int, [_vp]),
     "wap_adjoint_record": (_c_int, [_vp, _c_i64, _c_dbl, _c_int]),

@@ -26,12 +26,20 @@ from .engine import FDTD3D
 
 
 class _Adjoint:
+    """The C adjoint of one handle; its records live in a torch-allocated workspace, so
+    that the caching allocator keeps the memory between backward passes."""
+
     def __init__(self, sim, slots):
         self._L = lib()
         self.sim = sim
+        nb = ctypes.c_int64()
+        check(self._L.wap_adjoint_workspace_bytes(sim.handle, int(slots), ctypes.byref(nb)),
+              "wap_adjoint_workspace_bytes")
+        self._ws = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
         a = ctypes.c_void_p()
-        check(self._L.wap_adjoint_create(sim.handle, int(slots), ctypes.byref(a)),
-              "wap_adjoint_create")
+        check(self._L.wap_adjoint_create_in(sim.handle, int(slots),
+                                            ctypes.c_void_p(self._ws.data_ptr()), int(nb.value),
+                                            ctypes.byref(a)), "wap_adjoint_create_in")
         self._a = a
 
     def record(self, step, slot):
@@ -53,8 +61,9 @@ class _Adjoint:
 
     def close(self):
         if self._a is not None:
-            self._L.wap_adjoint_destroy(self._a)
+            check(self._L.wap_adjoint_destroy(self._a), "wap_adjoint_destroy")
             self._a = None
+            self._ws = None   # back to the caching allocator (destroy synchronised)
 
 
 def default_samples(nsteps, n_frames=None, start_frac=0.6):

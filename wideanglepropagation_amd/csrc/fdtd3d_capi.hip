@@ -13,6 +13,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wap_fdtd.h"
@@ -1317,21 +1318,80 @@ struct wap_adjoint {
   void *kbb[2] = {nullptr, nullptr};
   double *vbar = nullptr;
   void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type
+  bool owns = true;      // false: the buffers are carved from a caller's workspace
+  void *orig[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [comp]: the handle's
+                                                                  // state, scratch at create
   void *lam(int c) const { return lam2[cur][c]; }
 };
 
 namespace {
 
+// The device buffers of an adjoint and their sizes (st / rec already sized).
+std::vector<std::pair<void **, size_t>> adjoint_buffers(wap_adjoint *a) {
+  const wap_handle *h = a->h;
+  const size_t fb = h->n * h->cs;
+  std::vector<std::pair<void **, size_t>> v;
+  if (a->fused) {
+    for (void *&b : a->st) v.push_back({&b, fb});
+    for (int c = 0; c < 2; ++c) {
+      v.push_back({&a->lam2[0][c], fb});
+      v.push_back({&a->lam2[1][c], fb});
+    }
+    v.push_back({&a->vacc, h->n * h->rs});
+  } else {
+    for (void *&b : a->rec) v.push_back({&b, fb});
+    for (int c = 0; c < a->ncomp; ++c) {
+      v.push_back({&a->lam2[0][c], fb});
+      v.push_back({&a->acc[c], fb});
+      v.push_back({&a->kba[c], fb});
+      v.push_back({&a->kbb[c], fb});
+    }
+  }
+  v.push_back({(void **)&a->vbar, h->n * sizeof(double)});
+  return v;
+}
+
+constexpr size_t kWsAlign = 256;
+size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
+
+// Give the handle back its own state buffers.  A fused record swaps the step's output
+// buffer with a slot, so after records the handle may hold record buffers and the slots
+// the handle's; the state is copied when it sits in a record buffer.  A no-op when
+// nothing was swapped.
+int adjoint_restore(wap_adjoint *a) {
+  if (!a->fused) return WAP_OK;
+  wap_handle *h = a->h;
+  void **cur[2] = {&h->psi, &h->pi}, **ya[2] = {&h->ya_psi, &h->ya_pi};
+  for (int c = 0; c < 2; ++c) {
+    void *o1 = a->orig[c][0], *o2 = a->orig[c][1];
+    auto where = [&](void *p) -> void ** {
+      if (*ya[c] == p) return ya[c];
+      for (int s = 0; s < a->slots; ++s)
+        if (a->st[(size_t)s * 2 + c] == p) return &a->st[(size_t)s * 2 + c];
+      return nullptr;
+    };
+    if (*cur[c] != o1 && *cur[c] != o2) {
+      void **x = where(o1);
+      if (!x) return fail(WAP_ERR_STATE, "adjoint: a state buffer of the handle is missing");
+      WAP_HIP(hipMemcpyAsync(o1, *cur[c], h->n * h->cs, hipMemcpyDeviceToDevice, h->stream));
+      *x = *cur[c];
+      *cur[c] = o1;
+    }
+    void *other = (*cur[c] == o1) ? o2 : o1;
+    if (*ya[c] != other) {
+      void **x = where(other);
+      if (!x || x == ya[c]) return fail(WAP_ERR_STATE, "adjoint: a state buffer of the handle is missing");
+      std::swap(*x, *ya[c]);
+    }
+  }
+  return WAP_OK;
+}
+
 void adjoint_free(wap_adjoint *a) {
-  for (void *&p : a->rec)
-    if (p) { (void)hipFree(p); p = nullptr; }
-  for (void *&p : a->st)
-    if (p) { (void)hipFree(p); p = nullptr; }
-  void **bufs[] = {&a->lam2[0][0], &a->lam2[0][1], &a->lam2[1][0], &a->lam2[1][1],
-                   &a->acc[0], &a->acc[1], &a->kba[0], &a->kba[1],
-                   &a->kbb[0], &a->kbb[1], (void **)&a->vbar, &a->vacc};
-  for (void **b : bufs)
-    if (*b) { (void)hipFree(*b); *b = nullptr; }
+  for (auto &b : adjoint_buffers(a)) {
+    if (*b.first && a->owns) (void)hipFree(*b.first);
+    *b.first = nullptr;
+  }
 }
 
 void *rec_ptr(const wap_adjoint *a, int slot, int stage, int comp) {
@@ -1491,11 +1551,8 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   return WAP_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
+// validation and the host object of an adjoint (no device memory yet)
+int adjoint_new(wap_handle *h, int slots, wap_adjoint **out) {
   if (!h || !out) return fail(WAP_ERR_ARG, "NULL argument");
   *out = nullptr;
   if (slots < 1) return fail(WAP_ERR_ARG, "slots must be >= 1");
@@ -1508,39 +1565,35 @@ int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
   a->slots = slots;
   a->ncomp = is_kg(h) ? 2 : 1;
   a->fused = is_kg(h) && fused_plane_fits(h->d);
-  const size_t fb = h->n * h->cs;
-  int rc = WAP_OK;
-  auto alloc = [&](void **b) {
-    if (rc) return;
-    hipError_t e = hipMalloc(b, fb);
-    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
-  };
   if (a->fused) {
     a->st.assign((size_t)slots * 2, nullptr);
     a->st_step.assign(slots, 0);
     a->st_t0.assign(slots, 0.0);
-    for (void *&b : a->st) alloc(&b);
-    for (int c = 0; c < 2; ++c) {
-      alloc(&a->lam2[0][c]);
-      alloc(&a->lam2[1][c]);
-    }
-    if (!rc) {
-      hipError_t e = hipMalloc(&a->vacc, h->n * h->rs);
-      if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
-    }
+    a->orig[0][0] = h->psi;
+    a->orig[0][1] = h->ya_psi;
+    a->orig[1][0] = h->pi;
+    a->orig[1][1] = h->ya_pi;
   } else {
     a->rec.assign((size_t)slots * 4 * a->ncomp, nullptr);
-    for (void *&b : a->rec) alloc(&b);
-    for (int c = 0; c < a->ncomp; ++c) {
-      alloc(&a->lam2[0][c]);
-      alloc(&a->acc[c]);
-      alloc(&a->kba[c]);
-      alloc(&a->kbb[c]);
-    }
   }
-  if (!rc) {
-    hipError_t e = hipMalloc((void **)&a->vbar, h->n * sizeof(double));
-    if (e != hipSuccess) rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
+  *out = a;
+  return WAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
+  wap_adjoint *a = nullptr;
+  int rc = adjoint_new(h, slots, &a);
+  if (rc) return rc;
+  for (auto &b : adjoint_buffers(a)) {
+    hipError_t e = hipMalloc(b.first, b.second);
+    if (e != hipSuccess) {
+      rc = fail(WAP_ERR_OOM, std::string("adjoint: ") + hipGetErrorString(e));
+      break;
+    }
   }
   if (rc) {
     adjoint_free(a);
@@ -1551,9 +1604,47 @@ int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out) {
   return wap_adjoint_reset(a);
 }
 
+int wap_adjoint_workspace_bytes(wap_handle *h, int slots, int64_t *bytes) {
+  if (!bytes) return fail(WAP_ERR_ARG, "NULL argument");
+  wap_adjoint *a = nullptr;
+  int rc = adjoint_new(h, slots, &a);
+  if (rc) return rc;
+  size_t total = 0;
+  for (auto &b : adjoint_buffers(a)) total += ws_round(b.second);
+  delete a;
+  *bytes = (int64_t)total;
+  return WAP_OK;
+}
+
+int wap_adjoint_create_in(wap_handle *h, int slots, void *workspace, int64_t bytes,
+                          wap_adjoint **out) {
+  if (!workspace) return fail(WAP_ERR_ARG, "NULL workspace");
+  if ((uintptr_t)workspace % kWsAlign) return fail(WAP_ERR_ARG, "workspace must be 256-byte aligned");
+  wap_adjoint *a = nullptr;
+  int rc = adjoint_new(h, slots, &a);
+  if (rc) return rc;
+  a->owns = false;
+  auto bufs = adjoint_buffers(a);
+  size_t total = 0;
+  for (auto &b : bufs) total += ws_round(b.second);
+  if (bytes < 0 || (size_t)bytes < total) {
+    delete a;
+    return fail(WAP_ERR_ARG, "workspace smaller than wap_adjoint_workspace_bytes");
+  }
+  char *p = (char *)workspace;
+  for (auto &b : bufs) {
+    *b.first = p;
+    p += ws_round(b.second);
+  }
+  *out = a;
+  return wap_adjoint_reset(a);
+}
+
 int wap_adjoint_destroy(wap_adjoint *a) {
   if (!a) return WAP_OK;
+  const int rc = adjoint_restore(a);
   (void)hipStreamSynchronize(a->h->stream);
+  if (rc) return rc;  // the handle still holds record buffers: keep them allocated
   adjoint_free(a);
   delete a;
   return WAP_OK;
