@@ -11,7 +11,9 @@ in HBM before the timed region; the timed region is K steps bracketed by
 Printed (one JSON line): Gvoxel-steps/s; the roofline of the dominant kernel
 (algorithmic bytes per launch / its mean HIP-event duration, measured on the
 solver's stream); the CPU baseline (the FP64 NumPy oracle on a bounded slab
-of the same workload, timed on this host).
+of the same workload, timed on this host); for Klein-Gordon, the cost of the
+gradient's reverse steps on the same grid (``adjoint``: a recorded forward
+step and a fused adjoint step against a plain step, per step).
 
 Roofline bytes.  ``achieved`` counts the canonical bytes of an RK4 step that
 streams once per stage (Klein-Gordon complex64: 272 B per voxel-step,
@@ -94,7 +96,37 @@ def parse():
                     choices=["auto", "stages", "kg_2pass", "kg_fused"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration)")
+    ap.add_argument("--adjoint-steps", type=int, default=16,
+                    help="reverse steps timed for the adjoint figures (0: skip)")
     return ap.parse_args()
+
+
+def adjoint_cost(sim, nsteps, slots=8):
+    """Per-step times of the backward's two parts on this handle: a recorded forward step
+    (the recompute of a checkpoint segment) and an adjoint step, with sqrt(T)-style slot
+    reuse; the handle is advanced by the records."""
+    import torch
+    from wideanglepropagation_amd.adjoint import _Adjoint
+    slots = max(1, min(slots, nsteps))
+    adj = _Adjoint(sim, slots)
+    try:
+        for s in range(slots):       # warm-up
+            adj.record(s, s)
+            adj.step(slots - 1 - s)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for s in range(nsteps):
+            adj.record(s, s % slots)
+        torch.cuda.synchronize()
+        rec = (time.perf_counter() - t) / nsteps
+        t = time.perf_counter()
+        for s in range(nsteps):
+            adj.step((nsteps - 1 - s) % slots)
+        torch.cuda.synchronize()
+        rev = (time.perf_counter() - t) / nsteps
+    finally:
+        adj.close()
+    return rec, rev
 
 
 def copy_calibration(gib, reps=3):
@@ -208,6 +240,18 @@ def main():
     step_bytes = sum(pb) * vox
     step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
     norm = sim.norm2()
+    adjoint = None
+    if args.adjoint_steps > 0 and p.equation == "klein_gordon" and p.precision != "f32_kahan":
+        rec_s, rev_s = adjoint_cost(sim, args.adjoint_steps)
+        step_s = elapsed / args.steps
+        adjoint = {"steps": args.adjoint_steps, "step_ms": round(step_s * 1e3, 4),
+                   "record_ms": round(rec_s * 1e3, 4), "adjoint_ms": round(rev_s * 1e3, 4),
+                   "backward_over_forward": round((rec_s + rev_s) / step_s, 3),
+                   "adjoint_kernel": ("kg1_adj_pass" if vox * 8 < 2 ** 31 else
+                                      "adj_pass (4 stage passes)"),
+                   "adjoint_Gvoxels": round(vox / rev_s / 1e9, 2)}
+        print(f"[bench] adjoint: record {rec_s * 1e3:.3f} ms, reverse step {rev_s * 1e3:.3f} ms "
+              f"per step", file=sys.stderr, flush=True)
     sim.close()
     from wideanglepropagation_amd._lib import stream_bandwidth
     copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
@@ -253,6 +297,7 @@ def main():
                      "traffic_frac": (round(traffic / dom_s / 1e9 / HBM_PEAK_GBS, 4)
                                       if traffic else None)},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
         "detail": {"pass_ms": [round(a, 5) for a in avg],
                    "algorithmic_bytes_per_voxel_step": sum(pb),
                    "step_effective_GBs": round(step_gbs, 1),
