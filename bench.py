@@ -51,6 +51,8 @@ PASS_BYTES = {
     # both stencil sweeps in one pass
     ("kg_fused", "klein_gordon", "f32"): (36,),
     ("kg_fused", "klein_gordon", "f64"): (72,),
+    # Schrodinger RK4 polynomial in Horner form: 2 passes of 2 stencil levels
+    ("schr_fused", "schrodinger", "f32"): (20, 28),
 }
 
 
@@ -93,7 +95,7 @@ def parse():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--schedule", default="auto",
-                    choices=["auto", "stages", "kg_2pass", "kg_fused"])
+                    choices=["auto", "stages", "kg_2pass", "kg_fused", "schr_fused"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration)")
     ap.add_argument("--adjoint-steps", type=int, default=16,
@@ -256,8 +258,8 @@ def main():
     from wideanglepropagation_amd._lib import stream_bandwidth
     copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
 
-    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass"}.get(
-        sched, f"rk4_pass_s{dom + 1}")
+    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass",
+              "schr_fused": f"schr2_pass_p{dom + 1}"}.get(sched, f"rk4_pass_s{dom + 1}")
     traffic = profiled_traffic(kernel, p)
     cpu = None
     if not args.no_cpu:
@@ -285,7 +287,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
-                                "kg_fused": "kg1_pass"}.get(sched, f"rk4_pass stage {dom + 1}"),
+                                "kg_fused": "kg1_pass",
+                                "schr_fused": f"schr2_pass pass {'AB'[dom]}"}.get(
+                                    sched, f"rk4_pass stage {dom + 1}"),
                      "basis": f"canonical {CANONICAL_BYTES[(p.equation, p.precision)]} B per "
                               "voxel-step (one streaming pass per RK4 stage)",
                      "bytes_per_launch": canon_bytes,

@@ -59,10 +59,14 @@ extern "C" {
 #define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365 */
 
 /* pass schedule of a Klein-Gordon step (Schrodinger always uses 4 stage passes) */
-#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED for f32, KG_2PASS for f64 and f32_kahan */
+#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED for f32, KG_2PASS for f64 and f32_kahan;
+                               * Schrodinger: SCHR_FUSED for f32 (3D), else STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
 #define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state) */
+#define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger f32: the RK4 polynomial in Horner form, two passes
+                                 * of two stencil levels each (AUTO picks it for f32 with
+                                 * frozen end planes) */
 
 /* planes k = 0 and nz-1 (never stepped) */
 #define WAP_ENDS_FROZEN 0  /* stage inputs there = the injected state (fdtd_3d.cu:184-190) */
@@ -86,7 +90,7 @@ typedef struct wap_desc {
   int32_t tile_y;             /* rows per workgroup (64 x tile_y threads); 0 = auto.
                                  KG_FUSED uses 4, 8 or 16 rows (auto 16) */
   int32_t tile_z;             /* planes per thread column; 0 = auto */
-  int32_t schedule;           /* WAP_SCHED_*: how a KG step is split into passes */
+  int32_t schedule;           /* WAP_SCHED_*: how a step is split into passes */
   int32_t end_planes;         /* WAP_ENDS_*: planes k = 0, nz-1 */
   int32_t reserved0;
   double xmin, xmax, ymin, ymax, zmin, zmax; /* XMIN..ZMAX; spacing = extent/(n-1);

@@ -94,17 +94,40 @@ def test_f64_ragged_shapes_and_tiles(shape, ty, tz, eq, sched, rng):
     assert rel_l2(got, want) < 1e-12
 
 
-@pytest.mark.parametrize("eq,sched", [("schrodinger", "auto"), ("klein_gordon", "kg_2pass"),
+@pytest.mark.parametrize("eq,sched", [("schrodinger", "stages"), ("schrodinger", "schr_fused"),
+                                       ("klein_gordon", "kg_2pass"),
                                        ("klein_gordon", "stages"), ("klein_gordon", "kg_fused")])
 @pytest.mark.parametrize("prec", ["f32", "f32_kahan"])
 def test_f32_matches_oracle(eq, sched, prec, rng):
     print(f"case {prec} {eq} {sched}", flush=True)
-    if sched == "kg_fused" and prec == "f32_kahan":
-        pytest.skip("the fused schedule runs f32 and f64")
+    if sched in ("kg_fused", "schr_fused") and prec == "f32_kahan":
+        pytest.skip("the fused schedules run without the Kahan buffer")
     p = _params((24, 16, 16), eq, precision=prec, schedule=sched)
     V, st0 = _random_case(p, rng)
     got, want = _run_both(p, V, st0, 20)
     assert rel_l2(got, want) < 2e-6
+
+
+@pytest.mark.parametrize("shape,ty,tz", [((9, 5, 70), 4, 1), ((9, 5, 70), 8, 3),
+                                          ((11, 13, 130), 16, 100), ((7, 3, 2), 4, 0),
+                                          ((12, 40, 125), 16, 5), ((10, 3, 61), 8, 2),
+                                          ((40, 26, 64), 16, 0)])
+def test_schr_fused_ragged_shapes_and_tiles(shape, ty, tz, rng):
+    # the 2-pass Schrodinger schedule against the FP64 oracle on partial tiles, grids
+    # narrower than a tile, short z chunks, and both Horner passes' frozen planes
+    print(f"case schr_fused {shape} ty={ty} tz={tz}", flush=True)
+    p = _params(shape, "schrodinger", precision="f32", tile_y=ty, tile_z=tz, pml_thick=0.1,
+                k_source=2, schedule="schr_fused")
+    V, st0 = _random_case(p, rng)
+    got, want = _run_both(p, V, st0, 8)
+    assert rel_l2(got, want) < 2e-6
+
+
+def test_schr_fused_is_the_f32_default():
+    p = _params((24, 16, 16), "schrodinger", precision="f32")
+    assert p.resolved_schedule() == "schr_fused"
+    assert p.with_(precision="f32_kahan").resolved_schedule() == "stages"
+    assert p.with_(end_planes="zero").resolved_schedule() == "stages"
 
 
 def test_source_profile_and_reference_clock(rng):
@@ -175,5 +198,9 @@ def test_errors_are_reported_not_fatal():
         FDTD3D(p.with_(schedule="kg_2pass"))
     with pytest.raises(WapError, match="for the Klein-Gordon equation"):
         FDTD3D(p.with_(schedule="kg_fused"))
+    with pytest.raises(WapError, match="Schrodinger fused schedule"):
+        FDTD3D(p.with_(schedule="schr_fused", precision="f64"))
+    with pytest.raises(WapError, match="Schrodinger fused schedule"):
+        FDTD3D(p.with_(schedule="schr_fused", precision="f32", equation="klein_gordon"))
     with pytest.raises(ValueError):
         FDTD3D(p, V=np.zeros(7))

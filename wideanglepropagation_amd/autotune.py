@@ -71,8 +71,8 @@ def tune(params, tile_ys=TILE_Y, tile_zs=TILE_Z, steps=5, warmup=2, use_cache=Tr
          verbose=False):
     """Return (tile_y, tile_z, ms_per_step, table) of the fastest candidate."""
     key = _key(params)
-    if params.resolved_schedule() == "kg_fused" and tile_ys == TILE_Y:
-        tile_ys = (4, 8, 16)   # the fused pass needs >= 3 rows (one output row + halo)
+    if params.resolved_schedule() in ("kg_fused", "schr_fused") and tile_ys == TILE_Y:
+        tile_ys = (4, 8, 16)   # the fused passes need >= 3 rows (one output row + halo)
     cache = load_cache() if use_cache else {}
     if use_cache and key in cache:
         e = cache[key]

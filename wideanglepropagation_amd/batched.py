@@ -27,7 +27,8 @@ class BatchedFDTD:
         groups = {}
         for s in self.sims:
             p = s.params
-            sched = "stages" if p.resolved_schedule() == "stages" else "kg_2pass"
+            # batched launches run the stage passes (Schrodinger) or the 2-pass KG kernels
+            sched = "kg_2pass" if p.resolved_schedule() in ("kg_2pass", "kg_fused") else "stages"
             groups.setdefault((p.equation, p.precision, sched), []).append(s)
         self.groups = []
         self.stream = torch.cuda.current_stream()

@@ -358,8 +358,19 @@ void split_hi_lo(double x, R &hi, R &lo) {
   lo = (R)(x - (double)hi);
 }
 
+// the Schrodinger fused schedule: f32, frozen end planes (the 2D rule zeroes the stage
+// inputs on the end rows, which the RK4 polynomial form does not express), fields < 2 GiB
+bool schr_fused_ok(const wap_desc &d) {
+  return d.equation == WAP_EQ_SCHRODINGER && d.precision == WAP_PREC_F32 &&
+         d.end_planes == WAP_ENDS_FROZEN && fused_plane_fits(d);
+}
+
 int sched_of(const wap_handle *h) {
-  if (!is_kg(h)) return WAP_SCHED_STAGES;
+  if (!is_kg(h)) {
+    if (h->d.schedule == WAP_SCHED_AUTO)
+      return schr_fused_ok(h->d) ? WAP_SCHED_SCHR_FUSED : WAP_SCHED_STAGES;
+    return h->d.schedule;
+  }
   if (h->d.schedule == WAP_SCHED_AUTO)  // the fused pass spills registers in float64
     return h->d.precision == WAP_PREC_F32 && fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED
                                                                    : WAP_SCHED_KG_2PASS;
@@ -462,6 +473,67 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   return WAP_OK;
 }
 
+// Schrodinger f32 step in two fused passes (schr2_pass): A psi -> u3 (ya), B (u3, psi)
+// -> psi' (yb), then the state buffer and yb swap.
+int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                    double *pass_ms) {
+  using R = float;
+  using C = wap::cplx<R>;
+  const int waves = kg1_waves(h);
+  const int ntx = kg1_ntx(h);
+  const int nty = kg1_nty(h);
+  const int tz = kg1_tile_z(h);
+  const int ntz = (h->d.nz + tz - 1) / tz;
+  const dim3 block(wap::kWaveX, waves);
+  const dim3 grid((unsigned)(ntx * nty * ntz));
+  wap::Schr2Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = ntx; a.nty = nty; a.ntz = ntz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  const double dt = h->d.dt;
+  auto launch = [&](auto pb) {
+    constexpr int PB = decltype(pb)::value;
+    switch (waves) {
+      case 4: hipLaunchKernelGGL((wap::schr2_pass<R, 3, PB>), grid, block, 0, h->stream, a); break;
+      case 8: hipLaunchKernelGGL((wap::schr2_pass<R, 7, PB>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::schr2_pass<R, 15, PB>), grid, block, 0, h->stream, a); break;
+    }
+  };
+  for (int64_t n = 0; n < nsteps; ++n) {
+    fill_source(h, step_time(h, step0, n, t0), a);
+    if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
+    a.in = (const C *)h->psi;
+    a.base = (const C *)h->psi;
+    a.out = (C *)h->ya_psi;
+    a.c1 = (R)(dt / 4.0);
+    a.c2 = (R)(dt / 3.0);
+    launch(std::integral_constant<int, 0>{});
+    if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
+    a.in = (const C *)h->ya_psi;
+    a.out = (C *)h->yb_psi;
+    a.c1 = (R)(dt / 2.0);
+    a.c2 = (R)dt;
+    launch(std::integral_constant<int, 1>{});
+    std::swap(h->psi, h->yb_psi);
+    if (ev) {
+      WAP_HIP(hipEventRecord(ev[2], h->stream));
+      WAP_HIP(hipEventSynchronize(ev[2]));
+      for (int s = 0; s < 2; ++s) {
+        float ms = 0.f;
+        WAP_HIP(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+        pass_ms[s] += ms;
+      }
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
 template <typename R>
 wap::KG2Args<R> kg2_args(const wap_handle *h, int ty = 0) {
   using C = wap::cplx<R>;
@@ -531,6 +603,7 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
+  if (sched_of(h) == WAP_SCHED_SCHR_FUSED) return run_steps_schr2(h, nsteps, step0, t0, ev, pass_ms);
   if (sched_of(h) == WAP_SCHED_KG_FUSED) {
     if (h->d.precision == WAP_PREC_F64) return run_steps_kg1<double>(h, nsteps, step0, t0, ev, pass_ms);
     return run_steps_kg1<float>(h, nsteps, step0, t0, ev, pass_ms);
@@ -607,8 +680,11 @@ int validate(const wap_desc *d) {
   if (d->tile_y < 0 || d->tile_y > 16 || (d->tile_y & (d->tile_y - 1)))
     return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8, 16");
   if (d->tile_z < 0) return fail(WAP_ERR_ARG, "tile_z must be >= 0");
-  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_KG_FUSED)
+  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_SCHR_FUSED)
     return fail(WAP_ERR_ARG, "unknown schedule");
+  if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
+    return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, f32, "
+                             "frozen end planes and fields under 2 GiB");
   if ((d->schedule == WAP_SCHED_KG_2PASS || d->schedule == WAP_SCHED_KG_FUSED) &&
       d->equation != WAP_EQ_KLEIN_GORDON)
     return fail(WAP_ERR_ARG, "the 2-pass and fused schedules are for the Klein-Gordon equation");
@@ -978,7 +1054,7 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
-  if (sched_of(h) == WAP_SCHED_KG_FUSED) {
+  if (sched_of(h) == WAP_SCHED_KG_FUSED || sched_of(h) == WAP_SCHED_SCHR_FUSED) {
     const int tz = kg1_tile_z(h);
     if (tile_y) *tile_y = kg1_waves(h);
     if (tile_z) *tile_z = tz;

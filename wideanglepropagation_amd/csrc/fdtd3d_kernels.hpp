@@ -900,6 +900,221 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   }
 }
 
+// ----------------------------------------- Schrodinger RK4 step in 2 fused passes
+//
+// For the linear system d psi / dt = M psi, M = -i H (H = -A Lap_pml + V, and zero on
+// the frozen end planes), one classical RK4 step from b = inject(psi_n) equals the
+// polynomial P4(h M) b, evaluated here in Horner form as four stencil levels:
+//   u4 = b + h/4 M b,  u3 = b + h/3 M u4,  u2 = b + h/2 M u3,  psi' = b + h M u2.
+// Each pass runs two levels with the layout of kg1_pass (level 1 at plane m, level 2
+// at plane m - 1, double-buffered LDS planes, one barrier per plane):
+//   pass A (PB = 0) reads psi_n, V and writes u3;
+//   pass B (PB = 1) reads u3, psi_n, V and writes psi_{n+1} to a third buffer
+// (other workgroups read psi_n at their side columns and halo rows, so it cannot be
+// overwritten in place): 20 + 28 = 48 B per voxel-step, against 144 B for one
+// streaming pass per stage.  The soft source is applied to psi_n on load in both
+// passes.  On the frozen planes the level factor is 0, so u = b exactly.
+//   H y = hx (x+ + x- - 2 y) + hy (y+ + y- - 2 y) + wp (z+ - y) - wm (y - z-) + V y,
+// hx, hy = -A / d^2, and wp, wm the complex per-plane z weights (PML) staged in LDS.
+template <typename R>
+struct Schr2Args {
+  using C = cplx<R>;
+  const C *in;    // A: psi_n; B: u3
+  const C *base;  // B: psi_n (b = inject(psi_n) again); A: unused
+  C *out;         // A: u3; B: psi_{n+1}
+  const R *V;
+  const C *zc;    // zc[2k], zc[2k+1]: weights of y_{k+1}, y_{k-1} on plane k
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  R hx, hy;
+  R c1, c2;  // the pass's level factors: A (h/4, h/3), B (h/2, h)
+  int src_k0, src_n;
+  const SrcCoef<R> *src_tab;
+  C src_ph;
+};
+
+// H y from the lateral neighbour sums sx, sy and the z neighbours zp, zm
+template <typename R>
+__device__ __forceinline__ cplx<R> schr_h(const Schr2Args<R> &p, R v, cplx<R> wp, cplx<R> wm,
+                                          cplx<R> y, cplx<R> sx, cplx<R> sy, cplx<R> zp,
+                                          cplx<R> zm) {
+  const cplx<R> dp = csub<R>(zp, y), dm = csub<R>(y, zm);
+  const R lx = sx.x - R(2) * y.x, ly = sy.x - R(2) * y.x;
+  const R mx = sx.y - R(2) * y.y, my = sy.y - R(2) * y.y;
+  return mkc<R>(p.hx * lx + p.hy * ly + (wp.x * dp.x - wp.y * dp.y) - (wm.x * dm.x - wm.y * dm.y) + v * y.x,
+                p.hx * mx + p.hy * my + (wp.x * dp.y + wp.y * dp.x) - (wm.x * dm.y + wm.y * dm.x) + v * y.y);
+}
+
+// b + f M y = b + f (-i H y)
+template <typename R>
+__device__ __forceinline__ cplx<R> horner(cplx<R> b, R f, cplx<R> hy) {
+  return mkc<R>(b.x + f * hy.y, b.y - f * hy.x);
+}
+
+template <typename R, int ROWS, int PB>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass(
+    const Schr2Args<R> p) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
+  constexpr int pw = kWaveX + 4;
+  constexpr int nps = pw * (ROWS + 2);
+  __shared__ C lsb[2][nps];  // level-1 input of the plane (A: b, B: u3) with the halo ring
+  __shared__ C lub[2][nps];  // level-1 output (A: u4, B: u2): tile and side columns
+  __shared__ C lzc[2 * (kKG1MaxTileZ + 2)];
+  __shared__ SrcCoef<R> lsrc[kMaxBand];
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const bool side = wv == ROWS;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const int sl = lx % (2 * ROWS);
+  const int r = side ? sl % ROWS : wv;
+  const bool right = sl >= ROWS;
+  const int c = side ? (right ? kWaveX + 2 : 1) : lx + 2;
+  const int gi = side ? (right ? tx * kWaveX + kWaveX : tx * kWaveX - 1) : tx * kWaveX + lx;
+  const int gj = tyi * (ROWS - 2) - 1 + r;
+  const int i = ((gi % nx) + nx) % nx;
+  const int j = ((gj % ny) + ny) % ny;
+  const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
+  const bool writer = wave_writes && gi < nx && gj < ny;
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rbs = buf_rsrc(PB ? p.base : p.in);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const int ij = j * nx + i;
+  const int ss = (r + 1) * pw + c;
+  int oo = -1, hs = ss;
+  if (side) {
+    oo = j * nx + (right ? ((i + 1 == nx) ? 0 : i + 1) : ((i == 0) ? nx - 1 : i - 1));
+    hs = right ? ss + 1 : ss - 1;
+  } else if (wv == 0) {
+    oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+    hs = ss - pw;
+  } else if (wv == ROWS - 1) {
+    oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+    hs = ss + pw;
+  }
+  const bool has_o = oo >= 0;
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t ow = writer ? oc : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C ph = p.src_ph;
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+
+  struct Raw {  // one plane of this lane: level-1 input own and outer neighbour, base, V
+    C s, os, b;
+    R v;
+  };
+  auto load_plane = [&](int m, Raw &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    w.s = buf_ld<C>(rin, oc, bc);
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    w.os = buf_ld<C>(rin, ooc, bc);
+    if (PB) w.b = buf_ld<C>(rbs, oc, bc);
+  };
+  // the soft source on psi_n: A injects its level-1 input, B its base
+  auto inject_plane = [&](int m, Raw &w) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = lsrc[bnd].keep;
+    sp.c = cmul<R>(lsrc[bnd].c, ph);
+    sp.cq = sp.c;
+    if (PB) {
+      w.b = inj<R>(w.b, sp, sp.c, prof, ij);
+    } else {
+      w.s = inj<R>(w.s, sp, sp.c, prof, ij);
+      if (has_o) w.os = inj<R>(w.os, sp, sp.c, prof, oo);
+    }
+  };
+  auto base_of = [&](const Raw &w) { return PB ? w.b : w.s; };
+  auto store_state = [&](int b, const Raw &w) {
+    lsb[b][ss] = w.s;
+    if (has_o) lsb[b][hs] = w.os;
+  };
+
+  const int m_start = max(k0 - 1, 0);
+  const int m_end = (k1 == nz) ? nz - 1 : k1;  // inclusive
+  {
+    const int tid = wv * kWaveX + lx;
+    for (int t = tid; t <= m_end - m_start; t += kWaveX * (ROWS + 1)) {
+      lzc[2 * t] = p.zc[2 * (m_start + t)];
+      lzc[2 * t + 1] = p.zc[2 * (m_start + t) + 1];
+    }
+    if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
+  }
+  Raw r0, r1, r2, r3;
+  load_plane(m_start - 1, r0);
+  load_plane(m_start, r1);
+  load_plane(m_start + 1, r2);
+  __syncthreads();  // lsrc, lzc
+  inject_plane(m_start - 1, r0);
+  inject_plane(m_start, r1);
+  store_state(m_start & 1, r1);
+  __syncthreads();
+
+  C uprev = zero, umm = zero;  // level-1 outputs at m-1 and m-2 (this column)
+
+  // one plane: ra, rb hold planes m-1, m (prepared), rc plane m+1 (raw), pre receives m+2
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
+    const int rd = m & 1, wr = rd ^ 1;
+    load_plane(m + 2, pre);
+    inject_plane(m + 1, rc);
+    // ---- level 1 at m
+    C ucur;
+    {
+      const bool frozen = m == 0 || m == nz - 1;
+      const C *ls = lsb[rd];
+      const C sx = cadd<R>(ls[ss + 1], ls[ss - 1]), sy = cadd<R>(ls[ss + pw], ls[ss - pw]);
+      const int t = m - m_start;
+      const C hyv = schr_h<R>(p, rb.v, lzc[2 * t], lzc[2 * t + 1], rb.s, sx, sy, rc.s, ra.s);
+      ucur = horner<R>(base_of(rb), frozen ? R(0) : p.c1, hyv);
+    }
+    // ---- level 2 at k = m-1 (stored when inside the chunk); k <= nz - 2 since m <= nz - 1
+    const int k = m - 1;
+    if (wave_writes) {
+      const C *lu = lub[rd];
+      const C sx = cadd<R>(lu[ss + 1], lu[ss - 1]), sy = cadd<R>(lu[ss + pw], lu[ss - pw]);
+      const int t = max(k, m_start) - m_start;
+      const C hyv = schr_h<R>(p, ra.v, lzc[2 * t], lzc[2 * t + 1], uprev, sx, sy, ucur, umm);
+      const C o = horner<R>(base_of(ra), k == 0 ? R(0) : p.c2, hyv);
+      const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
+      buf_st<C>(rout, (k >= k0) ? ow : kBufSkip, bc, o);
+    }
+    lub[wr][ss] = ucur;
+    store_state(wr, rc);
+    __syncthreads();  // plane m+1 / m visible; every lane is done reading buffer rd
+    umm = uprev;
+    uprev = ucur;
+  };
+
+  int m = m_start;
+  for (; m + 3 <= m_end; m += 4) {
+    plane_step(m, r0, r1, r2, r3);
+    plane_step(m + 1, r1, r2, r3, r0);
+    plane_step(m + 2, r2, r3, r0, r1);
+    plane_step(m + 3, r3, r0, r1, r2);
+  }
+  if (m <= m_end) plane_step(m, r0, r1, r2, r3);
+  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0);
+  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1);
+  // the frozen top plane nz-1 of the last chunk: its level-1 output is b exactly
+  if (k1 == nz) {
+    const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
+    buf_st<C>(rout, ow, bc, uprev);
+  }
+}
+
 // ------------------------------------------------------------------ adjoint
 //
 // Reverse sweep of one RK4 step u' = b + h/6 (k1 + 2k2 + 2k3 + k4), k_s = M Y_s,

@@ -30,6 +30,7 @@ PML_THETA_DEFAULT = 0.785
 PML_POWER_DEFAULT = 3.0
 
 EQUATIONS = ("schrodinger", "klein_gordon")
+SCHEDULES = ("auto", "stages", "kg_2pass", "kg_fused", "schr_fused")
 PRECISIONS = ("f64", "f32", "f32_kahan")
 
 
@@ -98,7 +99,7 @@ class FDTDParams:
     src_alpha: float = SRC_ALPHA
     tile_y: int = 0
     tile_z: int = 0
-    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused"
+    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused" | "schr_fused"
     end_planes: str = "frozen"        # "frozen" (3D reference) | "zero" (2D reference)
 
     def __post_init__(self):
@@ -106,8 +107,8 @@ class FDTDParams:
             raise ValueError(f"equation must be one of {EQUATIONS}")
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
-        if self.schedule not in ("auto", "stages", "kg_2pass", "kg_fused"):
-            raise ValueError("schedule must be 'auto', 'stages', 'kg_2pass' or 'kg_fused'")
+        if self.schedule not in SCHEDULES:
+            raise ValueError(f"schedule must be one of {SCHEDULES}")
         if self.end_planes not in ("frozen", "zero"):
             raise ValueError("end_planes must be 'frozen' or 'zero'")
         if self.time_mode not in ("exact", "f32"):
@@ -135,11 +136,14 @@ class FDTDParams:
 
     def resolved_schedule(self):
         """The pass schedule the library runs for these parameters."""
+        # the fused passes address a field with 32-bit offsets
+        fits = self.nx * self.ny * self.nz * (16 if self.precision == "f64" else 8) < 2 ** 31
         if self.equation != "klein_gordon":
-            return "stages"
+            if self.schedule == "auto":
+                ok = self.precision == "f32" and self.end_planes == "frozen" and fits
+                return "schr_fused" if ok else "stages"
+            return self.schedule
         if self.schedule == "auto":
-            # the fused pass addresses a field with 32-bit offsets
-            fits = self.nx * self.ny * self.nz * 8 < 2 ** 31
             return "kg_fused" if self.precision == "f32" and fits else "kg_2pass"
         return self.schedule
 

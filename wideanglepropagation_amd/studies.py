@@ -147,7 +147,7 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
         ext = centred_extent(shape, spacing, spacing_z)
         mk = schrodinger_params if eq == "schrodinger" else klein_gordon_params
         p = mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18, precision="f32",
-               schedule="kg_2pass" if eq == "klein_gordon" else "auto")
+               schedule="kg_2pass" if eq == "klein_gordon" else "stages")
         key = (eq, shape)
         if autotune and key not in tuned:
             ty, tz, ms, _ = tune(p, tile_ys=(4,), tile_zs=(4, 8, 16, 32), steps=3, warmup=1)
