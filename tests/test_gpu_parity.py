@@ -204,3 +204,24 @@ def test_errors_are_reported_not_fatal():
         FDTD3D(p.with_(schedule="schr_fused", precision="f32", equation="klein_gordon"))
     with pytest.raises(ValueError):
         FDTD3D(p, V=np.zeros(7))
+
+
+def test_schr_fused_precision_64cubed_200_steps():
+    # the 64^3 wavepacket over 200 steps: the Horner-form 2-pass schedule in float32 is
+    # as close to the FP64 run as one pass per stage in float32
+    from wideanglepropagation_amd.workloads import schrodinger_gaussian_64
+    FDTD3D = _gpu()
+    print("case 64^3 wavepacket f32 schr_fused vs stages vs f64", flush=True)
+    p64, V, psi0, nsteps = schrodinger_gaussian_64("f64")
+    out = {}
+    for name, p in (("f64", p64), ("stages", p64.with_(precision="f32", schedule="stages")),
+                    ("fused", p64.with_(precision="f32", schedule="schr_fused"))):
+        sim = FDTD3D(p, V=V.astype(np.float64 if name == "f64" else np.float32))
+        sim.set_state(psi0)
+        sim.step(nsteps)
+        out[name] = sim.get_state(np.complex128)
+        sim.close()
+    e_st = rel_l2(out["stages"], out["f64"])
+    e_fu = rel_l2(out["fused"], out["f64"])
+    print(f"rel L2 vs f64: stages {e_st:.3e}, fused {e_fu:.3e}", flush=True)
+    assert e_fu < 1e-5 and e_fu < 1.5 * e_st + 1e-7
