@@ -27,7 +27,9 @@ What is new (only through keyword arguments)
   source (bool; the reference always injects), source_profile (ny, nx),
   exit_plane (k: a GPU-accumulated demodulated average of plane k over the
   saved frames, returned as "exit_wave"), backend ("hip" | "cli": run the
-  ``fdtd_3d_bin`` command through the file contract), return_timing.
+  ``fdtd_3d_bin`` command through the file contract), return_timing,
+  schedule ("auto" | "stages" | "schr_fused" | "kg_2pass" | "kg_fused": how a step
+  is split into passes; "auto" fuses where the precision allows), tile_y, tile_z.
 
 Deviations (documented): AU mode computes P_INF from the wavelength instead
 of leaving a stale value in the source (``fdtd_solver.py:344-349``); the
@@ -405,6 +407,8 @@ class RelativisticSolver3D(RelativisticSolver):
         for key in ("tile_y", "tile_z"):
             if key in kw:
                 extra[key] = int(kw[key])
+        if "schedule" in kw:
+            extra["schedule"] = str(kw["schedule"])
         if equation == "klein_gordon":
             if not use_angstrom_units:
                 raise ValueError("klein_gordon needs use_angstrom_units=True")
