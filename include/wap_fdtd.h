@@ -176,16 +176,20 @@ int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *g
  * Batches: many independent grids (the reference runs one grid per process)
  * stepped by one launch per pass.  All handles must share equation, precision
  * and schedule and have V set; sizes, energies, dt and sources may differ.
+ * Fused schedules (KG_FUSED, SCHR_FUSED) run their fused passes over all grids;
+ * they ping-pong each grid's state, so after an odd number of batched steps
+ * (or replays of an odd-length graph) the handles' state buffers are swapped.
  * The batch keeps pointers to the handles' buffers: destroy it before them.
  */
 typedef struct wap_batch wap_batch;
 
-/* Group n handles; tile_y = workgroup rows for all (0 = the first handle's). */
+/* Group n handles; tile_y = workgroup rows for all (0 = the first handle's; the
+ * fused schedules take 4, 8 or 16 waves, the others 1, 2, 4 or 8 rows). */
 int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **out);
 int wap_batch_destroy(wap_batch *b);
 /* Stream of the batch's launches (default: the first handle's stream). */
 int wap_batch_set_stream(wap_batch *b, void *stream);
-/* Total workgroups per pass and the shared tile_y. */
+/* Total workgroups per pass and the shared tile_y (fused: waves per workgroup). */
 int wap_batch_tiles(wap_batch *b, int64_t *tiles, int32_t *tile_y);
 /* Advance every grid nsteps steps (step n of grid i injects at its own t_n). */
 int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0);

@@ -110,3 +110,41 @@ def test_batch_rejects_mixed_precision(rng):
     assert rc != 0 and b"share equation" in lib().wap_last_error()
     for s in sims:
         s.close()
+
+
+@pytest.mark.parametrize("eq", ["schrodinger", "klein_gordon"])
+def test_fused_batch_odd_steps_and_odd_graph(eq, rng):
+    # fused batches ping-pong the states: odd eager steps and replays of an odd-length
+    # graph leave the state in the other buffer, which the handles must then point to
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.batched import BatchedFDTD
+    print(f"case fused batch odd steps {eq}", flush=True)
+    cases = _cases(rng, eq, "f32", n=4)
+    assert all(p.resolved_schedule() in ("kg_fused", "schr_fused") for p, _, _ in cases)
+    single = _make(cases)
+    for s in single:
+        s.step(9)
+    want = [s.get_state(np.complex64) for s in single]
+    sims = _make(cases)
+    b = BatchedFDTD(sims)
+    b.step(3)
+    b.capture(3)
+    b.replay()
+    b.replay()
+    b.synchronize()
+    for s, w in zip(sims, want):
+        got = s.get_state(np.complex64)
+        for g, x in zip(got if isinstance(got, tuple) else (got,),
+                        w if isinstance(w, tuple) else (w,)):
+            assert np.linalg.norm(g - x) <= 1e-6 * np.linalg.norm(x)
+    # the handles keep stepping on their own after the batch
+    for s in single + sims:
+        s.step(2)
+    for s, t in zip(single, sims):
+        a, c = s.get_state(np.complex64), t.get_state(np.complex64)
+        for g, x in zip(c if isinstance(c, tuple) else (c,), a if isinstance(a, tuple) else (a,)):
+            assert np.linalg.norm(g - x) <= 1e-6 * np.linalg.norm(x)
+    b.close()
+    for s in single + sims:
+        s.close()

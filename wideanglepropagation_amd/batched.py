@@ -27,15 +27,15 @@ class BatchedFDTD:
         groups = {}
         for s in self.sims:
             p = s.params
-            # batched launches run the stage passes (Schrodinger) or the 2-pass KG kernels
-            sched = "kg_2pass" if p.resolved_schedule() in ("kg_2pass", "kg_fused") else "stages"
-            groups.setdefault((p.equation, p.precision, sched), []).append(s)
+            groups.setdefault((p.equation, p.precision, p.resolved_schedule()), []).append(s)
         self.groups = []
         self.stream = torch.cuda.current_stream()
         for key, members in groups.items():
             arr = (ctypes.c_void_p * len(members))(*[m.handle.value for m in members])
             b = ctypes.c_void_p()
-            check(self._L.wap_batch_create(arr, len(members), int(tile_y), ctypes.byref(b)),
+            # fused groups keep their handles' workgroup shape (64 x 4, 8 or 16 waves)
+            ty = 0 if key[2] in ("kg_fused", "schr_fused") else int(tile_y)
+            check(self._L.wap_batch_create(arr, len(members), ty, ctypes.byref(b)),
                   "wap_batch_create")
             self.groups.append((key, members, b))
         self.step_index = 0

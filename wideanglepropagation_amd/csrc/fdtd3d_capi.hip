@@ -7,12 +7,14 @@
 // voxel on the device.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -421,31 +423,41 @@ int kg1_tile_z(const wap_handle *h, int waves, int per_cu) {
 }
 int kg1_tile_z(const wap_handle *h) { return kg1_tile_z(h, kg1_waves(h), 16 / kg1_waves(h)); }
 
-template <typename R>
-int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
-                  double *pass_ms) {
-  using C = wap::cplx<R>;
-  const int waves = kg1_waves(h);
-  const int ntx = kg1_ntx(h);
-  const int nty = kg1_nty(h);
-  const int tz = kg1_tile_z(h);
-  const int ntz = (h->d.nz + tz - 1) / tz;
-  const dim3 block(wap::kWaveX, waves);
-  const dim3 grid((unsigned)(ntx * nty * ntz));
+// tiles of the fused passes over one grid with `waves` waves per workgroup
+int64_t fused_tiles(const wap_handle *h, int waves) {
+  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  return (int64_t)kg1_ntx(h) * kg1_nty(h, waves) * ((h->d.nz + tz - 1) / tz);
+}
 
+// the launch-invariant arguments of kg1_pass (buffers and source are set per launch)
+template <typename R>
+wap::KG1Args<R> kg1_args(const wap_handle *h, int waves) {
+  using C = wap::cplx<R>;
   wap::KG1Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
+  const int tz = kg1_tile_z(h, waves, 16 / waves);
   a.V = (const R *)h->V;
   a.gam = (const R *)h->gam;
   a.prof = (const C *)h->prof;
   a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
   a.tile_z = tz;
-  a.ntx = ntx; a.nty = nty; a.ntz = ntz;
+  a.ntx = kg1_ntx(h); a.nty = kg1_nty(h, waves); a.ntz = (h->d.nz + tz - 1) / tz;
   const double hz = kg_hz(h);
   a.hx = (R)h->hx; a.hy = (R)h->hy;
   split_hi_lo(hz, a.hz_hi, a.hz_lo);
   split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + hz) + h->m2, a.d0_hi, a.d0_lo);
   a.h = (R)h->d.dt;
+  return a;
+}
+
+template <typename R>
+int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                  double *pass_ms) {
+  using C = wap::cplx<R>;
+  const int waves = kg1_waves(h);
+  const dim3 block(wap::kWaveX, waves);
+  const dim3 grid((unsigned)fused_tiles(h, waves));
+  wap::KG1Args<R> a = kg1_args<R>(h, waves);
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
     a.in_psi = (const C *)h->psi;
@@ -473,6 +485,26 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   return WAP_OK;
 }
 
+// the launch-invariant arguments of schr2_pass; pass 0 (A) or 1 (B) sets the level factors
+wap::Schr2Args<float> schr2_args(const wap_handle *h, int waves, int pass) {
+  using R = float;
+  using C = wap::cplx<R>;
+  wap::Schr2Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = kg1_ntx(h); a.nty = kg1_nty(h, waves); a.ntz = (h->d.nz + tz - 1) / tz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  const double dt = h->d.dt;
+  a.c1 = (R)(pass == 0 ? dt / 4.0 : dt / 2.0);
+  a.c2 = (R)(pass == 0 ? dt / 3.0 : dt);
+  return a;
+}
+
 // Schrodinger f32 step in two fused passes (schr2_pass): A psi -> u3 (ya), B (u3, psi)
 // -> psi' (yb), then the state buffer and yb swap.
 int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
@@ -480,24 +512,11 @@ int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
   using R = float;
   using C = wap::cplx<R>;
   const int waves = kg1_waves(h);
-  const int ntx = kg1_ntx(h);
-  const int nty = kg1_nty(h);
-  const int tz = kg1_tile_z(h);
-  const int ntz = (h->d.nz + tz - 1) / tz;
   const dim3 block(wap::kWaveX, waves);
-  const dim3 grid((unsigned)(ntx * nty * ntz));
-  wap::Schr2Args<R> a;
-  std::memset((void *)&a, 0, sizeof(a));
-  a.V = (const R *)h->V;
-  a.zc = (const C *)h->zc;
-  a.prof = (const C *)h->prof;
-  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
-  a.tile_z = tz;
-  a.ntx = ntx; a.nty = nty; a.ntz = ntz;
-  a.hx = (R)h->hx; a.hy = (R)h->hy;
-  const double dt = h->d.dt;
-  auto launch = [&](auto pb) {
-    constexpr int PB = decltype(pb)::value;
+  const dim3 grid((unsigned)fused_tiles(h, waves));
+  wap::Schr2Args<R> pa = schr2_args(h, waves, 0), pb = schr2_args(h, waves, 1);
+  auto launch = [&](auto pass, const wap::Schr2Args<R> &a) {
+    constexpr int PB = decltype(pass)::value;
     switch (waves) {
       case 4: hipLaunchKernelGGL((wap::schr2_pass<R, 3, PB>), grid, block, 0, h->stream, a); break;
       case 8: hipLaunchKernelGGL((wap::schr2_pass<R, 7, PB>), grid, block, 0, h->stream, a); break;
@@ -505,20 +524,18 @@ int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
     }
   };
   for (int64_t n = 0; n < nsteps; ++n) {
-    fill_source(h, step_time(h, step0, n, t0), a);
+    const double t = step_time(h, step0, n, t0);
+    fill_source(h, t, pa);
+    fill_source(h, t, pb);
+    pa.in = (const C *)h->psi;
+    pa.out = (C *)h->ya_psi;
+    pb.in = (const C *)h->ya_psi;
+    pb.base = (const C *)h->psi;
+    pb.out = (C *)h->yb_psi;
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    a.in = (const C *)h->psi;
-    a.base = (const C *)h->psi;
-    a.out = (C *)h->ya_psi;
-    a.c1 = (R)(dt / 4.0);
-    a.c2 = (R)(dt / 3.0);
-    launch(std::integral_constant<int, 0>{});
+    launch(std::integral_constant<int, 0>{}, pa);
     if (ev) WAP_HIP(hipEventRecord(ev[1], h->stream));
-    a.in = (const C *)h->ya_psi;
-    a.out = (C *)h->yb_psi;
-    a.c1 = (R)(dt / 2.0);
-    a.c2 = (R)dt;
-    launch(std::integral_constant<int, 1>{});
+    launch(std::integral_constant<int, 1>{}, pb);
     std::swap(h->psi, h->yb_psi);
     if (ev) {
       WAP_HIP(hipEventRecord(ev[2], h->stream));
@@ -1073,6 +1090,8 @@ int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *g
 struct wap_batch {
   std::vector<wap_handle *> hs;
   bool kg2 = false;
+  int sched = WAP_SCHED_STAGES;  // the handles' schedule (all equal)
+  int waves = 16;                // fused schedules: waves per workgroup
   int precision = 0;
   int equation = 0;
   int tile_y = 4;
@@ -1113,9 +1132,71 @@ void timeless_source(const wap_handle *h, A &a) {
 
 // (Re)write the per-instance arguments into the batch's device arrays (the
 // arrays themselves never move: a captured graph keeps pointing at them).
+bool batch_fused(const wap_batch *b) {
+  return b->sched == WAP_SCHED_KG_FUSED || b->sched == WAP_SCHED_SCHR_FUSED;
+}
+
+// Fused batches ping-pong every instance's state: step t of a launch sequence uses the
+// argument set of parity t & 1 (parity 0 reads the handle's current state buffer).
+template <typename R>
+int batch_upload_fused(wap_batch *b) {
+  using C = wap::cplx<R>;
+  const int n = (int)b->hs.size();
+  WAP_HIP(hipStreamSynchronize(b->stream));
+  for (int par = 0; par < 2; ++par) {
+    if (b->sched == WAP_SCHED_KG_FUSED) {
+      std::vector<wap::KG1Args<R>> a(n);
+      for (int i = 0; i < n; ++i) {
+        const wap_handle *h = b->hs[i];
+        a[i] = kg1_args<R>(h, b->waves);
+        timeless_source(h, a[i]);
+        a[i].in_psi = (const C *)(par ? h->ya_psi : h->psi);
+        a[i].in_pi = (const C *)(par ? h->ya_pi : h->pi);
+        a[i].out_psi = (C *)(par ? h->psi : h->ya_psi);
+        a[i].out_pi = (C *)(par ? h->pi : h->ya_pi);
+      }
+      WAP_HIP(hipMemcpy(b->d_args[par], a.data(), n * sizeof(a[0]), hipMemcpyHostToDevice));
+    } else if constexpr (std::is_same<R, float>::value) {
+      std::vector<wap::Schr2Args<float>> pa(n), pb(n);
+      for (int i = 0; i < n; ++i) {
+        const wap_handle *h = b->hs[i];
+        const C *cur = (const C *)(par ? h->yb_psi : h->psi);
+        pa[i] = schr2_args(h, b->waves, 0);
+        pb[i] = schr2_args(h, b->waves, 1);
+        timeless_source(h, pa[i]);
+        timeless_source(h, pb[i]);
+        pa[i].in = cur;
+        pa[i].out = (C *)h->ya_psi;
+        pb[i].in = (const C *)h->ya_psi;
+        pb[i].base = cur;
+        pb[i].out = (C *)(par ? h->psi : h->yb_psi);
+      }
+      WAP_HIP(hipMemcpy(b->d_args[2 * par], pa.data(), n * sizeof(pa[0]), hipMemcpyHostToDevice));
+      WAP_HIP(hipMemcpy(b->d_args[2 * par + 1], pb.data(), n * sizeof(pb[0]), hipMemcpyHostToDevice));
+    }
+  }
+  b->baked_psi.resize(n);
+  for (int i = 0; i < n; ++i) b->baked_psi[i] = b->hs[i]->psi;
+  return WAP_OK;
+}
+
+// after an odd number of fused batched steps the state sits in the other buffer
+void batch_flip(wap_batch *b, int64_t nsteps) {
+  if (!batch_fused(b) || !(nsteps & 1)) return;
+  for (wap_handle *h : b->hs) {
+    if (b->sched == WAP_SCHED_KG_FUSED) {
+      std::swap(h->psi, h->ya_psi);
+      std::swap(h->pi, h->ya_pi);
+    } else {
+      std::swap(h->psi, h->yb_psi);
+    }
+  }
+}
+
 template <typename R>
 int batch_upload(wap_batch *b) {
   const int n = (int)b->hs.size();
+  if (batch_fused(b)) return batch_upload_fused<R>(b);
   if (b->kg2) {
     std::vector<wap::KG2Args<R>> a(n);
     for (int i = 0; i < n; ++i) {
@@ -1148,13 +1229,23 @@ int batch_build(wap_batch *b) {
   for (int i = 0; i < n; ++i) {
     const wap_handle *h = b->hs[i];
     const int nty = (h->d.ny + b->tile_y - 1) / b->tile_y;
-    begin[i + 1] = begin[i] + h->ntx * nty * h->ntz;
+    const int64_t t = batch_fused(b) ? fused_tiles(h, b->waves) : (int64_t)h->ntx * nty * h->ntz;
+    if (begin[i] + t > INT32_MAX) return fail(WAP_ERR_ARG, "too many tiles in one batch");
+    begin[i + 1] = begin[i] + (int)t;
   }
   b->total_tiles = begin[n];
   WAP_HIP(hipMalloc((void **)&b->d_tile_begin, (n + 1) * sizeof(int)));
   WAP_HIP(hipMemcpy(b->d_tile_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-  const size_t one = b->kg2 ? sizeof(wap::KG2Args<R>) : sizeof(wap::PassArgs<R>);
-  for (int s = 0; s < (b->kg2 ? 1 : 4); ++s) WAP_HIP(hipMalloc(&b->d_args[s], n * one));
+  size_t one = b->kg2 ? sizeof(wap::KG2Args<R>) : sizeof(wap::PassArgs<R>);
+  int nargs = b->kg2 ? 1 : 4;
+  if (b->sched == WAP_SCHED_KG_FUSED) {
+    one = sizeof(wap::KG1Args<R>);
+    nargs = 2;
+  } else if (b->sched == WAP_SCHED_SCHR_FUSED) {
+    one = sizeof(wap::Schr2Args<float>);
+    nargs = 4;
+  }
+  for (int s = 0; s < nargs; ++s) WAP_HIP(hipMalloc(&b->d_args[s], n * one));
   return batch_upload<R>(b);
 }
 
@@ -1192,6 +1283,41 @@ int batch_phases(wap_batch *b, int64_t nsteps, int64_t step0) {
   return WAP_OK;
 }
 
+template <typename R>
+void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
+  const int n = (int)b->hs.size();
+  const dim3 block(wap::kWaveX, b->waves);
+  const dim3 grid((unsigned)b->total_tiles);
+  const int par = (int)(t & 1);
+  if (b->sched == WAP_SCHED_KG_FUSED) {
+    const wap::KG1Args<R> *a = (const wap::KG1Args<R> *)b->d_args[par];
+    switch (b->waves) {
+      case 4: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 3>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+      case 8: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 7>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+      default: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 15>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+    }
+    return;
+  }
+  if constexpr (std::is_same<R, float>::value) {
+    const wap::Schr2Args<float> *pa = (const wap::Schr2Args<float> *)b->d_args[2 * par];
+    const wap::Schr2Args<float> *pb = (const wap::Schr2Args<float> *)b->d_args[2 * par + 1];
+    switch (b->waves) {
+      case 4:
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 3, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 3, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
+        break;
+      case 8:
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 7, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 7, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
+        break;
+      default:
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 15, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
+        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 15, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
+        break;
+    }
+  }
+}
+
 template <typename R, int EQ, bool K>
 int batch_launch(wap_batch *b, int64_t nsteps) {
   using C = wap::cplx<R>;
@@ -1200,7 +1326,9 @@ int batch_launch(wap_batch *b, int64_t nsteps) {
   const dim3 grid((unsigned)b->total_tiles);
   for (int64_t t = 0; t < nsteps; ++t) {
     const C *ph = (const C *)b->d_phase + (size_t)t * n;
-    if (b->kg2) {
+    if (batch_fused(b)) {
+      batch_launch_fused<R>(b, t, ph);
+    } else if (b->kg2) {
       const wap::KG2Args<R> *a = (const wap::KG2Args<R> *)b->d_args[0];
       hipLaunchKernelGGL((wap::kg2_pass_batched<R, 1, K>), grid, block, 0, b->stream, a,
                          b->d_tile_begin, n, ph);
@@ -1246,22 +1374,29 @@ extern "C" {
 int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **out) {
   if (!out || !handles || n < 1) return fail(WAP_ERR_ARG, "need at least one handle");
   *out = nullptr;
-  if (tile_y < 0 || tile_y > 8 || (tile_y & (tile_y - 1)))
-    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8");
+  if (tile_y < 0 || tile_y > 16 || (tile_y & (tile_y - 1)))
+    return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8, 16");
   const wap_handle *h0 = handles[0];
   if (!h0) return fail(WAP_ERR_ARG, "NULL handle");
   for (int i = 0; i < n; ++i) {
     const wap_handle *h = handles[i];
     if (!h) return fail(WAP_ERR_ARG, "NULL handle");
     if (h->d.equation != h0->d.equation || h->d.precision != h0->d.precision ||
-        kg_two_pass(h) != kg_two_pass(h0))
+        sched_of(h) != sched_of(h0))
       return fail(WAP_ERR_ARG, "batched handles must share equation, precision and schedule");
     if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set on a batched handle");
   }
+  const int s0 = sched_of(h0);
+  const bool fused = s0 == WAP_SCHED_KG_FUSED || s0 == WAP_SCHED_SCHR_FUSED;
+  if (fused ? (tile_y != 0 && tile_y < 4) : tile_y > 8)
+    return fail(WAP_ERR_ARG, fused ? "fused batches take tile_y 0 (auto), 4, 8 or 16"
+                                   : "tile_y must be 0 (auto) or 1, 2, 4, 8");
   wap_batch *b = new (std::nothrow) wap_batch();
   if (!b) return fail(WAP_ERR_OOM, "host allocation failed");
   b->hs.assign(handles, handles + n);
-  b->kg2 = kg_two_pass(h0);
+  b->sched = s0;
+  b->kg2 = kg_two_pass(h0) && !fused;
+  b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;
@@ -1296,7 +1431,7 @@ int wap_batch_set_stream(wap_batch *b, void *stream) {
 int wap_batch_tiles(wap_batch *b, int64_t *tiles, int32_t *tile_y) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   if (tiles) *tiles = b->total_tiles;
-  if (tile_y) *tile_y = b->tile_y;
+  if (tile_y) *tile_y = batch_fused(b) ? b->waves : b->tile_y;
   return WAP_OK;
 }
 
@@ -1308,7 +1443,9 @@ int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0) {
   if (rc) return rc;
   rc = batch_phase_dispatch(b, nsteps, step0);
   if (rc) return rc;
-  return batch_dispatch(b, nsteps);
+  rc = batch_dispatch(b, nsteps);
+  if (!rc) batch_flip(b, nsteps);
+  return rc;
 }
 
 int wap_batch_capture(wap_batch *b, int64_t nsteps) {
@@ -1355,6 +1492,7 @@ int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
     WAP_HIP(hipEventRecord(e0, b->stream));
   }
   WAP_HIP(hipGraphLaunch(b->exec, b->stream));
+  batch_flip(b, b->graph_steps);
   if (ms) {
     WAP_HIP(hipEventRecord(e1, b->stream));
     WAP_HIP(hipEventSynchronize(e1));

@@ -687,8 +687,7 @@ __device__ __forceinline__ cplx<R> kg1_damp_t(R dg, R dv, cplx<R> x) {
 // step needs, never for the prefetch behind it.  The loop is unrolled by four,
 // so the ring of four plane slots rotates by renaming.
 template <typename R, int ROWS>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass(
-    const KG1Args<R> p) {
+__device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
   // double-buffered planes: iteration m reads buffer m & 1 and writes buffer (m + 1) & 1,
@@ -703,7 +702,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // (a uniform global load would be a vector load, and its wait a full drain)
   __shared__ R lgam[kKG1MaxTileZ + 2];
   __shared__ SrcCoef<R> lsrc[kMaxBand];
-  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
@@ -748,7 +746,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
-  const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
@@ -900,6 +897,24 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   }
 }
 
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass(
+    const KG1Args<R> p) {
+  kg1_body<R, ROWS>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
+}
+
+// many grids in one launch: the workgroup finds its instance in the tile prefix sums and
+// reads that instance's arguments and step phasor from device memory
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass_batched(
+    const KG1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
+    const cplx<R> *__restrict__ ph) {
+  const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
+  const int inst = find_instance(tile_begin, ninst, t);
+  const KG1Args<R> a = args[inst];
+  kg1_body<R, ROWS>(a, t - tile_begin[inst], ph[inst]);
+}
+
 // ----------------------------------------- Schrodinger RK4 step in 2 fused passes
 //
 // For the linear system d psi / dt = M psi, M = -i H (H = -A Lap_pml + V, and zero on
@@ -954,8 +969,7 @@ __device__ __forceinline__ cplx<R> horner(cplx<R> b, R f, cplx<R> hy) {
 }
 
 template <typename R, int ROWS, int PB>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass(
-    const Schr2Args<R> p) {
+__device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
   constexpr int pw = kWaveX + 4;
@@ -964,7 +978,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   __shared__ C lub[2][nps];  // level-1 output (A: u4, B: u2): tile and side columns
   __shared__ C lzc[2 * (kKG1MaxTileZ + 2)];
   __shared__ SrcCoef<R> lsrc[kMaxBand];
-  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
@@ -1004,7 +1017,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
-  const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
@@ -1113,6 +1125,22 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
     buf_st<C>(rout, ow, bc, uprev);
   }
+}
+
+template <typename R, int ROWS, int PB>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass(
+    const Schr2Args<R> p) {
+  schr2_body<R, ROWS, PB>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
+}
+
+template <typename R, int ROWS, int PB>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass_batched(
+    const Schr2Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
+    const cplx<R> *__restrict__ ph) {
+  const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
+  const int inst = find_instance(tile_begin, ninst, t);
+  const Schr2Args<R> a = args[inst];
+  schr2_body<R, ROWS, PB>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ------------------------------------------------------------------ adjoint

@@ -14,9 +14,10 @@ precision  Klein-Gordon 200 keV plane wave through a 400-atom slab
 batched    48 instances, sizes from {96^3, 128^3, 160x160x256, 192^3}, beam
            energies U[60, 300] keV (seed 1), convergent Gaussian probes at
            random lateral positions over a shared 200-atom slab; alternating
-           Schrodinger / Klein-Gordon, complex64, 1000 steps in one hipGraph
-           per equation group.  Reports aggregate Gvoxel-steps/s and the
-           launch shapes chosen by the autotuner per size class.
+           Schrodinger / Klein-Gordon, complex64 (the fused schedules), 1000
+           steps in hipGraphs of 100 steps per equation group.  Reports
+           aggregate Gvoxel-steps/s and the z chunk the autotuner picked per
+           size class.
 adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
            L = || |fft2(exit)|^2 - I_target ||^2 with I_target from a copy of the
            150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
@@ -146,11 +147,12 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
         lam = wavelength_from_kinetic_energy(ekin)
         ext = centred_extent(shape, spacing, spacing_z)
         mk = schrodinger_params if eq == "schrodinger" else klein_gordon_params
-        p = mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18, precision="f32",
-               schedule="kg_2pass" if eq == "klein_gordon" else "stages")
+        p = mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18, precision="f32")
         key = (eq, shape)
         if autotune and key not in tuned:
-            ty, tz, ms, _ = tune(p, tile_ys=(4,), tile_zs=(4, 8, 16, 32), steps=3, warmup=1)
+            # the fused schedules (the float32 default): workgroups of 16 waves, tune the
+            # z chunk
+            ty, tz, ms, _ = tune(p, tile_ys=(16,), tile_zs=(8, 16, 32, 64), steps=3, warmup=1)
             tuned[key] = (ty, tz, ms)
         if key in tuned:
             p = p.with_(tile_y=tuned[key][0], tile_z=tuned[key][1])
