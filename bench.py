@@ -11,9 +11,9 @@ in HBM before the timed region; the timed region is K steps bracketed by
 Printed (one JSON line): Gvoxel-steps/s; the roofline of the dominant kernel
 (algorithmic bytes per launch / its mean HIP-event duration, measured on the
 solver's stream); the CPU baseline (the FP64 NumPy oracle on a bounded slab
-of the same workload, timed on this host); for Klein-Gordon, the cost of the
-gradient's reverse steps on the same grid (``adjoint``: a recorded forward
-step and a fused adjoint step against a plain step, per step).
+of the same workload, timed on this host); the cost of the gradient's reverse
+steps on the same grid (``adjoint``: a recorded forward step and an adjoint
+step against a plain step, per step).
 
 Roofline bytes.  ``achieved`` counts the canonical bytes of an RK4 step that
 streams once per stage (Klein-Gordon complex64: 272 B per voxel-step,
@@ -243,14 +243,15 @@ def main():
     step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
     norm = sim.norm2()
     adjoint = None
-    if args.adjoint_steps > 0 and p.equation == "klein_gordon" and p.precision != "f32_kahan":
+    if args.adjoint_steps > 0 and p.precision != "f32_kahan":
         rec_s, rev_s = adjoint_cost(sim, args.adjoint_steps)
         step_s = elapsed / args.steps
         adjoint = {"steps": args.adjoint_steps, "step_ms": round(step_s * 1e3, 4),
                    "record_ms": round(rec_s * 1e3, 4), "adjoint_ms": round(rev_s * 1e3, 4),
                    "backward_over_forward": round((rec_s + rev_s) / step_s, 3),
-                   "adjoint_kernel": ("kg1_adj_pass" if vox * 8 < 2 ** 31 else
-                                      "adj_pass (4 stage passes)"),
+                   "adjoint_kernel": ("schr_adj_pass (2 passes)" if sched == "schr_fused" else
+                                      "kg1_adj_pass" if p.equation == "klein_gordon"
+                                      and vox * 8 < 2 ** 31 else "adj_pass (4 stage passes)"),
                    "adjoint_Gvoxels": round(vox / rev_s / 1e9, 2)}
         print(f"[bench] adjoint: record {rec_s * 1e3:.3f} ms, reverse step {rev_s * 1e3:.3f} ms "
               f"per step", file=sys.stderr, flush=True)

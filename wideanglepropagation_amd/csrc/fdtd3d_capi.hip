@@ -1532,6 +1532,7 @@ struct wap_adjoint {
   void *kbb[2] = {nullptr, nullptr};
   double *vbar = nullptr;
   void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type
+  bool schr = false;     // fused Schrodinger (one state component; u3, v3, S in kba, kbb, acc)
   bool owns = true;      // false: the buffers are carved from a caller's workspace
   void *orig[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [comp]: the handle's
                                                                   // state, scratch at create
@@ -1547,9 +1548,14 @@ std::vector<std::pair<void **, size_t>> adjoint_buffers(wap_adjoint *a) {
   std::vector<std::pair<void **, size_t>> v;
   if (a->fused) {
     for (void *&b : a->st) v.push_back({&b, fb});
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < a->ncomp; ++c) {
       v.push_back({&a->lam2[0][c], fb});
       v.push_back({&a->lam2[1][c], fb});
+    }
+    if (a->schr) {  // the pass-I outputs u3, v3 and S
+      v.push_back({&a->kba[0], fb});
+      v.push_back({&a->kbb[0], fb});
+      v.push_back({&a->acc[0], fb});
     }
     v.push_back({&a->vacc, h->n * h->rs});
   } else {
@@ -1575,13 +1581,16 @@ size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
 int adjoint_restore(wap_adjoint *a) {
   if (!a->fused) return WAP_OK;
   wap_handle *h = a->h;
-  void **cur[2] = {&h->psi, &h->pi}, **ya[2] = {&h->ya_psi, &h->ya_pi};
-  for (int c = 0; c < 2; ++c) {
+  // the state buffer and the one the fused step's output swaps with the slots
+  void **cur[2] = {&h->psi, &h->pi};
+  void **ya[2] = {a->schr ? &h->yb_psi : &h->ya_psi, &h->ya_pi};
+  const int nc = a->ncomp;
+  for (int c = 0; c < nc; ++c) {
     void *o1 = a->orig[c][0], *o2 = a->orig[c][1];
     auto where = [&](void *p) -> void ** {
       if (*ya[c] == p) return ya[c];
       for (int s = 0; s < a->slots; ++s)
-        if (a->st[(size_t)s * 2 + c] == p) return &a->st[(size_t)s * 2 + c];
+        if (a->st[(size_t)s * nc + c] == p) return &a->st[(size_t)s * nc + c];
       return nullptr;
     };
     if (*cur[c] != o1 && *cur[c] != o2) {
@@ -1705,9 +1714,14 @@ int kg1_adj_waves(const wap_handle *h) {
 // the handle's spare buffer afterwards and changes owner with the slot's (no copy).
 int adjoint_record_fused(wap_adjoint *ad, int64_t step, double t0, int slot) {
   wap_handle *h = ad->h;
-  void **sp = &ad->st[(size_t)slot * 2];
+  void **sp = &ad->st[(size_t)slot * ad->ncomp];
   int rc;
-  if (sched_of(h) == WAP_SCHED_KG_FUSED) {
+  if (ad->schr) {
+    // the 2-pass step leaves psi_n in the buffer its output swapped with
+    rc = run_steps_schr2(h, 1, step, t0, nullptr, nullptr);
+    if (rc) return rc;
+    std::swap(h->yb_psi, sp[0]);
+  } else if (sched_of(h) == WAP_SCHED_KG_FUSED) {
     rc = is_f64(h) ? run_steps_kg1<double>(h, 1, step, t0, nullptr, nullptr)
                    : run_steps_kg1<float>(h, 1, step, t0, nullptr, nullptr);
     if (rc) return rc;
@@ -1765,6 +1779,52 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   return WAP_OK;
 }
 
+// Reverse of one recorded Schrodinger 2-pass step: schr_adj_pass I then II.
+int adjoint_step_schr(wap_adjoint *ad, int slot) {
+  using R = float;
+  using C = wap::cplx<R>;
+  wap_handle *h = ad->h;
+  const int waves = kg1_waves(h);
+  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  wap::SchrAdjArgs<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  a.rec = (const C *)ad->st[slot];
+  a.lam = (const C *)ad->lam2[ad->cur][0];
+  a.u3 = (C *)ad->kba[0];
+  a.v3 = (C *)ad->kbb[0];
+  a.sacc = (C *)ad->acc[0];
+  a.lam_out = (C *)ad->lam2[ad->cur ^ 1][0];
+  a.vbar = (R *)ad->vacc;
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = kg1_ntx(h); a.nty = kg1_nty(h, waves); a.ntz = (h->d.nz + tz - 1) / tz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  a.h = (R)h->d.dt;
+  fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
+  const dim3 block(wap::kWaveX, waves);
+  const dim3 grid((unsigned)fused_tiles(h, waves));
+  switch (waves) {
+    case 4:
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 3, 1>), grid, block, 0, h->stream, a);
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 3, 2>), grid, block, 0, h->stream, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 7, 1>), grid, block, 0, h->stream, a);
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 7, 2>), grid, block, 0, h->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 15, 1>), grid, block, 0, h->stream, a);
+      hipLaunchKernelGGL((wap::schr_adj_pass<R, 15, 2>), grid, block, 0, h->stream, a);
+      break;
+  }
+  WAP_HIP(hipGetLastError());
+  ad->cur ^= 1;
+  return WAP_OK;
+}
+
 // validation and the host object of an adjoint (no device memory yet)
 int adjoint_new(wap_handle *h, int slots, wap_adjoint **out) {
   if (!h || !out) return fail(WAP_ERR_ARG, "NULL argument");
@@ -1778,13 +1838,14 @@ int adjoint_new(wap_handle *h, int slots, wap_adjoint **out) {
   a->h = h;
   a->slots = slots;
   a->ncomp = is_kg(h) ? 2 : 1;
-  a->fused = is_kg(h) && fused_plane_fits(h->d);
+  a->schr = !is_kg(h) && sched_of(h) == WAP_SCHED_SCHR_FUSED;
+  a->fused = (is_kg(h) && fused_plane_fits(h->d)) || a->schr;
   if (a->fused) {
-    a->st.assign((size_t)slots * 2, nullptr);
+    a->st.assign((size_t)slots * a->ncomp, nullptr);
     a->st_step.assign(slots, 0);
     a->st_t0.assign(slots, 0.0);
     a->orig[0][0] = h->psi;
-    a->orig[0][1] = h->ya_psi;
+    a->orig[0][1] = a->schr ? h->yb_psi : h->ya_psi;
     a->orig[1][0] = h->pi;
     a->orig[1][1] = h->ya_pi;
   } else {
@@ -1887,6 +1948,7 @@ int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
 int wap_adjoint_step(wap_adjoint *a, int slot) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
+  if (a->schr) return adjoint_step_schr(a, slot);
   if (a->fused) return is_f64(a->h) ? adjoint_step_fused<double>(a, slot) : adjoint_step_fused<float>(a, slot);
   const bool kg = is_kg(a->h);
   if (is_f64(a->h)) return kg ? adjoint_step_t<double, 1>(a, slot) : adjoint_step_t<double, 0>(a, slot);

@@ -1321,6 +1321,319 @@ __global__ __launch_bounds__(512) void adj_pass(const AdjArgs<R> p) {
   }
 }
 
+// --------------------------------- Schrodinger adjoint step in 2 fused passes
+//
+// Reverse of one schr2_pass step (u4 = b + h/4 M b, u3 = b + h/3 M u4, u2 = b + h/2 M u3,
+// psi' = b + h M u2, b = K psi_n + source).  With lam = dL/dpsi' (gradients of real L
+// packed as d/dRe + i d/dIm) and x^ = x restricted to the interior planes (M has zero
+// rows on the frozen planes):
+//   v2 = h M^H lam^,  v3 = h/2 M^H v2^,  v4 = h/3 M^H v3^,
+//   lam_n = K (lam + v2 + v3 + v4 + h/4 M^H v4^),
+//   dL/dV += h Im(conj(lam) u2) + h/2 Im(conj(v2) u3) + h/3 Im(conj(v3) u4)
+//            + h/4 Im(conj(v4) b)                     (interior voxels),
+// M^H x = i H^H x, whose z part is conj(cp_{j-1}) x_{j-1} - conj(cp_j + cm_j) x_j +
+// conj(cm_{j+1}) x_{j+1} (cp, cm: the weights of y_{k+1}, y_{k-1} on plane k).  The
+// forward intermediates are recomputed from the recorded psi_n.  Two passes with the
+// schr2_pass layout (level 1 at plane m, level 2 at m - 1):
+//   pass I:  u4 = b + h/4 M b and v2 = h M^H lam^ at m; u3 = b + h/3 M u4 and
+//            v3 = h/2 M^H v2^ at m - 1; writes u3, v3, S = lam + v2 + v3 and adds
+//            h/2 Im(conj(v2) u3) + h/3 Im(conj(v3) u4) to dL/dV;
+//   pass II: u2 = b + h/2 M u3 and v4 = h/3 M^H v3^ at m; lam_n = K (S + v4 +
+//            h/4 M^H v4^) at m - 1, adding h Im(conj(lam) u2) + h/4 Im(conj(v4) b).
+// The last chunk runs one plane past the grid (m = nz, all inputs masked) so that the
+// top frozen plane gets its level-2 values.  52 + 60 B per voxel-step.
+template <typename R>
+struct SchrAdjArgs {
+  using C = cplx<R>;
+  const C *rec;  // the step's input state psi_n (b = inject(psi_n))
+  const C *lam;  // dL/dpsi_{n+1}
+  C *u3, *v3, *sacc;  // pass I writes, pass II reads
+  C *lam_out;         // dL/dpsi_n (pass II)
+  R *vbar;            // dL/dV accumulator (read-modify-write in both passes)
+  const R *V;
+  const C *zc;
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  R hx, hy, h;
+  int src_k0, src_n;
+  const SrcCoef<R> *src_tab;
+  C src_ph;
+};
+
+// H x with the lateral neighbour sums sx, sy and the z neighbours xp, xm
+template <typename R>
+__device__ __forceinline__ cplx<R> schr_hf(R hx, R hy, R v, cplx<R> wp, cplx<R> wm, cplx<R> y,
+                                           cplx<R> sx, cplx<R> sy, cplx<R> zp, cplx<R> zm) {
+  const cplx<R> dp = csub<R>(zp, y), dm = csub<R>(y, zm);
+  const R lx = sx.x - R(2) * y.x, ly = sy.x - R(2) * y.x;
+  const R mx = sx.y - R(2) * y.y, my = sy.y - R(2) * y.y;
+  return mkc<R>(hx * lx + hy * ly + (wp.x * dp.x - wp.y * dp.y) - (wm.x * dm.x - wm.y * dm.y) + v * y.x,
+                hx * mx + hy * my + (wp.x * dp.y + wp.y * dp.x) - (wm.x * dm.y + wm.y * dm.x) + v * y.y);
+}
+
+// c M^H x = c i H^H x, with cpm = cp_{k-1}, cdk = cp_k + cm_k, cmp = cm_{k+1}
+template <typename R>
+__device__ __forceinline__ cplx<R> schr_mh(R hx, R hy, R v, R c, cplx<R> cpm, cplx<R> cdk,
+                                           cplx<R> cmp, cplx<R> x, cplx<R> sx, cplx<R> sy,
+                                           cplx<R> xp, cplx<R> xm) {
+  // conj(a) z = (a.x z.x + a.y z.y, a.x z.y - a.y z.x)
+  const R hr = hx * (sx.x - R(2) * x.x) + hy * (sy.x - R(2) * x.x) + v * x.x +
+               (cpm.x * xm.x + cpm.y * xm.y) - (cdk.x * x.x + cdk.y * x.y) + (cmp.x * xp.x + cmp.y * xp.y);
+  const R hi = hx * (sx.y - R(2) * x.y) + hy * (sy.y - R(2) * x.y) + v * x.y +
+               (cpm.x * xm.y - cpm.y * xm.x) - (cdk.x * x.y - cdk.y * x.x) + (cmp.x * xp.y - cmp.y * xp.x);
+  return mkc<R>(-c * hi, c * hr);
+}
+
+// Im(conj(a) b)
+template <typename R>
+__device__ __forceinline__ R imcj(cplx<R> a, cplx<R> b) {
+  return a.x * b.y - a.y * b.x;
+}
+
+template <typename R, int ROWS, int PASS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr_adj_pass(
+    const SchrAdjArgs<R> p) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
+  constexpr int pw = kWaveX + 4;
+  constexpr int nps = pw * (ROWS + 2);
+  // level-1 inputs with the halo ring (I: b and lam^; II: u3 and v3^) and level-1
+  // outputs for level 2 (I: u4 and v2^; II: v4^), double-buffered
+  __shared__ C lfa[2][nps], laa[2][nps], lfb[2][nps], lab[2][nps];
+  __shared__ C lzc[2 * (kKG1MaxTileZ + 6)];  // planes m_start - 2 .. m_end + 1
+  __shared__ SrcCoef<R> lsrc[kMaxBand];
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const bool side = wv == ROWS;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const int sl = lx % (2 * ROWS);
+  const int r = side ? sl % ROWS : wv;
+  const bool right = sl >= ROWS;
+  const int c = side ? (right ? kWaveX + 2 : 1) : lx + 2;
+  const int gi = side ? (right ? tx * kWaveX + kWaveX : tx * kWaveX - 1) : tx * kWaveX + lx;
+  const int gj = tyi * (ROWS - 2) - 1 + r;
+  const int i = ((gi % nx) + nx) % nx;
+  const int j = ((gj % ny) + ny) % ny;
+  const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
+  const bool writer = wave_writes && gi < nx && gj < ny;
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc(p.rec), rlam = buf_rsrc(p.lam);
+  const __amdgpu_buffer_rsrc_t ru3 = buf_rsrc(p.u3), rv3 = buf_rsrc(p.v3), rsa = buf_rsrc(p.sacc);
+  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc(p.lam_out), rvb = buf_rsrc(p.vbar), rv = buf_rsrc(p.V);
+  const int ij = j * nx + i;
+  const int ss = (r + 1) * pw + c;
+  int oo = -1, hs = ss;
+  if (side) {
+    oo = j * nx + (right ? ((i + 1 == nx) ? 0 : i + 1) : ((i == 0) ? nx - 1 : i - 1));
+    hs = right ? ss + 1 : ss - 1;
+  } else if (wv == 0) {
+    oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+    hs = ss - pw;
+  } else if (wv == ROWS - 1) {
+    oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+    hs = ss + pw;
+  }
+  const bool has_o = oo >= 0;
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t ow = writer ? oc : kBufSkip;
+  const uint32_t owr = writer ? ov : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C ph = p.src_ph;
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+  const R h = p.h, h2 = p.h * R(0.5), h3 = p.h / R(3), h4 = p.h * R(0.25);
+
+  const int m_start = max(k0 - 1, 0);
+  const int m_end = (k1 == nz) ? nz : k1;  // inclusive; m = nz is the empty plane above
+  const int zlo = m_start - 2;
+  {
+    const int tid = wv * kWaveX + lx;
+    for (int t = tid; t <= m_end + 1 - zlo; t += kWaveX * (ROWS + 1)) {
+      const int k = zlo + t;
+      const bool in = k >= 0 && k < nz;
+      lzc[2 * t] = in ? p.zc[2 * k] : zero;
+      lzc[2 * t + 1] = in ? p.zc[2 * k + 1] : zero;
+    }
+    if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
+  }
+  auto interior = [&](int m) { return m > 0 && m < nz - 1; };
+  auto zw = [&](int k, int which) { return lzc[2 * (k - zlo) + which]; };  // 0: cp, 1: cm
+  auto keep_of = [&](int k) {
+    const int bnd = k - p.src_k0;
+    return (bnd >= 0 && bnd < p.src_n && k < nz) ? lsrc[bnd].keep : R(1);
+  };
+  auto inject = [&](int m, C &x, int at) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = lsrc[bnd].keep;
+    sp.c = cmul<R>(lsrc[bnd].c, ph);
+    sp.cq = sp.c;
+    x = inj<R>(x, sp, sp.c, prof, at);
+  };
+  // forward H (level weights of plane k) and adjoint c M^H (weights of k-1, k, k+1)
+  auto hfwd = [&](int k, R v, C y, C sx, C sy, C zp, C zm) {
+    return schr_hf<R>(p.hx, p.hy, v, zw(k, 0), zw(k, 1), y, sx, sy, zp, zm);
+  };
+  auto madj = [&](int k, R v, R cc, C x, C sx, C sy, C xp, C xm) {
+    const C cpk = zw(k, 0), cmk = zw(k, 1);
+    return schr_mh<R>(p.hx, p.hy, v, cc, zw(k - 1, 0), cadd<R>(cpk, cmk), zw(k + 1, 1), x, sx, sy,
+                      xp, xm);
+  };
+  auto lat = [&](const C *L, C &sx, C &sy) {
+    sx = cadd<R>(L[ss + 1], L[ss - 1]);
+    sy = cadd<R>(L[ss + pw], L[ss - pw]);
+  };
+  auto mask = [&](int m, C x) { return interior(m) ? x : zero; };
+
+  struct Raw {  // one plane of this lane: the two level-1 inputs, own and outer, V and
+    C f, of, a, oa, b;  // (pass II) the base b
+    R v;
+  };
+  struct Pt {  // pointwise inputs of the level-2 plane, loaded one step ahead
+    C s, l;    // pass II: S and lam
+    R vb;      // the dL/dV accumulator
+  };
+  auto load_plane = [&](int m, Raw &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    const __amdgpu_buffer_rsrc_t rf = PASS == 1 ? rrec : ru3, ra = PASS == 1 ? rlam : rv3;
+    w.f = buf_ld<C>(rf, oc, bc);
+    w.a = buf_ld<C>(ra, oc, bc);
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    w.of = buf_ld<C>(rf, ooc, bc);
+    w.oa = buf_ld<C>(ra, ooc, bc);
+    if (PASS == 2) w.b = buf_ld<C>(rrec, oc, bc);
+  };
+  // the accumulator (and pass II's S, lam) of plane m for the writer lanes
+  auto load_pt = [&](int m, Pt &w) {
+    const uint32_t bcr = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = bcr * (uint32_t)sizeof(C);
+    const bool in = m >= k0 && m < k1;
+    w.vb = buf_ld<R>(rvb, in ? owr : kBufSkip, bcr * (uint32_t)sizeof(R));
+    if (PASS == 2) {
+      w.s = buf_ld<C>(rsa, in ? ow : kBufSkip, bc);
+      w.l = buf_ld<C>(rlam, in ? ow : kBufSkip, bc);
+    }
+  };
+  // b = inject(psi_n): pass I's forward input (own and outer neighbour), pass II's base
+  auto prepare = [&](int m, Raw &w) {
+    if (PASS == 1) {
+      inject(m, w.f, ij);
+      if (has_o) inject(m, w.of, oo);
+    } else {
+      inject(m, w.b, ij);
+    }
+  };
+  // level-1 inputs of plane m into LDS buffer bb (the adjoint one masked to the interior)
+  auto store_in = [&](int bb, int m, const Raw &w) {
+    lfa[bb][ss] = w.f;
+    laa[bb][ss] = mask(m, w.a);
+    if (has_o) {
+      lfa[bb][hs] = w.of;
+      laa[bb][hs] = mask(m, w.oa);
+    }
+  };
+
+  Raw r0, r1, r2, r3;
+  Pt q0, q1;
+  load_plane(m_start - 1, r0);
+  load_plane(m_start, r1);
+  load_plane(m_start + 1, r2);
+  load_pt(m_start - 1, q0);
+  __syncthreads();  // lsrc, lzc
+  prepare(m_start - 1, r0);
+  prepare(m_start, r1);
+  store_in(m_start & 1, m_start, r1);
+  __syncthreads();
+
+  // level-1 outputs of this column at m-1 and m-2: forward (I: u4; II: u2), adjoint
+  // (I: v2; II: v4), unmasked
+  C fprev = zero, fmm = zero, aprev = zero, amm = zero;
+
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre, Pt &qk, Pt &qn) {
+    const int rd = m & 1, wr = rd ^ 1;
+    load_plane(m + 2, pre);
+    load_pt(m, qn);
+    prepare(m + 1, rc);
+    // ---- level 1 at m
+    C fcur, acur;
+    {
+      C sx, sy;
+      lat(lfa[rd], sx, sy);
+      const C hy = hfwd(m, rb.v, rb.f, sx, sy, rc.f, ra.f);
+      // I: u4 = b + h/4 M b; II: u2 = b + h/2 M u3 (b from the pointwise inputs of m)
+      const R f1 = interior(m) ? (PASS == 1 ? h4 : h2) : R(0);
+      const C bm = PASS == 1 ? rb.f : rb.b;
+      fcur = horner<R>(bm, f1, hy);
+      lat(laa[rd], sx, sy);
+      // I: v2 = h M^H lam^; II: v4 = h/3 M^H v3^
+      acur = madj(m, rb.v, PASS == 1 ? h : h3, mask(m, rb.a), sx, sy, mask(m + 1, rc.a),
+                  mask(m - 1, ra.a));
+    }
+    // ---- level 2 at k = m-1 (stored when inside the chunk)
+    const int k = m - 1;
+    if (wave_writes) {
+      const bool ik = interior(k);
+      C sx, sy;
+      const uint32_t bcr = (uint32_t)max(k, 0) * plane;
+      const uint32_t bc = bcr * (uint32_t)sizeof(C);
+      const uint32_t o = (k >= k0) ? ow : kBufSkip;
+      const uint32_t orr = (k >= k0) ? owr : kBufSkip;
+      const R vb = qk.vb;
+      if (PASS == 1) {
+        lat(lfb[rd], sx, sy);
+        const C hy = hfwd(k, ra.v, fprev, sx, sy, fcur, fmm);
+        const C u3 = horner<R>(ra.f, ik ? h3 : R(0), hy);   // b + h/3 M u4
+        lat(lab[rd], sx, sy);
+        const C v3 = madj(k, ra.v, h2, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
+        const C sk = cadd<R>(cadd<R>(ra.a, aprev), v3);    // lam + v2 + v3
+        const R g = ik ? h2 * imcj<R>(aprev, u3) + h3 * imcj<R>(v3, fprev) : R(0);
+        buf_st<C>(ru3, o, bc, u3);
+        buf_st<C>(rv3, o, bc, v3);
+        buf_st<C>(rsa, o, bc, sk);
+        buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), vb + g);
+      } else {
+        lat(lab[rd], sx, sy);
+        const C w = madj(k, ra.v, h4, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
+        const C ln = cscale<R>(cadd<R>(cadd<R>(qk.s, aprev), w), keep_of(k));
+        const R g = ik ? h * imcj<R>(qk.l, fprev) + h4 * imcj<R>(aprev, ra.b) : R(0);
+        buf_st<C>(rlo, o, bc, ln);
+        buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), vb + g);
+      }
+    }
+    if (PASS == 1) lfb[wr][ss] = fcur;  // pass II's forward output is only used pointwise
+    lab[wr][ss] = mask(m, acur);
+    store_in(wr, m + 1, rc);
+    __syncthreads();
+    fmm = fprev;
+    fprev = fcur;
+    amm = aprev;
+    aprev = acur;
+  };
+
+  int m = m_start;
+  for (; m + 3 <= m_end; m += 4) {
+    plane_step(m, r0, r1, r2, r3, q0, q1);
+    plane_step(m + 1, r1, r2, r3, r0, q1, q0);
+    plane_step(m + 2, r2, r3, r0, r1, q0, q1);
+    plane_step(m + 3, r3, r0, r1, r2, q1, q0);
+  }
+  if (m <= m_end) plane_step(m, r0, r1, r2, r3, q0, q1);
+  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0, q1, q0);
+  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, q0, q1);
+}
+
 // ------------------------------------- Klein-Gordon adjoint step in 1 fused pass
 //
 // Reverse of one KG RK4 step n: lam = dL/du_{n+1} in, dL/du_n out, dL/dV += the
