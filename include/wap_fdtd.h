@@ -211,15 +211,16 @@ int wap_batch_synchronize(wap_batch *b);
  * advances the handle), and walk the segment backwards with wap_adjoint_step
  * (slot m-1 .. 0), adding exit-plane seeds with wap_adjoint_seed.  f64 and f32
  * handles with frozen end planes.  Klein-Gordon grids within the fused pass's field
- * limit record each step's input state (the record step is the fused forward step,
- * whose output buffer is swapped with the slot) and reverse a step in one fused
- * pass; Schrodinger records the 4 stage inputs and reverses stage by stage.  Destroy
- * the adjoint before its handle.
+ * limit and Schrodinger handles on the fused schedule record each step's input state
+ * (the record step is the fused forward step, whose output buffer is swapped with the
+ * slot) and reverse a step in one (KG) or two (Schrodinger) fused passes; other
+ * handles record the 4 stage inputs and reverse stage by stage.  Destroy the adjoint
+ * before its handle.
  */
 typedef struct wap_adjoint wap_adjoint;
 
-/* Adjoint workspace with `slots` per-step records (KG: the input state, 2 fields;
- * Schrodinger: 4 stage inputs). */
+/* Adjoint workspace with `slots` per-step records (fused schedules: the input state;
+ * otherwise the 4 stage inputs). */
 int wap_adjoint_create(wap_handle *h, int slots, wap_adjoint **out);
 /* Device bytes wap_adjoint_create_in needs for `slots` records on this handle. */
 int wap_adjoint_workspace_bytes(wap_handle *h, int slots, int64_t *bytes);
