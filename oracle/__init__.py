@@ -16,6 +16,8 @@ fdtd3d    FP64 NumPy restatement of ``fdtd_3d.cu`` (race-free classical RK4,
 extract   the notebooks' exit-wave / far-field recipe
           (``FDTDmethod/fdtd_3D_baseline_comparison.ipynb:267-290``,
           ``propagation.py:346-350``).
+spectral  the Fresnel / angular-spectrum multislice the notebooks compare FDTD
+          exit waves against (``propagation.py:14-177, 305-352``), JAX-free.
 
 Pinning (what checks each function, in ``tests/``):
   physics.*            reference-held golden scalars (notebook outputs and the
@@ -30,4 +32,9 @@ Pinning (what checks each function, in ``tests/``):
                        (omega = V +- sqrt(c^2 k_d^2 + m^2)); charge
                        conservation without sponge.
   extract.*            synthetic snapshots with a known answer.
+  spectral.*           textbook electron wavelengths, the refractive index in
+                       60-digit decimal arithmetic, single Fourier modes
+                       propagated analytically, the paraxial expansion, kernel
+                       composition, vacuum / constant-potential multislices,
+                       norm conservation (``tests/test_spectral_oracle.py``).
 """
