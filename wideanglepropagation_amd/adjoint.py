@@ -95,16 +95,24 @@ class ExitWave(torch.autograd.Function):
         tmp = torch.zeros_like(E)
         ckpt = {}
         keep = bool(ctx.needs_input_grad[0])   # no checkpoints when no gradient is wanted
-        for n in range(nsteps):
-            if keep and n % C == 0:
+        # step from event to event (a checkpoint before step n, a sample after step s)
+        events = sorted({s for s in samples if 0 < s <= nsteps} |
+                        ({n for n in range(0, nsteps, C)} if keep else set()) | {nsteps})
+        n = 0
+        for e in events:
+            if keep and n % C == 0 and n not in ckpt:
                 ckpt[n] = sim.get_state(np.complex128 if p.precision == "f64" else np.complex64,
                                         device=True)
-            sim.step(1)
-            s = n + 1
-            if s in samples:
+            if e > n:
+                sim.step(e - n)
+                n = e
+            if n in samples:
                 tmp.zero_()
-                sim.accumulate_plane(k_exit, p.e_sim * (sim.t0 + s * p.dt), tmp)
-                E += samples[s] * tmp
+                sim.accumulate_plane(k_exit, p.e_sim * (sim.t0 + n * p.dt), tmp)
+                E += samples[n] * tmp
+            if keep and n % C == 0 and n < nsteps and n not in ckpt:
+                ckpt[n] = sim.get_state(np.complex128 if p.precision == "f64" else np.complex64,
+                                        device=True)
         if not keep:
             sim.close()
             ctx.sim = None
