@@ -227,20 +227,9 @@ def adjoint_study(quick=False, n_fd=32):
         E = exit_wave(Vx, p, nsteps, k_exit, samples=samples)
         return torch.sum((far_field_intensity(E) - It) ** 2)
 
-    # forward only (timed), then forward + backward (timed)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     with torch.no_grad():
         L0 = loss_of(V32, p32)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    Vg = V32.clone().requires_grad_(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    L = loss_of(Vg, p32)
-    L.backward()
-    torch.cuda.synchronize()
-    t_fb = time.perf_counter() - t0
+    t_fwd, t_fb, Vg = _time_forward_backward(lambda Vx: loss_of(Vx, p32), V32)
     g32 = Vg.grad.double().cpu().numpy()
     print(f"[adjoint] forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s", file=sys.stderr,
           flush=True)
@@ -288,6 +277,28 @@ def adjoint_study(quick=False, n_fd=32):
                         for r in rows[:6]]}
 
 
+def _time_forward_backward(loss_of, V, reps=3):
+    """Best of reps wall times of loss_of(V) without gradient and of loss_of(V) plus its
+    backward (the first backward also allocates the records); returns them and the last
+    leaf tensor, whose .grad holds the gradient."""
+    t_fwd = t_fb = math.inf
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            loss_of(V)
+        torch.cuda.synchronize()
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    for _ in range(reps):
+        Vg = V.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss_of(Vg).backward()
+        torch.cuda.synchronize()
+        t_fb = min(t_fb, time.perf_counter() - t0)
+    return t_fwd, t_fb, Vg
+
+
 def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
     """Forward and forward + backward time of the adjoint study's loss on another grid
     (random target: timing only)."""
@@ -306,18 +317,7 @@ def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
 
     with torch.no_grad():  # warm-up (allocations, library load)
         loss_of(V32)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    with torch.no_grad():
-        loss_of(V32)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    Vg = V32.clone().requires_grad_(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    loss_of(Vg).backward()
-    torch.cuda.synchronize()
-    t_fb = time.perf_counter() - t0
+    t_fwd, t_fb, _ = _time_forward_backward(loss_of, V32)
     print(f"[adjoint] {shape}: forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s",
           file=sys.stderr, flush=True)
     return {"grid_nz_ny_nx": list(shape), "steps": nsteps,
