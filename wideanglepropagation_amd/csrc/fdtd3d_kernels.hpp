@@ -1884,6 +1884,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     lols[own] = r1.ls;
     lolq[own] = r1.lq;
     lov[own] = r1.v;
+    load_halo(m_start + 1, hn);  // the outer neighbours of the first loop plane's m + 1
   }
   __syncthreads();
 
@@ -1907,11 +1908,11 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
                         const R vb, R &vb_next) {
     constexpr bool DG = decltype(dgt)::value;
     const int rd = m & 1, wr = rd ^ 1;
-    // loads: vbar for the next iteration's pass-2 plane, the outer neighbours of plane
-    // m+1, then the prefetch of plane m+2 (older loads are the ones waited for)
+    // loads: vbar for the next iteration's pass-2 plane, then the prefetch of plane m+2
+    // (older loads are the ones waited for); hn, the outer neighbours of plane m+1, was
+    // loaded at the end of the previous iteration
     vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
                         (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
-    load_halo(m + 1, hn);
     load_plane(m + 2, pre);
     inject(m + 1, rc.s, rc.q, ij);
     C ca4, ca3;
@@ -2007,6 +2008,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     lolq[own] = rc.lq;
     lov[own] = rc.v;
     store_plane(dgt, wr, m + 1, rc.s, rc.q, ca4, ca3, hn);
+    // the outer neighbours of the next iteration's plane m+2, issued a whole iteration
+    // before their use (issued at the top of the iteration, the halo lanes waited for
+    // them at its end: 8 % of the kernel's time at 512 x 512 x 256)
+    load_halo(m + 2, hn);
     __syncthreads();
   };
 
