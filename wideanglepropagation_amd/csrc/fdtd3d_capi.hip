@@ -1537,6 +1537,9 @@ struct wap_adjoint {
   void *orig[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [comp]: the handle's
                                                                   // state, scratch at create
   void *lam(int c) const { return lam2[cur][c]; }
+  // fused Klein-Gordon: lam2[b][0] holds LamPair (lam_psi, lam_pi) per voxel, lam2[b][1] is
+  // unused
+  bool pairs() const { return fused && !schr; }
 };
 
 namespace {
@@ -1548,9 +1551,12 @@ std::vector<std::pair<void **, size_t>> adjoint_buffers(wap_adjoint *a) {
   std::vector<std::pair<void **, size_t>> v;
   if (a->fused) {
     for (void *&b : a->st) v.push_back({&b, fb});
-    for (int c = 0; c < a->ncomp; ++c) {
-      v.push_back({&a->lam2[0][c], fb});
-      v.push_back({&a->lam2[1][c], fb});
+    if (a->pairs()) {
+      v.push_back({&a->lam2[0][0], 2 * fb});
+      v.push_back({&a->lam2[1][0], 2 * fb});
+    } else {
+      v.push_back({&a->lam2[0][0], fb});
+      v.push_back({&a->lam2[1][0], fb});
     }
     if (a->schr) {  // the pass-I outputs u3, v3 and S
       v.push_back({&a->kba[0], fb});
@@ -1751,10 +1757,8 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   std::memset((void *)&a, 0, sizeof(a));
   a.u_psi = (const C *)ad->st[(size_t)slot * 2];
   a.u_pi = (const C *)ad->st[(size_t)slot * 2 + 1];
-  a.lam_in_psi = (const C *)ad->lam2[ad->cur][0];
-  a.lam_in_pi = (const C *)ad->lam2[ad->cur][1];
-  a.lam_out_psi = (C *)ad->lam2[ad->cur ^ 1][0];
-  a.lam_out_pi = (C *)ad->lam2[ad->cur ^ 1][1];
+  a.lam_in = (const wap::LamPair<R> *)ad->lam2[ad->cur][0];
+  a.lam_out = (wap::LamPair<R> *)ad->lam2[ad->cur ^ 1][0];
   a.vbar = (R *)ad->vacc;
   a.V = (const R *)h->V;
   a.gam = (const R *)h->gam;
@@ -1928,7 +1932,10 @@ int wap_adjoint_destroy(wap_adjoint *a) {
 int wap_adjoint_reset(wap_adjoint *a) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   const size_t fb = a->h->n * a->h->cs;
-  for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam(c), 0, fb, a->h->stream));
+  if (a->pairs())
+    WAP_HIP(hipMemsetAsync(a->lam(0), 0, 2 * fb, a->h->stream));
+  else
+    for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam(c), 0, fb, a->h->stream));
   WAP_HIP(hipMemsetAsync(a->vbar, 0, a->h->n * sizeof(double), a->h->stream));
   if (a->vacc) WAP_HIP(hipMemsetAsync(a->vacc, 0, a->h->n * a->h->rs, a->h->stream));
   return WAP_OK;
@@ -1962,12 +1969,13 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
   const int n = (int)h->plane;
   const dim3 grid((n + 255) / 256), block(256);
   const size_t off = (size_t)k * h->plane;
+  const int stride = a->pairs() ? 2 : 1;
   if (is_f64(h))
     hipLaunchKernelGGL(wap::seed_plane<double>, grid, block, 0, h->stream, (double2 *)a->lam(0),
-                       off, n, (const double2 *)g, c_re, c_im);
+                       off, n, stride, (const double2 *)g, c_re, c_im);
   else
     hipLaunchKernelGGL(wap::seed_plane<float>, grid, block, 0, h->stream, (float2 *)a->lam(0), off,
-                       n, (const double2 *)g, c_re, c_im);
+                       n, stride, (const double2 *)g, c_re, c_im);
   WAP_HIP(hipGetLastError());
   return WAP_OK;
 }

@@ -1661,12 +1661,47 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 // x^ is 0 on frozen planes (and on the planes loaded in place of out-of-grid ones).
 // The layout, loads and barriers follow kg1_pass; the last chunk runs one extra,
 // empty plane m = nz so that the top frozen plane gets its sweep-2 values.
+// The Klein-Gordon adjoint state of one voxel, (lam_psi, lam_pi) side by side: one
+// 16-byte load or store per voxel in float32 instead of two of 8 bytes (the adjoint pass
+// is bound by its vector-memory instruction count, not by bytes)
+template <typename R>
+struct alignas(2 * sizeof(cplx<R>)) LamPair {
+  cplx<R> s, q;
+};
+
+// lam pair at lane offset off of plane k: the descriptor starts at the plane, so the
+// 32-bit lane offsets stay below 2 GiB whatever the size of the whole field
+template <typename R>
+__device__ __forceinline__ LamPair<R> ld_pair(const LamPair<R> *base, uint32_t k, uint32_t plane,
+                                              uint32_t off) {
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (size_t)k * plane);
+  if constexpr (sizeof(R) == 4) {
+    return buf_ld<LamPair<R>>(r, off, 0);
+  } else {
+    LamPair<R> v;
+    v.s = buf_ld<cplx<R>>(r, off, 0);
+    v.q = buf_ld<cplx<R>>(r, off + (uint32_t)sizeof(cplx<R>), 0);
+    return v;
+  }
+}
+template <typename R>
+__device__ __forceinline__ void st_pair(LamPair<R> *base, uint32_t k, uint32_t plane, uint32_t off,
+                                        LamPair<R> v) {
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (size_t)k * plane);
+  if constexpr (sizeof(R) == 4) {
+    buf_st<LamPair<R>>(r, off, 0, v);
+  } else {
+    buf_st<cplx<R>>(r, off, 0, v.s);
+    buf_st<cplx<R>>(r, off + (uint32_t)sizeof(cplx<R>), 0, v.q);
+  }
+}
+
 template <typename R>
 struct KG1AdjArgs {
   using C = cplx<R>;
   const C *u_psi, *u_pi;          // the step's input state u_n
-  const C *lam_in_psi, *lam_in_pi;  // dL/du_{n+1}
-  C *lam_out_psi, *lam_out_pi;      // dL/du_n (another buffer)
+  const LamPair<R> *lam_in;       // dL/du_{n+1}
+  LamPair<R> *lam_out;            // dL/du_n (another buffer)
   R *vbar;                          // dL/dV accumulator (the handle's real type)
   const R *V;
   const R *gam;
@@ -1726,8 +1761,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
   const __amdgpu_buffer_rsrc_t rup = buf_rsrc(p.u_psi), ruq = buf_rsrc(p.u_pi);
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V);
-  const __amdgpu_buffer_rsrc_t rlp = buf_rsrc(p.lam_in_psi), rlq = buf_rsrc(p.lam_in_pi);
-  const __amdgpu_buffer_rsrc_t rop = buf_rsrc(p.lam_out_psi), roq = buf_rsrc(p.lam_out_pi);
   const __amdgpu_buffer_rsrc_t rvb = buf_rsrc(p.vbar);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;
@@ -1746,7 +1779,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t oor = has_o ? (uint32_t)oo * sizeof(R) : kBufSkip;
-  const uint32_t ow = writer ? oc : kBufSkip;
+  constexpr uint32_t lps = sizeof(LamPair<R>);
+  const uint32_t ol = (uint32_t)ij * lps;
+  const uint32_t ool = has_o ? (uint32_t)oo * lps : kBufSkip;
+  const uint32_t owl = writer ? ol : kBufSkip;
   const uint32_t owd = writer ? (uint32_t)ij * sizeof(R) : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
   const C ph = p.src_ph;
@@ -1784,8 +1820,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     w.s = buf_ld<C>(rup, oc, bc);
     w.q = buf_ld<C>(ruq, oc, bc);
     w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
-    w.ls = buf_ld<C>(rlp, oc, bc);
-    w.lq = buf_ld<C>(rlq, oc, bc);
+    const LamPair<R> l = ld_pair<R>(p.lam_in, (uint32_t)min(max(m, 0), nz - 1), plane, ol);
+    w.ls = l.s;
+    w.lq = l.q;
   };
   auto load_halo = [&](int m, Halo &o) {
     const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
@@ -1793,8 +1830,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     o.s = buf_ld<C>(rup, ooc, bc);
     o.q = buf_ld<C>(ruq, ooc, bc);
     o.v = buf_ld<R>(rv, oor, b * (uint32_t)sizeof(R));
-    o.ls = buf_ld<C>(rlp, ooc, bc);
-    o.lq = buf_ld<C>(rlq, ooc, bc);
+    const LamPair<R> l = ld_pair<R>(p.lam_in, (uint32_t)min(max(m, 0), nz - 1), plane, ool);
+    o.ls = l.s;
+    o.lq = l.q;
   };
   // the soft source of step n on a band plane (forward state only)
   auto inject = [&](int m, C &xs, C &xq, int at) {
@@ -1996,9 +2034,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     }
     {
       const uint32_t b = (uint32_t)max(k, 0) * plane;
-      const uint32_t o = (k >= k0) ? ow : kBufSkip;
-      buf_st<C>(rop, o, b * (uint32_t)sizeof(C), lns);
-      buf_st<C>(roq, o, b * (uint32_t)sizeof(C), lnq);
+      st_pair<R>(p.lam_out, (uint32_t)max(k, 0), plane, (k >= k0) ? owl : kBufSkip, {lns, lnq});
       buf_st<R>(rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);
     }
     ly3b[wr][ss] = cy3;
@@ -2028,16 +2064,19 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   }
 }
 
-// lam[off + t] += conj(c) g[t]  (an exit-plane seed)
+// lam[(off + t) stride] += conj(c) g[t]  (an exit-plane seed; stride 2 addresses the psi
+// component of LamPair buffers)
 template <typename R>
-__global__ void seed_plane(cplx<R> *lam, size_t off, int n, const double2 *g, double cr, double ci) {
+__global__ void seed_plane(cplx<R> *lam, size_t off, int n, int stride, const double2 *g, double cr,
+                           double ci) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const double2 gv = g[t];
-  cplx<R> l = lam[off + t];
+  const size_t at = (off + (size_t)t) * (size_t)stride;
+  cplx<R> l = lam[at];
   l.x += (R)(cr * gv.x + ci * gv.y);
   l.y += (R)(cr * gv.y - ci * gv.x);
-  lam[off + t] = l;
+  lam[at] = l;
 }
 
 // ---------------------------------------------------------------- utilities
