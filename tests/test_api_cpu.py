@@ -221,3 +221,16 @@ def test_no_gpu_means_loud_failure():
     res = RelativisticSolver3D().run(np.zeros((8, 4, 4)), 0.05, (-1, 1, -1, 1, -1, 1),
                                      n_frames=1, steps_per_frame=1)
     assert res is None
+
+
+def test_package_exports_device_classes_lazily():
+    import wideanglepropagation_amd as w
+    from wideanglepropagation_amd.adjoint import ExitWave, exit_wave
+    from wideanglepropagation_amd.batched import BatchedFDTD
+    from wideanglepropagation_amd.engine import FDTD3D
+    import torch
+    assert w.FDTDAdjointFunction is ExitWave and w.ExitWave is ExitWave
+    assert issubclass(w.FDTDAdjointFunction, torch.autograd.Function)
+    assert w.exit_wave is exit_wave and w.BatchedFDTD is BatchedFDTD and w.FDTD3D is FDTD3D
+    with pytest.raises(AttributeError):
+        w.not_a_name

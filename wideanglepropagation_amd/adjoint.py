@@ -160,6 +160,10 @@ class ExitWave(torch.autograd.Function):
         return grad.to(ctx.v_dtype), None
 
 
+# the name the drop-in's component list gives the differentiable FDTD operator
+FDTDAdjointFunction = ExitWave
+
+
 def exit_wave(V, params, nsteps, k_exit, samples=None, checkpoint_every=None, psi0=None,
               pi0=None, source_profile=None, t0=0.0):
     """Differentiable exit wave E(V) (complex128 (ny, nx) on the GPU)."""
