@@ -1603,34 +1603,42 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     }
     // ---- level 2 at k = m-1 (stored when inside the chunk)
     const int k = m - 1;
+    // every wave issues the stores (skipped off the writer lanes): with the same
+    // vector-memory count on every wave the load-counter waits stay exact (1.5 % faster
+    // than stores inside the branch)
+    C u3 = zero, v3 = zero, sk = zero, ln = zero;
+    R g = R(0);
     if (wave_writes) {
       const bool ik = interior(k);
       C sx, sy;
+      if (PASS == 1) {
+        lat(lfb[rd], sx, sy);
+        const C hy = hfwd(k, ra.v, fprev, sx, sy, fcur, fmm);
+        u3 = horner<R>(ra.f, ik ? h3 : R(0), hy);   // b + h/3 M u4
+        lat(lab[rd], sx, sy);
+        v3 = madj(k, ra.v, h2, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
+        sk = cadd<R>(cadd<R>(ra.a, aprev), v3);    // lam + v2 + v3
+        g = ik ? h2 * imcj<R>(aprev, u3) + h3 * imcj<R>(v3, fprev) : R(0);
+      } else {
+        lat(lab[rd], sx, sy);
+        const C w = madj(k, ra.v, h4, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
+        ln = cscale<R>(cadd<R>(cadd<R>(qk.s, aprev), w), keep_of(k));
+        g = ik ? h * imcj<R>(qk.l, fprev) + h4 * imcj<R>(aprev, ra.b) : R(0);
+      }
+    }
+    {
       const uint32_t bcr = (uint32_t)max(k, 0) * plane;
       const uint32_t bc = bcr * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
       const uint32_t orr = (k >= k0) ? owr : kBufSkip;
-      const R vb = qk.vb;
       if (PASS == 1) {
-        lat(lfb[rd], sx, sy);
-        const C hy = hfwd(k, ra.v, fprev, sx, sy, fcur, fmm);
-        const C u3 = horner<R>(ra.f, ik ? h3 : R(0), hy);   // b + h/3 M u4
-        lat(lab[rd], sx, sy);
-        const C v3 = madj(k, ra.v, h2, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
-        const C sk = cadd<R>(cadd<R>(ra.a, aprev), v3);    // lam + v2 + v3
-        const R g = ik ? h2 * imcj<R>(aprev, u3) + h3 * imcj<R>(v3, fprev) : R(0);
         buf_st<C>(ru3, o, bc, u3);
         buf_st<C>(rv3, o, bc, v3);
         buf_st<C>(rsa, o, bc, sk);
-        buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), vb + g);
       } else {
-        lat(lab[rd], sx, sy);
-        const C w = madj(k, ra.v, h4, mask(k, aprev), sx, sy, mask(m, acur), mask(k - 1, amm));
-        const C ln = cscale<R>(cadd<R>(cadd<R>(qk.s, aprev), w), keep_of(k));
-        const R g = ik ? h * imcj<R>(qk.l, fprev) + h4 * imcj<R>(aprev, ra.b) : R(0);
         buf_st<C>(rlo, o, bc, ln);
-        buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), vb + g);
       }
+      buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), qk.vb + g);
     }
     if (PASS == 1) lfb[wr][ss] = fcur;  // pass II's forward output is only used pointwise
     lab[wr][ss] = mask(m, acur);
