@@ -261,7 +261,9 @@ def main():
 
     kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass",
               "schr_fused": f"schr2_pass_p{dom + 1}"}.get(sched, f"rk4_pass_s{dom + 1}")
-    traffic = profiled_traffic(kernel, p)
+    # the Schrodinger summary covers both passes and quotes pass B's bytes
+    traffic = (profiled_traffic("schr2_pass", p) if sched == "schr_fused" and dom == 1
+               else profiled_traffic(kernel, p))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p.equation, shape)

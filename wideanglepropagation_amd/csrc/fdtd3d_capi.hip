@@ -347,6 +347,14 @@ bool fused_plane_fits(const wap_desc &d) {
   return (size_t)d.nx * d.ny * d.nz * cs < ((size_t)1 << 31);
 }
 
+// The fused passes store their results with non-temporal stores when the fields one
+// step touches (4 of the state's size: in and out of two components, or input, u3 and
+// output) outgrow the 256 MB Infinity Cache; below that the next pass or step rereads
+// them from the cache, which non-temporal stores would bypass.  A batch counts its
+// instances together.
+constexpr size_t kStreamStoreBytes = (size_t)256 << 20;
+bool stream_stores(size_t voxels, size_t cs) { return voxels * cs * 4 > kStreamStoreBytes; }
+
 // (hbar c)^2 / dz^2: the uniform z weight of the Klein-Gordon operator (no stretch)
 double kg_hz(const wap_handle *h) {
   const double dz = (h->d.zmax - h->d.zmin) / (h->d.nz - 1.0);
@@ -447,6 +455,7 @@ wap::KG1Args<R> kg1_args(const wap_handle *h, int waves) {
   split_hi_lo(hz, a.hz_hi, a.hz_lo);
   split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + hz) + h->m2, a.d0_hi, a.d0_lo);
   a.h = (R)h->d.dt;
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
   return a;
 }
 
@@ -502,6 +511,7 @@ wap::Schr2Args<float> schr2_args(const wap_handle *h, int waves, int pass) {
   const double dt = h->d.dt;
   a.c1 = (R)(pass == 0 ? dt / 4.0 : dt / 2.0);
   a.c2 = (R)(pass == 0 ? dt / 3.0 : dt);
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
   return a;
 }
 
@@ -1142,6 +1152,9 @@ template <typename R>
 int batch_upload_fused(wap_batch *b) {
   using C = wap::cplx<R>;
   const int n = (int)b->hs.size();
+  size_t voxels = 0;
+  for (const wap_handle *h : b->hs) voxels += h->n;
+  const int stream = stream_stores(voxels, sizeof(C)) ? 1 : 0;  // the batch's fields together
   WAP_HIP(hipStreamSynchronize(b->stream));
   for (int par = 0; par < 2; ++par) {
     if (b->sched == WAP_SCHED_KG_FUSED) {
@@ -1154,6 +1167,7 @@ int batch_upload_fused(wap_batch *b) {
         a[i].in_pi = (const C *)(par ? h->ya_pi : h->pi);
         a[i].out_psi = (C *)(par ? h->psi : h->ya_psi);
         a[i].out_pi = (C *)(par ? h->pi : h->ya_pi);
+        a[i].stream_out = stream;
       }
       WAP_HIP(hipMemcpy(b->d_args[par], a.data(), n * sizeof(a[0]), hipMemcpyHostToDevice));
     } else if constexpr (std::is_same<R, float>::value) {
@@ -1170,6 +1184,7 @@ int batch_upload_fused(wap_batch *b) {
         pb[i].in = (const C *)h->ya_psi;
         pb[i].base = cur;
         pb[i].out = (C *)(par ? h->psi : h->yb_psi);
+        pa[i].stream_out = pb[i].stream_out = stream;
       }
       WAP_HIP(hipMemcpy(b->d_args[2 * par], pa.data(), n * sizeof(pa[0]), hipMemcpyHostToDevice));
       WAP_HIP(hipMemcpy(b->d_args[2 * par + 1], pb.data(), n * sizeof(pb[0]), hipMemcpyHostToDevice));
@@ -1771,6 +1786,7 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   a.h = (R)h->d.dt;
   a.gz_lo = h->gz_lo;
   a.gz_hi = h->gz_hi;
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
   fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)(ntx * nty * ntz));
@@ -1807,6 +1823,7 @@ int adjoint_step_schr(wap_adjoint *ad, int slot) {
   a.ntx = kg1_ntx(h); a.nty = kg1_nty(h, waves); a.ntz = (h->d.nz + tz - 1) / tz;
   a.hx = (R)h->hx; a.hy = (R)h->hy;
   a.h = (R)h->d.dt;
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
   fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)fused_tiles(h, waves));

@@ -102,16 +102,34 @@ __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
   }
 }
-template <typename T>
+// cache policy of a buffer store: kStream sets the nt (non-temporal) bit, for results
+// that no later access of the same launch reads: on grids larger than the Infinity Cache
+// they stream past L2 instead of evicting the inputs that neighbouring tiles still
+// re-read (6 % per step at 256 x 256 x 512); on grids the cache holds, the next pass
+// rereads them from it, so they stay cached there (nt costs 14 % at 128 x 128 x 256)
+constexpr int kCached = 0;
+constexpr int kStream = 2;
+template <typename T, int CPOL = kCached>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, T v) {
   if constexpr (sizeof(T) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, soff, CPOL);
   } else if constexpr (sizeof(T) == 8) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, soff, CPOL);
   } else {
     static_assert(sizeof(T) == 16, "4, 8 or 16 bytes");
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, soff, CPOL);
   }
+}
+
+// a result store whose cache policy the launch picks (stream: a wave-uniform flag; both
+// sides issue one store, so the load-counter accounting is the same either way)
+template <typename T>
+__device__ __forceinline__ void buf_st_pol(bool stream, __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                           uint32_t soff, T v) {
+  if (stream)
+    buf_st<T, kStream>(r, off, soff, v);
+  else
+    buf_st<T, kCached>(r, off, soff, v);
 }
 
 // A 4-byte store at an out-of-range offset: writes nothing, but counts as one
@@ -639,6 +657,7 @@ struct KG1Args {
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
+  int stream_out;  // non-temporal result stores (grids larger than the Infinity Cache)
 };
 
 template <typename R>
@@ -694,22 +713,41 @@ __device__ __forceinline__ cplx<R> kg1_damp_t(R dg, R dv, cplx<R> x) {
 // counter waits exactly for the plane a step needs, never for the prefetch
 // behind it.  The loop is unrolled by four,
 // so the ring of four plane slots rotates by renaming.
+// The LDS of kg1_body, one instance per <R, ROWS> (both store-policy copies of the body
+// share it).  Double-buffered planes: iteration m reads buffer m & 1 and writes buffer
+// (m + 1) & 1, so one barrier per plane separates every write from the reads that
+// follow it.
 template <typename R, int ROWS>
+struct KG1Lds {
+  static constexpr int pw = kWaveX + 4;  // LDS row pitch: columns -2 .. 65 of the tile
+  static constexpr int nps = pw * (ROWS + 2);
+  cplx<R> lq3b[2][nps];  // Y3 (only the tile's columns and side columns are used)
+  cplx<R> lq4b[2][nps];  // Y4
+  cplx<R> lssb[2][nps];  // state psi
+  cplx<R> lsqb[2][nps];  // state pi
+  // sponge rates of the chunk's planes and the source table: uniform reads from LDS
+  // (a uniform global load would be a vector load, and its wait a full drain)
+  R lgam[kKG1MaxTileZ + 2];
+  SrcCoef<R> lsrc[kMaxBand];
+};
+template <typename R, int ROWS>
+__device__ __forceinline__ KG1Lds<R, ROWS> &kg1_lds() {
+  __shared__ KG1Lds<R, ROWS> l;
+  return l;
+}
+
+template <typename R, int ROWS, int CPOL>
 __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
-  // double-buffered planes: iteration m reads buffer m & 1 and writes buffer (m + 1) & 1,
-  // so one barrier per plane separates every write from the reads that follow it.
-  constexpr int pw = kWaveX + 4;  // LDS row pitch: columns -2 .. 65 of the tile
-  constexpr int nps = pw * (ROWS + 2);
-  __shared__ C lq3b[2][nps];  // Y3 (only the tile's columns and side columns are used)
-  __shared__ C lq4b[2][nps];  // Y4
-  __shared__ C lssb[2][nps];  // state psi
-  __shared__ C lsqb[2][nps];  // state pi
-  // sponge rates of the chunk's planes and the source table: uniform reads from LDS
-  // (a uniform global load would be a vector load, and its wait a full drain)
-  __shared__ R lgam[kKG1MaxTileZ + 2];
-  __shared__ SrcCoef<R> lsrc[kMaxBand];
+  KG1Lds<R, ROWS> &L = kg1_lds<R, ROWS>();
+  constexpr int pw = KG1Lds<R, ROWS>::pw;
+  auto &lq3b = L.lq3b;
+  auto &lq4b = L.lq4b;
+  auto &lssb = L.lssb;
+  auto &lsqb = L.lsqb;
+  auto &lgam = L.lgam;
+  auto &lsrc = L.lsrc;
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
@@ -886,8 +924,8 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
     {
       const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
-      buf_st<C>(rop, o, bc, opsi);
-      buf_st<C>(roq, o, bc, opi);
+      buf_st<C, CPOL>(rop, o, bc, opsi);
+      buf_st<C, CPOL>(roq, o, bc, opi);
     }
     lq3b[wr][ss] = cur.y3;
     lq4b[wr][ss] = cur.y4;
@@ -912,15 +950,22 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
   if (k1 == nz) {
     const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
-    buf_st<C>(rop, ow, bc, prev.pp);
-    buf_st<C>(roq, ow, bc, prev.qq);
+    buf_st<C, CPOL>(rop, ow, bc, prev.pp);
+    buf_st<C, CPOL>(roq, ow, bc, prev.qq);
   }
 }
 
 template <typename R, int ROWS>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass(
     const KG1Args<R> p) {
-  kg1_body<R, ROWS>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
+  // the store policy picks between two copies of the body (a branch per store would
+  // split the stores into blocks of their own and cost most of the non-temporal gain)
+  // (float64, which spills here already, keeps one copy)
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  if (sizeof(R) == 4 && p.stream_out)
+    kg1_body<R, ROWS, kStream>(p, tile, p.src_ph);
+  else
+    kg1_body<R, ROWS, kCached>(p, tile, p.src_ph);
 }
 
 // many grids in one launch: the workgroup finds its instance in the tile prefix sums and
@@ -932,7 +977,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
   const int inst = find_instance(tile_begin, ninst, t);
   const KG1Args<R> a = args[inst];
-  kg1_body<R, ROWS>(a, t - tile_begin[inst], ph[inst]);
+  if (sizeof(R) == 4 && a.stream_out)
+    kg1_body<R, ROWS, kStream>(a, t - tile_begin[inst], ph[inst]);
+  else
+    kg1_body<R, ROWS, kCached>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ----------------------------------------- Schrodinger RK4 step in 2 fused passes
@@ -968,6 +1016,7 @@ struct Schr2Args {
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
+  int stream_out;  // non-temporal result stores
 };
 
 // H y from the lateral neighbour sums sx, sy and the z neighbours zp, zm
@@ -1037,6 +1086,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
+  const bool stream = p.stream_out != 0;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
@@ -1121,7 +1171,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
       const C hyv = schr_h<R>(p, ra.v, lzc[2 * t], lzc[2 * t + 1], uprev, sx, sy, ucur, umm);
       const C o = horner<R>(base_of(ra), k == 0 ? R(0) : p.c2, hyv);
       const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
-      buf_st<C>(rout, (k >= k0) ? ow : kBufSkip, bc, o);
+      buf_st_pol<C>(stream, rout, (k >= k0) ? ow : kBufSkip, bc, o);
     }
     lub[wr][ss] = ucur;
     store_state(wr, rc);
@@ -1143,7 +1193,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   // the frozen top plane nz-1 of the last chunk: its level-1 output is b exactly
   if (k1 == nz) {
     const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
-    buf_st<C>(rout, ow, bc, uprev);
+    buf_st_pol<C>(stream, rout, ow, bc, uprev);
   }
 }
 
@@ -1380,6 +1430,7 @@ struct SchrAdjArgs {
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
+  int stream_out;  // non-temporal result stores
 };
 
 // H x with the lateral neighbour sums sx, sy and the z neighbours xp, xm
@@ -1466,6 +1517,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t ow = writer ? oc : kBufSkip;
   const uint32_t owr = writer ? ov : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
+  const bool stream = p.stream_out != 0;
   const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
@@ -1632,13 +1684,13 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
       const uint32_t orr = (k >= k0) ? owr : kBufSkip;
       if (PASS == 1) {
-        buf_st<C>(ru3, o, bc, u3);
-        buf_st<C>(rv3, o, bc, v3);
-        buf_st<C>(rsa, o, bc, sk);
+        buf_st_pol<C>(stream, ru3, o, bc, u3);
+        buf_st_pol<C>(stream, rv3, o, bc, v3);
+        buf_st_pol<C>(stream, rsa, o, bc, sk);
       } else {
-        buf_st<C>(rlo, o, bc, ln);
+        buf_st_pol<C>(stream, rlo, o, bc, ln);
       }
-      buf_st<R>(rvb, orr, bcr * (uint32_t)sizeof(R), qk.vb + g);
+      buf_st_pol<R>(stream, rvb, orr, bcr * (uint32_t)sizeof(R), qk.vb + g);
     }
     if (PASS == 1) lfb[wr][ss] = fcur;  // pass II's forward output is only used pointwise
     lab[wr][ss] = mask(m, acur);
@@ -1713,14 +1765,14 @@ __device__ __forceinline__ LamPair<R> ld_pair(const LamPair<R> *base, uint32_t k
   }
 }
 template <typename R>
-__device__ __forceinline__ void st_pair(LamPair<R> *base, uint32_t k, uint32_t plane, uint32_t off,
-                                        LamPair<R> v) {
+__device__ __forceinline__ void st_pair(bool stream, LamPair<R> *base, uint32_t k, uint32_t plane,
+                                        uint32_t off, LamPair<R> v) {
   const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (size_t)k * plane);
   if constexpr (sizeof(R) == 4) {
-    buf_st<LamPair<R>>(r, off, 0, v);
+    buf_st_pol<LamPair<R>>(stream, r, off, 0, v);
   } else {
-    buf_st<cplx<R>>(r, off, 0, v.s);
-    buf_st<cplx<R>>(r, off + (uint32_t)sizeof(cplx<R>), 0, v.q);
+    buf_st_pol<cplx<R>>(stream, r, off, 0, v.s);
+    buf_st_pol<cplx<R>>(stream, r, off + (uint32_t)sizeof(cplx<R>), 0, v.q);
   }
 }
 
@@ -1744,6 +1796,7 @@ struct KG1AdjArgs {
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
+  int stream_out;  // non-temporal result stores
 };
 
 template <typename R, int ROWS>
@@ -1813,6 +1866,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t owl = writer ? ol : kBufSkip;
   const uint32_t owd = writer ? (uint32_t)ij * sizeof(R) : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
+  const bool stream = p.stream_out != 0;
   const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
@@ -2062,8 +2116,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     }
     {
       const uint32_t b = (uint32_t)max(k, 0) * plane;
-      st_pair<R>(p.lam_out, (uint32_t)max(k, 0), plane, (k >= k0) ? owl : kBufSkip, {lns, lnq});
-      buf_st<R>(rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);
+      st_pair<R>(stream, p.lam_out, (uint32_t)max(k, 0), plane, (k >= k0) ? owl : kBufSkip,
+                 {lns, lnq});
+      buf_st_pol<R>(stream, rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);
     }
     ly3b[wr][ss] = cy3;
     la2b[wr][ss] = ca2;
