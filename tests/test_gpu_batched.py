@@ -148,3 +148,42 @@ def test_fused_batch_odd_steps_and_odd_graph(eq, rng):
     b.close()
     for s in single + sims:
         s.close()
+
+
+def test_autotune_single_grid_returns_a_measured_candidate(tmp_path, monkeypatch):
+    from wideanglepropagation_amd import autotune
+    from wideanglepropagation_amd.physics import klein_gordon_params
+    from wideanglepropagation_amd.workloads import centred_extent
+    monkeypatch.setenv("WAP_AUTOTUNE_CACHE", str(tmp_path / "cache.json"))
+    shape = (40, 26, 64)
+    p = klein_gordon_params(0.025, centred_extent(shape, 0.05), shape, pml_thick=0.15,
+                            k_source=4, precision="f32")
+    ty, tz, ms, table = autotune.tune(p, tile_ys=(8, 16), tile_zs=(8, 16), steps=2, warmup=1)
+    assert len(table) == 4 and all(m > 0 for _, _, m in table)
+    assert (ty, tz, ms) == min(table, key=lambda r: r[2])
+    # the cached answer comes back without timing again
+    assert autotune.tune(p, tile_ys=(8, 16), tile_zs=(8, 16))[:2] == (ty, tz)
+
+
+def test_tune_batch_picks_a_chunk_per_class():
+    from wideanglepropagation_amd import autotune
+    from wideanglepropagation_amd.physics import klein_gordon_params, schrodinger_params
+    from wideanglepropagation_amd.workloads import centred_extent
+    ps = []
+    for i, shape in enumerate([(40, 26, 64), (48, 20, 40), (40, 26, 64), (48, 20, 40)]):
+        mk = schrodinger_params if i % 2 else klein_gordon_params
+        ps.append(mk(0.025, centred_extent(shape, 0.05), shape, pml_thick=0.15, k_source=4,
+                     precision="f32"))
+    choice, ms, table = autotune.tune_batch(ps, tile_zs=(8, 16), steps=2, warmup=1)
+    assert set(choice) == {autotune._class_of(p) for p in ps}
+    assert set(choice.values()) <= {8, 16}
+    assert ms == min(m for _, m in table) and ms > 0
+    # 2 uniform choices, then one alternative per class
+    assert len(table) == 2 + len(choice)
+
+
+def test_batched_study_quick_runs_finite():
+    from wideanglepropagation_amd.studies import batched_study
+    r = batched_study(quick=True)
+    assert r["all_finite"] and r["gvoxel_steps_per_s"] > 0
+    assert r["autotuned"] and all(v["tile_z"] in (16, 32, 64) for v in r["autotuned"].values())

@@ -9,6 +9,13 @@ every candidate with HIP events on a zero-potential grid of the requested
 size and returns the fastest; results are cached per (equation, precision,
 schedule, nz, ny, nx) in a JSON file (``$WAP_AUTOTUNE_CACHE`` or
 ``~/.cache/wideanglepropagation_amd/autotune.json``).
+
+``tune_batch`` tunes the z chunk per size class for grids that run together in
+one ``BatchedFDTD``: there the batch fills the device whatever the chunk, so a
+single grid's best chunk (short, to make enough workgroups on its own) is the
+wrong one (48 uneven grids: 82 Gvoxel-steps/s with single-grid choices against
+105 with 32-plane chunks).  It times the whole batch, first with one chunk for
+every class, then class by class.
 """
 import json
 import os
@@ -98,3 +105,67 @@ def tuned(params, **kw):
     """params with the tuned launch shape."""
     ty, tz, _, _ = tune(params, **kw)
     return params.with_(tile_y=ty, tile_z=tz)
+
+
+def measure_batch(param_list, steps=4, warmup=2):
+    """Device ms per step of one BatchedFDTD over zero-potential grids of param_list."""
+    from .batched import BatchedFDTD
+    sims = []
+    try:
+        for p in param_list:
+            dt = torch.float64 if p.precision == "f64" else torch.float32
+            sims.append(FDTD3D(p, V=torch.zeros(p.shape, dtype=dt, device="cuda")))
+        b = BatchedFDTD(sims)
+        try:
+            b.step(warmup)
+            b.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(b.stream)
+            b.step(steps)
+            e1.record(b.stream)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / steps
+        finally:
+            b.close()
+    finally:
+        for s_ in sims:
+            s_.close()
+
+
+def _class_of(p):
+    return (p.equation, p.precision, p.resolved_schedule(), p.shape)
+
+
+def tune_batch(param_list, tile_zs=(16, 32, 64), tile_y=16, steps=4, warmup=2, verbose=False):
+    """z chunk per size class for grids batched together.  Returns ({class: tile_z},
+    ms per batched step, table of (choice, ms)).  A class is (equation, precision,
+    schedule, shape)."""
+    classes = sorted({_class_of(p) for p in param_list}, key=str)
+
+    def run(choice):
+        ps = [p.with_(tile_y=tile_y, tile_z=choice[_class_of(p)]) for p in param_list]
+        ms = measure_batch(ps, steps, warmup)
+        table.append(({"/".join(map(str, c[:1] + ("x".join(map(str, c[3])),))): tz
+                       for c, tz in choice.items()}, ms))
+        if verbose:
+            print(f"[autotune] batch {table[-1][0]}: {ms:.3f} ms/step", flush=True)
+        return ms
+
+    table = []
+    best, best_ms = None, float("inf")
+    for tz in tile_zs:   # one chunk for every class
+        choice = {c: tz for c in classes}
+        ms = run(choice)
+        if ms < best_ms:
+            best, best_ms = choice, ms
+    for c in classes:    # then class by class
+        for tz in tile_zs:
+            if tz == best[c]:
+                continue
+            choice = dict(best)
+            choice[c] = tz
+            ms = run(choice)
+            if ms < best_ms:
+                best, best_ms = choice, ms
+    return best, best_ms, table

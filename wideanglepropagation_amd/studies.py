@@ -128,7 +128,7 @@ def probe_profile(shape, extent, x0, y0, lam, width=1.0, focus=-5.0):
 
 def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
     from .batched import BatchedFDTD
-    from .autotune import tune
+    from .autotune import tune_batch
     rng = np.random.default_rng(1)
     classes = [(96, 96, 96), (128, 128, 128), (256, 160, 160), (192, 192, 192)]
     if quick:
@@ -141,21 +141,29 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
     sims, meta = [], []
     shapes = [classes[i] for i in rng.integers(0, len(classes), n_inst)]
     energies = rng.uniform(60e3, 300e3, n_inst)
-    tuned = {}
+    params = []
     for i, (shape, ekin) in enumerate(zip(shapes, energies)):
         eq = "schrodinger" if i % 2 == 0 else "klein_gordon"
         lam = wavelength_from_kinetic_energy(ekin)
         ext = centred_extent(shape, spacing, spacing_z)
         mk = schrodinger_params if eq == "schrodinger" else klein_gordon_params
-        p = mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18, precision="f32")
-        key = (eq, shape)
-        if autotune and key not in tuned:
-            # the fused schedules (the float32 default): workgroups of 16 waves, tune the
-            # z chunk
-            ty, tz, ms, _ = tune(p, tile_ys=(16,), tile_zs=(8, 16, 32, 64), steps=3, warmup=1)
-            tuned[key] = (ty, tz, ms)
-        if key in tuned:
-            p = p.with_(tile_y=tuned[key][0], tile_z=tuned[key][1])
+        params.append(mk(lam, ext, shape, pml_thick=16 * spacing_z, k_source=18,
+                         precision="f32"))
+    tuned, tuned_ms = {}, None
+    if autotune:
+        # the fused schedules (the float32 default): workgroups of 16 waves; the z chunk
+        # per size class, timed on the whole batch
+        tuned, tuned_ms, _ = tune_batch(params, tile_zs=(16, 32, 64), tile_y=16)
+        print(f"[batched] tuned z chunks: {tuned_ms:.3f} ms per batched step", file=sys.stderr,
+              flush=True)
+    for i, (shape, p) in enumerate(zip(shapes, params)):
+        eq = p.equation
+        ekin = float(energies[i])
+        lam = wavelength_from_kinetic_energy(ekin)
+        ext = centred_extent(shape, spacing, spacing_z)
+        cls = (p.equation, p.precision, p.resolved_schedule(), p.shape)
+        if cls in tuned:
+            p = p.with_(tile_y=16, tile_z=tuned[cls])
         V = atom_potential(shape, ext, pos, Zs, scr, r_soft=spacing, cutoff=4.0, device="cuda")
         x0, y0 = rng.uniform(ext[0], ext[1]), rng.uniform(ext[2], ext[3])
         prof = probe_profile(shape, ext, x0, y0, lam)
@@ -187,9 +195,9 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
            "total_voxels": vox, "gvoxel_steps_per_s": vox * done / wall / 1e9,
            "gvoxel_steps_per_s_device": vox * done / (dev_ms * 1e-3) / 1e9,
            "graph_steps_per_launch": chunk, "launch_groups": b.tiles(),
-           "autotuned": {f"{k[0]}/{'x'.join(map(str, k[1]))}": {"tile_y": v[0], "tile_z": v[1],
-                                                                  "ms_per_step": v[2]}
+           "autotuned": {f"{k[0]}/{'x'.join(map(str, k[3]))}": {"tile_y": 16, "tile_z": v}
                          for k, v in tuned.items()},
+           "autotuned_ms_per_batched_step": tuned_ms,
            "all_finite": bool(np.all(np.isfinite(norms))),
            "instances_meta": meta[:6]}
     b.close()
