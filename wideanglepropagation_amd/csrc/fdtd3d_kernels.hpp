@@ -690,29 +690,6 @@ __device__ __forceinline__ cplx<R> kg1_damp_t(R dg, R dv, cplx<R> x) {
   return mkc<R>(-dv * x.y, dv * x.x);
 }
 
-// Workgroup layout: ROWS waves of 64 lanes, one row of 64 columns each (rows 0
-// and ROWS-1 are halo rows: pass-1 only), plus one side wave whose first
-// 2 ROWS lanes take the column just left and just right of the tile in every row
-// (pass-1 only).  The inner 64 x (ROWS-2) columns write psi', pi'.  Tiles are 64
-// columns wide, so a power-of-two nx needs no partial tile.  The haloed LDS
-// planes have 4 extra columns (the side columns and the outer neighbours the
-// side lanes read) and 2 extra rows (the outer neighbours the halo rows read).
-//
-// Loop over the pass-1 planes m of the chunk (plus one below and one above):
-//   top of iteration m: LDS S = injected state of plane m, LDS Y = Y3 / Y4 of
-//                       plane m - 1;
-//   compute pass-1 at m (lateral state from S) and pass-2 at m - 1 (lateral
-//   Y3 / Y4 from Y); write Y <- plane m, S <- plane m + 1 into the other buffer
-//   of each pair; barrier.
-// Memory: every lane issues the same five vector-memory operations per plane,
-// two planes ahead: own column psi, pi, V, then either one outer neighbour psi,
-// pi (the side wave and the halo rows; skipped by an out-of-range offset on
-// lanes that have none) or two stores that write nothing (the other waves), and
-// the same two buffer stores of the result (skipped off the writer lanes).  With
-// the same operation count on both sides of the wave-uniform branch the in-order
-// counter waits exactly for the plane a step needs, never for the prefetch
-// behind it.  The loop is unrolled by four,
-// so the ring of four plane slots rotates by renaming.
 // The LDS of kg1_body, one instance per <R, ROWS> (both store-policy copies of the body
 // share it).  Double-buffered planes: iteration m reads buffer m & 1 and writes buffer
 // (m + 1) & 1, so one barrier per plane separates every write from the reads that
@@ -736,6 +713,29 @@ __device__ __forceinline__ KG1Lds<R, ROWS> &kg1_lds() {
   return l;
 }
 
+// Workgroup layout: ROWS waves of 64 lanes, one row of 64 columns each (rows 0
+// and ROWS-1 are halo rows: pass-1 only), plus one side wave whose first
+// 2 ROWS lanes take the column just left and just right of the tile in every row
+// (pass-1 only).  The inner 64 x (ROWS-2) columns write psi', pi'.  Tiles are 64
+// columns wide, so a power-of-two nx needs no partial tile.  The haloed LDS
+// planes have 4 extra columns (the side columns and the outer neighbours the
+// side lanes read) and 2 extra rows (the outer neighbours the halo rows read).
+//
+// Loop over the pass-1 planes m of the chunk (plus one below and one above):
+//   top of iteration m: LDS S = injected state of plane m, LDS Y = Y3 / Y4 of
+//                       plane m - 1;
+//   compute pass-1 at m (lateral state from S) and pass-2 at m - 1 (lateral
+//   Y3 / Y4 from Y); write Y <- plane m, S <- plane m + 1 into the other buffer
+//   of each pair; barrier.
+// Memory: every lane issues the same five vector-memory operations per plane,
+// two planes ahead: own column psi, pi, V, then either one outer neighbour psi,
+// pi (the side wave and the halo rows; skipped by an out-of-range offset on
+// lanes that have none) or two stores that write nothing (the other waves), and
+// the same two buffer stores of the result (skipped off the writer lanes).  With
+// the same operation count on both sides of the wave-uniform branch the in-order
+// counter waits exactly for the plane a step needs, never for the prefetch
+// behind it.  The loop is unrolled by four, so the ring of four plane slots
+// rotates by renaming.
 template <typename R, int ROWS, int CPOL>
 __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
