@@ -123,6 +123,13 @@ def test_schr_fused_ragged_shapes_and_tiles(shape, ty, tz, rng):
     assert rel_l2(got, want) < 2e-6
 
 
+def test_kg_fused_is_the_f32_and_f64_default():
+    p = _params((24, 16, 16), "klein_gordon", precision="f32")
+    assert p.resolved_schedule() == "kg_fused"
+    assert p.with_(precision="f64").resolved_schedule() == "kg_fused"
+    assert p.with_(precision="f32_kahan").resolved_schedule() == "kg_2pass"
+
+
 def test_schr_fused_is_the_f32_default():
     p = _params((24, 16, 16), "schrodinger", precision="f32")
     assert p.resolved_schedule() == "schr_fused"

@@ -381,9 +381,9 @@ int sched_of(const wap_handle *h) {
       return schr_fused_ok(h->d) ? WAP_SCHED_SCHR_FUSED : WAP_SCHED_STAGES;
     return h->d.schedule;
   }
-  if (h->d.schedule == WAP_SCHED_AUTO)  // the fused pass spills registers in float64
-    return h->d.precision == WAP_PREC_F32 && fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED
-                                                                   : WAP_SCHED_KG_2PASS;
+  if (h->d.schedule == WAP_SCHED_AUTO)  // Kahan keeps its compensation in the 2-pass kernels
+    return h->d.precision != WAP_PREC_F32_KAHAN && fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED
+                                                                         : WAP_SCHED_KG_2PASS;
   return h->d.schedule;
 }
 
@@ -393,10 +393,12 @@ bool kg_two_pass(const wap_handle *h) {
 }
 
 // Fused-pass workgroups: tile_y = 4, 8 or 16 waves (default 16), i.e. rows = waves - 1
-// rows of 64 columns (two of them halo rows) plus the side-column wave.
+// rows of 64 columns (two of them halo rows) plus the side-column wave.  float64 takes at
+// most 8 (its fused pass needs ~180 VGPRs: 2 waves per SIMD).
 int kg1_waves(const wap_handle *h) {
   const int ty = h->d.tile_y;
-  return (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
+  const int w = (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
+  return (is_f64(h) && w > 8) ? 8 : w;
 }
 int kg1_ntx(const wap_handle *h) { return (h->d.nx + wap::kWaveX - 1) / wap::kWaveX; }
 int kg1_nty(const wap_handle *h, int waves) {
@@ -1412,6 +1414,7 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
   b->sched = s0;
   b->kg2 = kg_two_pass(h0) && !fused;
   b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
+  if (is_f64(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;

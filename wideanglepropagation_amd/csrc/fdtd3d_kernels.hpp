@@ -955,8 +955,14 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   }
 }
 
+// waves per SIMD the fused KG pass is compiled for: float32 fits 128 VGPRs (16-wave
+// workgroups); float64 needs ~180 and runs 8-wave workgroups (2 waves per SIMD) -- held
+// to 128 it spills ~170 registers and runs at a quarter of the speed
+template <typename R>
+constexpr int kKG1WavesPerEU = sizeof(R) == 4 ? 4 : 2;
+
 template <typename R, int ROWS>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass(
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kKG1WavesPerEU<R>))) void kg1_pass(
     const KG1Args<R> p) {
   // the store policy picks between two copies of the body (a branch per store would
   // split the stores into blocks of their own and cost most of the non-temporal gain)
@@ -971,7 +977,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 // many grids in one launch: the workgroup finds its instance in the tile prefix sums and
 // reads that instance's arguments and step phasor from device memory
 template <typename R, int ROWS>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_pass_batched(
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kKG1WavesPerEU<R>))) void kg1_pass_batched(
     const KG1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
     const cplx<R> *__restrict__ ph) {
   const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
