@@ -51,6 +51,7 @@ PASS_BYTES = {
     # both stencil sweeps in one pass
     ("kg_fused", "klein_gordon", "f32"): (36,),
     ("kg_fused", "klein_gordon", "f64"): (72,),
+    ("kg_fused", "klein_gordon", "f32_kahan"): (68,),   # + the compensation, read and written
     # Schrodinger RK4 polynomial in Horner form: 2 passes of 2 stencil levels
     ("schr_fused", "schrodinger", "f32"): (20, 28),
 }

@@ -59,11 +59,13 @@ extern "C" {
 #define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365 */
 
 /* pass schedule of a Klein-Gordon step (Schrodinger always uses 4 stage passes) */
-#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED for f32, KG_2PASS for f64 and f32_kahan;
-                               * Schrodinger: SCHR_FUSED for f32 (3D), else STAGES */
+#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a field is under 2 GiB,
+                               * else KG_2PASS; Schrodinger: SCHR_FUSED for f32 (3D), else
+                               * STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
-#define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state) */
+#define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state;
+                               * f32_kahan updates its compensation in place) */
 #define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger f32: the RK4 polynomial in Horner form, two passes
                                  * of two stencil levels each (AUTO picks it for f32 with
                                  * frozen end planes) */

@@ -381,9 +381,8 @@ int sched_of(const wap_handle *h) {
       return schr_fused_ok(h->d) ? WAP_SCHED_SCHR_FUSED : WAP_SCHED_STAGES;
     return h->d.schedule;
   }
-  if (h->d.schedule == WAP_SCHED_AUTO)  // Kahan keeps its compensation in the 2-pass kernels
-    return h->d.precision != WAP_PREC_F32_KAHAN && fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED
-                                                                         : WAP_SCHED_KG_2PASS;
+  if (h->d.schedule == WAP_SCHED_AUTO)
+    return fused_plane_fits(h->d) ? WAP_SCHED_KG_FUSED : WAP_SCHED_KG_2PASS;
   return h->d.schedule;
 }
 
@@ -458,6 +457,8 @@ wap::KG1Args<R> kg1_args(const wap_handle *h, int waves) {
   split_hi_lo(2.0 * ((double)a.hx + (double)a.hy + hz) + h->m2, a.d0_hi, a.d0_lo);
   a.h = (R)h->d.dt;
   a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
+  a.cmp_psi = (C *)h->cmp_psi;  // null unless f32_kahan
+  a.cmp_pi = (C *)h->cmp_pi;
   return a;
 }
 
@@ -469,6 +470,7 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)fused_tiles(h, waves));
   wap::KG1Args<R> a = kg1_args<R>(h, waves);
+  const bool kahan = is_kahan(h);  // compensation buffers updated in place
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
     a.in_psi = (const C *)h->psi;
@@ -476,11 +478,22 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
     a.out_psi = (C *)h->ya_psi;
     a.out_pi = (C *)h->ya_pi;
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    switch (waves) {
-      case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 3>), grid, block, 0, h->stream, a); break;
-      case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 7>), grid, block, 0, h->stream, a); break;
-      default: hipLaunchKernelGGL((wap::kg1_pass<R, 15>), grid, block, 0, h->stream, a); break;
+    if constexpr (sizeof(R) == 4) {
+      if (kahan) {
+        switch (waves) {
+          case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 3, true>), grid, block, 0, h->stream, a); break;
+          case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 7, true>), grid, block, 0, h->stream, a); break;
+          default: hipLaunchKernelGGL((wap::kg1_pass<R, 15, true>), grid, block, 0, h->stream, a); break;
+        }
+        goto launched;
+      }
     }
+    switch (waves) {
+      case 4: hipLaunchKernelGGL((wap::kg1_pass<R, 3, false>), grid, block, 0, h->stream, a); break;
+      case 8: hipLaunchKernelGGL((wap::kg1_pass<R, 7, false>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::kg1_pass<R, 15, false>), grid, block, 0, h->stream, a); break;
+    }
+  launched:
     // the new state is in the other buffer
     std::swap(h->psi, h->ya_psi);
     std::swap(h->pi, h->ya_pi);
@@ -717,8 +730,6 @@ int validate(const wap_desc *d) {
   if ((d->schedule == WAP_SCHED_KG_2PASS || d->schedule == WAP_SCHED_KG_FUSED) &&
       d->equation != WAP_EQ_KLEIN_GORDON)
     return fail(WAP_ERR_ARG, "the 2-pass and fused schedules are for the Klein-Gordon equation");
-  if (d->schedule == WAP_SCHED_KG_FUSED && d->precision == WAP_PREC_F32_KAHAN)
-    return fail(WAP_ERR_ARG, "the fused schedule runs f32 and f64 (f32_kahan uses 2 passes)");
   if (d->schedule == WAP_SCHED_KG_FUSED && !fused_plane_fits(*d))
     return fail(WAP_ERR_ARG, "the fused schedule needs each field under 2 GiB");
   return WAP_OK;
@@ -1308,10 +1319,20 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
   const int par = (int)(t & 1);
   if (b->sched == WAP_SCHED_KG_FUSED) {
     const wap::KG1Args<R> *a = (const wap::KG1Args<R> *)b->d_args[par];
+    if constexpr (sizeof(R) == 4) {
+      if (b->precision == WAP_PREC_F32_KAHAN) {
+        switch (b->waves) {
+          case 4: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 3, true>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+          case 8: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 7, true>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+          default: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 15, true>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+        }
+        return;
+      }
+    }
     switch (b->waves) {
-      case 4: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 3>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
-      case 8: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 7>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
-      default: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 15>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+      case 4: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 3, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+      case 8: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 7, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+      default: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 15, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
     }
     return;
   }

@@ -640,6 +640,7 @@ struct KG1Args {
   using C = cplx<R>;
   const C *in_psi, *in_pi;  // state n
   C *out_psi, *out_pi;      // state n + 1 (another buffer)
+  C *cmp_psi, *cmp_pi;      // Kahan compensation of the state (updated in place)
   const R *V;
   const R *gam;
   const C *prof;
@@ -736,7 +737,7 @@ __device__ __forceinline__ KG1Lds<R, ROWS> &kg1_lds() {
 // counter waits exactly for the plane a step needs, never for the prefetch
 // behind it.  The loop is unrolled by four, so the ring of four plane slots
 // rotates by renaming.
-template <typename R, int ROWS, int CPOL>
+template <typename R, int ROWS, int CPOL, bool KAHAN>
 __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
@@ -772,6 +773,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   const __amdgpu_buffer_rsrc_t rsp = buf_rsrc(p.in_psi), rsq = buf_rsrc(p.in_pi);
   const __amdgpu_buffer_rsrc_t rsv = buf_rsrc(p.V);
   const __amdgpu_buffer_rsrc_t rop = buf_rsrc(p.out_psi), roq = buf_rsrc(p.out_pi);
+  const __amdgpu_buffer_rsrc_t rcs = buf_rsrc(p.cmp_psi), rcq = buf_rsrc(p.cmp_pi);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;  // LDS slot
   // the outer neighbour this lane loads for the halo of S: x for side lanes,
@@ -856,6 +858,15 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
     for (int t = tid; t <= m_end - m_start; t += kWaveX * (ROWS + 1)) lgam[t] = p.gam[m_start + t];
     if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
   }
+  struct Cmp {  // Kahan compensation of psi, pi at one plane (own column, writer lanes)
+    C s, q;
+  };
+  auto load_cmp = [&](int m, Cmp &w) {
+    const uint32_t bc = (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(C);
+    const uint32_t o = (m >= k0 && m < k1) ? ow : kBufSkip;
+    w.s = buf_ld<C>(rcs, o, bc);
+    w.q = buf_ld<C>(rcq, o, bc);
+  };
   Raw r0, r1, r2, r3;
   load_plane(m_start - 1, r0);
   load_plane(m_start, r1);
@@ -872,10 +883,12 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   C y3mm = zero, y4mm = zero;  // Y3 / Y4 at m-2
 
   // one plane: ra, rb hold planes m-1, m (injected), rc plane m+1 (raw), pre receives m+2
-  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
+  // Kahan: ck holds the compensation of plane m-1 (the pass-2 plane), cn receives plane m's
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre, Cmp &ck, Cmp &cn) {
     const int rd = m & 1, wr = rd ^ 1;
     const C *lss = lssb[rd], *lsq = lsqb[rd];  // state of plane m
     const C *lq3 = lq3b[rd], *lq4 = lq4b[rd];  // Y3 / Y4 of plane m - 1
+    if constexpr (KAHAN) load_cmp(m, cn);
     load_plane(m + 2, pre);
     inject_plane(m + 1, rc);
     // ---- pass-1 at m.  On the frozen end planes the step factors are 0, which
@@ -900,8 +913,13 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
       cur.y3 = caxpy<R>(sb, f2, u);
       cur.z3 = caxpy<R>(qb, f2, gg);
       cur.y4 = caxpy<R>(sb, f1, cur.z3);
-      cur.pp = caxpy<R>(sb, f6, caxpy<R>(qb, R(2), cadd<R>(u, cur.z3)));
-      cur.qq = caxpy<R>(qb, f6, caxpy<R>(f, R(2), gg));
+      if (KAHAN && !frozen) {  // the increments; pass 2 adds them to the state compensated
+        cur.pp = cscale<R>(caxpy<R>(qb, R(2), cadd<R>(u, cur.z3)), f6);
+        cur.qq = cscale<R>(caxpy<R>(f, R(2), gg), f6);
+      } else {
+        cur.pp = caxpy<R>(sb, f6, caxpy<R>(qb, R(2), cadd<R>(u, cur.z3)));
+        cur.qq = caxpy<R>(qb, f6, caxpy<R>(f, R(2), gg));
+      }
     }
     // ---- pass-2 at k = m-1 (stored when inside the chunk); k = 0 is frozen (the
     // factor 0 leaves psi, pi), and k <= nz - 2 since m <= nz - 1
@@ -918,14 +936,33 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
       const C wv3 = caxpy<R>(qk, h, k3);
       const C k4 = cadd<R>(kg1_apply<R>(p, prev.dl, prev.y4, sx4, sy4, cadd<R>(cur.y4, y4mm)),
                            kg1_damp<R>(dg, prev.dv, wv3));
-      opsi = caxpy<R>(prev.pp, g6, wv3);
-      opi = caxpy<R>(prev.qq, g6, caxpy<R>(k4, R(2), k3));
+      if (!KAHAN || k == 0) {  // (Kahan: the frozen plane keeps its state, pp = psi)
+        opsi = caxpy<R>(prev.pp, g6, wv3);
+        opi = caxpy<R>(prev.qq, g6, caxpy<R>(k4, R(2), k3));
+      } else {
+        // state + increment with the compensation of fdtd3d's Kahan passes: y = inc - c,
+        // t = state + y, c' = (t - state) - y; on band planes the compensation is scaled by
+        // the source's keep factor like the state it belongs to
+        const int bnd = k - p.src_k0;
+        const R kp = (bnd >= 0 && bnd < p.src_n) ? lsrc[bnd].keep : R(1);
+        const C ys = csub<R>(caxpy<R>(prev.pp, h6, wv3), cscale<R>(ck.s, kp));
+        const C yq = csub<R>(caxpy<R>(prev.qq, h6, caxpy<R>(k4, R(2), k3)), cscale<R>(ck.q, kp));
+        const C sk = ra.s;  // injected psi at k
+        opsi = cadd<R>(sk, ys);
+        opi = cadd<R>(qk, yq);
+        ck.s = csub<R>(csub<R>(opsi, sk), ys);
+        ck.q = csub<R>(csub<R>(opi, qk), yq);
+      }
     }
     {
       const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
       buf_st<C, CPOL>(rop, o, bc, opsi);
       buf_st<C, CPOL>(roq, o, bc, opi);
+      if constexpr (KAHAN) {
+        buf_st<C, CPOL>(rcs, o, bc, ck.s);
+        buf_st<C, CPOL>(rcq, o, bc, ck.q);
+      }
     }
     lq3b[wr][ss] = cur.y3;
     lq4b[wr][ss] = cur.y4;
@@ -936,17 +973,19 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
     prev = cur;
   };
 
+  Cmp c0, c1;  // (plane m_start - 1 is never stored: its compensation is not loaded)
+  c0.s = c0.q = c1.s = c1.q = zero;
   int m = m_start;
   for (; m + 3 <= m_end; m += 4) {
-    plane_step(m, r0, r1, r2, r3);
-    plane_step(m + 1, r1, r2, r3, r0);
-    plane_step(m + 2, r2, r3, r0, r1);
-    plane_step(m + 3, r3, r0, r1, r2);
+    plane_step(m, r0, r1, r2, r3, c0, c1);
+    plane_step(m + 1, r1, r2, r3, r0, c1, c0);
+    plane_step(m + 2, r2, r3, r0, r1, c0, c1);
+    plane_step(m + 3, r3, r0, r1, r2, c1, c0);
   }
   // the last 0-3 planes continue the same rotation (m is uniform: no divergence)
-  if (m <= m_end) plane_step(m, r0, r1, r2, r3);
-  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0);
-  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1);
+  if (m <= m_end) plane_step(m, r0, r1, r2, r3, c0, c1);
+  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0, c1, c0);
+  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, c0, c1);
   // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
   if (k1 == nz) {
     const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
@@ -961,7 +1000,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
 template <typename R>
 constexpr int kKG1WavesPerEU = sizeof(R) == 4 ? 4 : 2;
 
-template <typename R, int ROWS>
+template <typename R, int ROWS, bool KAHAN>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kKG1WavesPerEU<R>))) void kg1_pass(
     const KG1Args<R> p) {
   // the store policy picks between two copies of the body (a branch per store would
@@ -969,14 +1008,14 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // (float64, which spills here already, keeps one copy)
   const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
   if (sizeof(R) == 4 && p.stream_out)
-    kg1_body<R, ROWS, kStream>(p, tile, p.src_ph);
+    kg1_body<R, ROWS, kStream, KAHAN>(p, tile, p.src_ph);
   else
-    kg1_body<R, ROWS, kCached>(p, tile, p.src_ph);
+    kg1_body<R, ROWS, kCached, KAHAN>(p, tile, p.src_ph);
 }
 
 // many grids in one launch: the workgroup finds its instance in the tile prefix sums and
 // reads that instance's arguments and step phasor from device memory
-template <typename R, int ROWS>
+template <typename R, int ROWS, bool KAHAN>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kKG1WavesPerEU<R>))) void kg1_pass_batched(
     const KG1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
     const cplx<R> *__restrict__ ph) {
@@ -984,9 +1023,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int inst = find_instance(tile_begin, ninst, t);
   const KG1Args<R> a = args[inst];
   if (sizeof(R) == 4 && a.stream_out)
-    kg1_body<R, ROWS, kStream>(a, t - tile_begin[inst], ph[inst]);
+    kg1_body<R, ROWS, kStream, KAHAN>(a, t - tile_begin[inst], ph[inst]);
   else
-    kg1_body<R, ROWS, kCached>(a, t - tile_begin[inst], ph[inst]);
+    kg1_body<R, ROWS, kCached, KAHAN>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ----------------------------------------- Schrodinger RK4 step in 2 fused passes

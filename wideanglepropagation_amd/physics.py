@@ -144,7 +144,7 @@ class FDTDParams:
                 return "schr_fused" if ok else "stages"
             return self.schedule
         if self.schedule == "auto":
-            return "kg_fused" if self.precision != "f32_kahan" and fits else "kg_2pass"
+            return "kg_fused" if fits else "kg_2pass"
         return self.schedule
 
     def with_(self, **kw):

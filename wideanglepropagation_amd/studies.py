@@ -87,8 +87,13 @@ def precision_study(quick=False):
     n_frames = 40
     out = {"study": "precision", "grid_nz_ny_nx": list(shape), "steps": nsteps, "runs": {}}
     ref = None
-    for prec in ("f64", "f32_kahan", "f32"):
+    # each precision on its default schedule (the fused pass), then the float32 modes on
+    # the 2-pass schedule, which differences the neighbours before scaling them
+    runs = [("f64", "auto"), ("f32_kahan", "auto"), ("f32", "auto"),
+            ("f32_kahan", "kg_2pass"), ("f32", "kg_2pass")]
+    for prec, sched in runs:
         p, V = kg_slab(shape, precision=prec, device="cuda")
+        p = p.with_(schedule=sched)
         V = V.to(torch.float32)  # every run sees the same float32-rounded potential
         V = V.to(torch.float64) if prec == "f64" else V
         k_exit = p.nz - 16 - 3
@@ -106,8 +111,9 @@ def precision_study(quick=False):
             r["rel_l2_far_field_vs_f64"] = rel_l2(far_field(ex), ref[1])
             r["norm_drift_vs_f64"] = (n_final - ref[2]) / ref[2]
             r["norm_drift_trajectory"] = [(s, (n - ref[3][s]) / ref[3][s]) for s, n in norms]
-        out["runs"][prec] = r
-        print(f"[precision] {prec}: " + json.dumps({k: v for k, v in r.items()
+        key = prec if sched == "auto" else f"{prec}/{sched}"
+        out["runs"][key] = r
+        print(f"[precision] {key}: " + json.dumps({k: v for k, v in r.items()
                                                     if "trajectory" not in k}), flush=True)
     return out
 
