@@ -234,3 +234,27 @@ def test_package_exports_device_classes_lazily():
     assert w.exit_wave is exit_wave and w.BatchedFDTD is BatchedFDTD and w.FDTD3D is FDTD3D
     with pytest.raises(AttributeError):
         w.not_a_name
+
+
+def test_adjoint_segment_plan():
+    # the segments tile [0, T) with the full-length one last; the plan is the longest
+    # segment within the budget (brute force over every length)
+    from wideanglepropagation_amd.adjoint import plan_segment, segment_starts
+    for T, S in ((1500, 39), (1500, 118), (10, 3), (10, 10), (10, 25), (7, 1)):
+        st = segment_starts(T, S)
+        ends = st[1:] + [T]
+        assert st[0] == 0 and all(b > a for a, b in zip(st, ends))
+        assert all(b - a <= S for a, b in zip(st, ends))
+        assert ends[-1] - st[-1] == min(S, T)
+        assert len(st) == -(-T // S)
+    slot, ck, fixed = 1 << 30, 1 << 30, 5 << 30
+    for T in (1, 2, 17, 100, 1500):
+        for budget in (0, 8 << 30, 40 << 30, 130 << 30, 2000 << 30):
+            lo = max(1, -(-int(T ** 0.5 * 1e6) // 1000000))
+            S = plan_segment(T, slot, ck, fixed, budget, lo)
+            fits = [s for s in range(lo, T + 1)
+                    if s * slot + (-(-T // s) - 1) * ck + fixed <= budget]
+            assert S == (max(fits) if fits else lo), (T, budget, S)
+    # 128 x 128 x 256 complex64 KG over 1500 steps fits whole in 140 GB: nothing recomputed
+    v = 128 * 128 * 256
+    assert plan_segment(1500, 16 * v, 16 * v, 40 * v, 140 << 30, 39) == 1500

@@ -256,3 +256,55 @@ def test_schr_fused_adjoint_leaves_handle_intact(rng):
     finally:
         a.close()
         b.close()
+
+
+def test_adjoint_segment_plans_agree(rng):
+    # E and dL/dV do not depend on how the steps are cut into segments (all recorded in the
+    # forward, the budget's fallback of sqrt(T), single steps): every record is the same
+    # deterministic forward step, so the results agree bitwise
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
+    print("case adjoint segment plans", flush=True)
+    p64, Vn = _setup("klein_gordon", "f64", rng)
+    p = p64.with_(precision="f32")
+    nsteps, k_exit = 30, p.nz - 5
+    samples = {30: 1.0, 24: 0.5, 17: 0.25, 5: 0.1}
+    out = []
+    for kw in ({"checkpoint_every": 7}, {"memory_budget": 1 << 40}, {"memory_budget": 0},
+               {"checkpoint_every": 1}, {"checkpoint_every": 30}):
+        V = torch.tensor(Vn, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = exit_wave(V, p, nsteps, k_exit, samples=samples, psi0=p64.init[0],
+                      pi0=p64.init[1], **kw)
+        torch.sum(far_field_intensity(E) ** 2).backward()
+        out.append((E.detach(), V.grad.detach()))
+    assert out[0][1].abs().max() > 0
+    for E, g in out[1:]:
+        assert torch.equal(E, out[0][0])
+        assert torch.equal(g, out[0][1])
+
+
+def test_adjoint_forward_without_backward(rng):
+    # a differentiable forward whose graph is dropped before its backward gives the handle
+    # its buffers back and frees its records; the next run is unaffected
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gc
+    from wideanglepropagation_amd.adjoint import exit_wave
+    print("case adjoint forward without backward", flush=True)
+    p64, Vn = _setup("klein_gordon", "f64", rng)
+    p = p64.with_(precision="f32")
+    kw = dict(samples={12: 1.0, 9: 0.5}, psi0=p64.init[0], pi0=p64.init[1])
+    with torch.no_grad():
+        E0 = exit_wave(torch.tensor(Vn, dtype=torch.float32, device="cuda"), p, 12,
+                       p.nz - 5, **kw)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    for _ in range(2):
+        V = torch.tensor(Vn, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = exit_wave(V, p, 12, p.nz - 5, **kw)
+        assert torch.equal(E.detach(), E0)
+        del E, V
+        gc.collect()
+        torch.cuda.synchronize()
+        assert torch.cuda.memory_allocated() == before

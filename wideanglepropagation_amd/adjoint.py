@@ -10,10 +10,17 @@ and returns the demodulated exit wave
 functional of the states psi_s after s steps, t_s = t0 + s dt).  It is a
 ``torch.autograd.Function``: the backward is the discrete adjoint of the
 same RK4 steps (hand-written HIP kernels, ``wap_adjoint_*`` in the C ABI),
-with sqrt(T) checkpointing -- states every C = ceil(sqrt(T)) steps are kept
-during the forward; each segment is recomputed once with its stage inputs
-recorded and then swept backwards.  Any torch loss of E (far field, FFT
-intensities) composes with it.
+with checkpointing: the T steps are cut into segments of S >= ceil(sqrt(T))
+steps, counted back from step T.  The forward keeps the state at the start of
+every segment but the last and records the last segment's steps as it runs
+them (a record step is the forward step with its output kept); the backward
+sweeps the last segment straight away and recomputes each earlier one once,
+with its records, before sweeping it.  S is the longest segment whose records
+and checkpoints fit a memory budget (half the free device memory unless
+``memory_budget`` says otherwise), so the backward recomputes T - S steps:
+none when the whole run fits, as it does for grids of a few million voxels
+over thousands of steps in 288 GB.  ``checkpoint_every=C`` fixes S = C.  Any
+torch loss of E (far field, FFT intensities) composes with it.
 """
 import ctypes
 import math
@@ -31,16 +38,22 @@ class _Adjoint:
 
     def __init__(self, sim, slots):
         self._L = lib()
-        self.sim = sim
-        nb = ctypes.c_int64()
-        check(self._L.wap_adjoint_workspace_bytes(sim.handle, int(slots), ctypes.byref(nb)),
-              "wap_adjoint_workspace_bytes")
-        self._ws = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
+        self.sim = sim   # held, so that the handle outlives this object
+        self._a = None
+        nb = self.workspace_bytes(sim, slots)
+        self._ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
         a = ctypes.c_void_p()
         check(self._L.wap_adjoint_create_in(sim.handle, int(slots),
-                                            ctypes.c_void_p(self._ws.data_ptr()), int(nb.value),
+                                            ctypes.c_void_p(self._ws.data_ptr()), nb,
                                             ctypes.byref(a)), "wap_adjoint_create_in")
         self._a = a
+
+    @staticmethod
+    def workspace_bytes(sim, slots):
+        nb = ctypes.c_int64()
+        check(lib().wap_adjoint_workspace_bytes(sim.handle, int(slots), ctypes.byref(nb)),
+              "wap_adjoint_workspace_bytes")
+        return int(nb.value)
 
     def record(self, step, slot):
         check(self._L.wap_adjoint_record(self._a, int(step), self.sim.t0, int(slot)),
@@ -60,10 +73,18 @@ class _Adjoint:
         return out
 
     def close(self):
+        # before the handle is destroyed: the records swap buffers with the handle, and
+        # destroy gives the handle its own buffers back
         if self._a is not None:
             check(self._L.wap_adjoint_destroy(self._a), "wap_adjoint_destroy")
             self._a = None
             self._ws = None   # back to the caching allocator (destroy synchronised)
+
+    def __del__(self):   # a forward whose backward never ran
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_samples(nsteps, n_frames=None, start_frac=0.6):
@@ -77,6 +98,51 @@ def default_samples(nsteps, n_frames=None, start_frac=0.6):
     return {(f + 1) * spf: w for f in frames}
 
 
+# the plan of the most recent differentiable forward (for reports): segment length,
+# checkpoints kept, steps the backward recomputes
+last_plan = {}
+
+
+def plan_segment(nsteps, slot_bytes, ckpt_bytes, fixed_bytes, budget, min_len):
+    """Segment length S: the longest in [min_len, T] whose S record slots, ceil(T/S) - 1
+    checkpoints and the adjoint's fixed buffers fit in budget bytes (min_len when none
+    does)."""
+    T = int(nsteps)
+    lo = max(1, min(int(min_len), T))
+    hi = T if slot_bytes <= 0 else max(lo, min(T, int((budget - fixed_bytes) // slot_bytes)))
+    for S in range(hi, lo - 1, -1):
+        if S * slot_bytes + (-(-T // S) - 1) * ckpt_bytes + fixed_bytes <= budget:
+            return S
+    return lo
+
+
+def segment_starts(nsteps, S):
+    """First steps of the segments of S steps counted back from nsteps (ascending; the first
+    segment is the short one)."""
+    return sorted({max(0, int(nsteps) - j * S) for j in range(1, -(-int(nsteps) // S) + 1)})
+
+
+def _default_budget():
+    free, _ = torch.cuda.mem_get_info()
+    cached = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+    return (free + cached) // 2
+
+
+def _segment_length(sim, nsteps, opts):
+    C = opts.get("checkpoint_every")
+    if C:
+        return max(1, min(int(C), int(nsteps)))
+    p = sim.params
+    w1 = _Adjoint.workspace_bytes(sim, 1)
+    slot = _Adjoint.workspace_bytes(sim, 2) - w1
+    ncomp = 2 if p.equation == "klein_gordon" else 1
+    ckpt = p.voxels * ncomp * (16 if p.precision == "f64" else 8)
+    budget = opts.get("memory_budget")
+    budget = _default_budget() if budget is None else int(budget)
+    return plan_segment(nsteps, slot, ckpt, w1 - slot, budget,
+                        max(1, math.ceil(math.sqrt(nsteps))))
+
+
 class ExitWave(torch.autograd.Function):
     """V -> E (see module docstring).  Non-tensor arguments are passed through ``opts``."""
 
@@ -86,7 +152,6 @@ class ExitWave(torch.autograd.Function):
         nsteps = int(opts["nsteps"])
         k_exit = int(opts["k_exit"])
         samples = opts.get("samples") or {nsteps: 1.0}
-        C = int(opts.get("checkpoint_every") or max(1, math.ceil(math.sqrt(nsteps))))
         sim = FDTD3D(p, V=V.detach().contiguous(), source_profile=opts.get("source_profile"),
                      t0=opts.get("t0", 0.0))
         if opts.get("psi0") is not None:
@@ -95,44 +160,70 @@ class ExitWave(torch.autograd.Function):
         tmp = torch.zeros_like(E)
         ckpt = {}
         keep = bool(ctx.needs_input_grad[0])   # no checkpoints when no gradient is wanted
-        # step from event to event (a checkpoint before step n, a sample after step s)
-        events = sorted({s for s in samples if 0 < s <= nsteps} |
-                        ({n for n in range(0, nsteps, C)} if keep else set()) | {nsteps})
-        n = 0
-        for e in events:
-            if keep and n % C == 0 and n not in ckpt:
-                ckpt[n] = sim.get_state(np.complex128 if p.precision == "f64" else np.complex64,
-                                        device=True)
-            if e > n:
-                sim.step(e - n)
-                n = e
+        cdt = np.complex128 if p.precision == "f64" else np.complex64
+
+        def sample(n):
             if n in samples:
                 tmp.zero_()
                 sim.accumulate_plane(k_exit, p.e_sim * (sim.t0 + n * p.dt), tmp)
-                E += samples[n] * tmp
-            if keep and n % C == 0 and n < nsteps and n not in ckpt:
-                ckpt[n] = sim.get_state(np.complex128 if p.precision == "f64" else np.complex64,
-                                        device=True)
+                E.add_(samples[n] * tmp)
+
+        adj = None
+        try:
+            if keep:
+                S = _segment_length(sim, nsteps, opts)
+                starts = segment_starts(nsteps, S)
+                last = starts[-1]
+                last_plan.clear()
+                last_plan.update(segment=S, checkpoints=len(starts) - 1, recomputed_steps=last)
+            else:
+                starts, last = [], nsteps
+            # plain steps up to the last segment, from event to event (a checkpoint at a
+            # segment start, a sample after step s)
+            events = sorted({s for s in samples if 0 < s <= last} | set(starts[:-1]) | {last})
+            n = 0
+            for e in events:
+                if e > n:
+                    sim.step(e - n)
+                    n = e
+                sample(n)
+                if n in starts[:-1]:
+                    ckpt[n] = sim.get_state(cdt, device=True)
+            if keep:   # the last segment: each step recorded for the backward
+                adj = _Adjoint(sim, nsteps - last)
+                for m in range(last, nsteps):
+                    adj.record(m, m - last)
+                    sample(m + 1)
+        except BaseException:
+            if adj is not None:
+                adj.close()
+            sim.close()
+            raise
         if not keep:
             sim.close()
             ctx.sim = None
             ctx.ckpt = None
             return E
         ctx.sim = sim
+        ctx.adj = adj
         ctx.ckpt = ckpt
-        ctx.opts = dict(opts, samples=samples, checkpoint_every=C)
+        ctx.starts = starts
+        ctx.opts = dict(opts, samples=samples)
         ctx.v_dtype = V.dtype
         return E
 
     @staticmethod
     def backward(ctx, gE):
         sim = ctx.sim
+        if sim is None:
+            raise RuntimeError("exit_wave: backward called twice")
         p = sim.params
         o = ctx.opts
-        nsteps, k_exit, C = int(o["nsteps"]), int(o["k_exit"]), int(o["checkpoint_every"])
+        nsteps, k_exit = int(o["nsteps"]), int(o["k_exit"])
         samples = o["samples"]
+        starts = ctx.starts
         g = gE.detach().to(torch.complex128).contiguous()
-        adj = _Adjoint(sim, C)
+        adj = ctx.adj
         try:
             def seed(s):
                 if s in samples:
@@ -140,15 +231,17 @@ class ExitWave(torch.autograd.Function):
                     c = samples[s] * complex(math.cos(p.e_sim * t), math.sin(p.e_sim * t))
                     adj.seed(k_exit, g, c)
             seed(nsteps)
-            for start in reversed(range(0, nsteps, C)):
-                end = min(start + C, nsteps)
-                st = ctx.ckpt[start]
-                if isinstance(st, tuple):
-                    sim.set_state(st[0], st[1])
-                else:
-                    sim.set_state(st)
-                for m in range(start, end):
-                    adj.record(m, m - start)
+            ends = starts[1:] + [nsteps]
+            for i in reversed(range(len(starts))):
+                start, end = starts[i], ends[i]
+                if i < len(starts) - 1:   # recompute (the last segment was recorded forward)
+                    st = ctx.ckpt.pop(start)
+                    if isinstance(st, tuple):
+                        sim.set_state(st[0], st[1])
+                    else:
+                        sim.set_state(st)
+                    for m in range(start, end):
+                        adj.record(m, m - start)
                 for m in reversed(range(start, end)):
                     adj.step(m - start)
                     seed(m)
@@ -156,6 +249,8 @@ class ExitWave(torch.autograd.Function):
         finally:
             adj.close()
             sim.close()
+            ctx.adj = None
+            ctx.sim = None
             ctx.ckpt = None
         return grad.to(ctx.v_dtype), None
 
@@ -165,11 +260,12 @@ FDTDAdjointFunction = ExitWave
 
 
 def exit_wave(V, params, nsteps, k_exit, samples=None, checkpoint_every=None, psi0=None,
-              pi0=None, source_profile=None, t0=0.0):
-    """Differentiable exit wave E(V) (complex128 (ny, nx) on the GPU)."""
+              pi0=None, source_profile=None, t0=0.0, memory_budget=None):
+    """Differentiable exit wave E(V) (complex128 (ny, nx) on the GPU).  memory_budget caps
+    the device bytes of the backward's records and checkpoints."""
     opts = {"params": params, "nsteps": nsteps, "k_exit": k_exit, "samples": samples,
             "checkpoint_every": checkpoint_every, "psi0": psi0, "pi0": pi0,
-            "source_profile": source_profile, "t0": t0}
+            "source_profile": source_profile, "t0": t0, "memory_budget": memory_budget}
     return ExitWave.apply(V, opts)
 
 

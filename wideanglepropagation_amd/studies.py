@@ -21,7 +21,8 @@ batched    48 instances, sizes from {96^3, 128^3, 160x160x256, 192^3}, beam
 adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
            L = || |fft2(exit)|^2 - I_target ||^2 with I_target from a copy of the
            150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
-           (seed 3).  sqrt(T) checkpointing.  Checks dL/dV against central
+           (seed 3).  Checkpointing in segments of >= sqrt(T) steps sized to half the
+           free device memory.  Checks dL/dV against central
            differences of the f64 forward at 32 random voxels (seed 4) and reports
            backward / forward time; the same loss and step count are also timed
            on the 512 x 512 x 256 grid of the precision study (timing only).
@@ -214,6 +215,7 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
 
 def adjoint_study(quick=False, n_fd=32):
     from .adjoint import exit_wave, far_field_intensity
+    from .adjoint import last_plan as adjoint_plan
     shape = (64, 32, 32) if quick else (256, 128, 128)
     nsteps = 150 if quick else 1500
     if quick:
@@ -244,6 +246,7 @@ def adjoint_study(quick=False, n_fd=32):
     with torch.no_grad():
         L0 = loss_of(V32, p32)
     t_fwd, t_fb, Vg = _time_forward_backward(lambda Vx: loss_of(Vx, p32), V32)
+    plan = dict(adjoint_plan)
     g32 = Vg.grad.double().cpu().numpy()
     print(f"[adjoint] forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s", file=sys.stderr,
           flush=True)
@@ -277,7 +280,7 @@ def adjoint_study(quick=False, n_fd=32):
     err64 = [abs(r[2] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
     err32 = [abs(r[3] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
     return {"study": "adjoint", "grid_nz_ny_nx": list(shape), "steps": nsteps,
-            "checkpoint_every": math.ceil(math.sqrt(nsteps)), "loss": float(L0),
+            "segment_plan": plan, "loss": float(L0),
             "t_forward_s": t_fwd, "t_forward_backward_s": t_fb,
             "backward_over_forward": (t_fb - t_fwd) / t_fwd,
             "fd_step": hstep, "fd_voxels": len(rows),
@@ -317,6 +320,7 @@ def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
     """Forward and forward + backward time of the adjoint study's loss on another grid
     (random target: timing only)."""
     from .adjoint import exit_wave, far_field_intensity
+    from .adjoint import last_plan as adjoint_plan
     p, V = kg_slab(shape, precision="f32", device="cuda", n_atoms=n_atoms, seed=2)
     p = p.with_(dt=1e-6)
     V32 = V.to(torch.float32)
@@ -332,10 +336,11 @@ def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
     with torch.no_grad():  # warm-up (allocations, library load)
         loss_of(V32)
     t_fwd, t_fb, _ = _time_forward_backward(loss_of, V32)
+    plan = dict(adjoint_plan)
     print(f"[adjoint] {shape}: forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s",
           file=sys.stderr, flush=True)
     return {"grid_nz_ny_nx": list(shape), "steps": nsteps,
-            "checkpoint_every": math.ceil(math.sqrt(nsteps)), "t_forward_s": t_fwd,
+            "segment_plan": plan, "t_forward_s": t_fwd,
             "t_forward_backward_s": t_fb, "backward_over_forward": (t_fb - t_fwd) / t_fwd}
 
 
