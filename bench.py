@@ -10,8 +10,8 @@ in HBM before the timed region; the timed region is K steps bracketed by
 
 Printed (one JSON line): Gvoxel-steps/s; the roofline of the dominant kernel
 (algorithmic bytes per launch / its mean HIP-event duration, measured on the
-solver's stream); the CPU baseline (the FP64 NumPy oracle on a bounded slab
-of the same workload, timed on this host); the cost of the gradient's reverse
+solver's stream); the CPU baseline (the oracle's FP64 step restated in C +
+OpenMP, on a bounded slab of the same workload, timed on this host's cores); the cost of the gradient's reverse
 steps on the same grid (``adjoint``: a recorded forward step and an adjoint
 step against a plain step, per step).
 
@@ -148,12 +148,14 @@ def copy_calibration(gib, reps=3):
                       "copy_GBs": 2 * n * 4 / el / 1e9}), flush=True)
 
 
-def cpu_baseline(eq, shape, budget_s=10.0, max_steps=64):
-    """FP64 NumPy oracle on a bounded slab of the workload: the same lateral size, 1/8 of
-    the planes, as many steps as fit in about budget_s seconds (at least one),
-    single-threaded NumPy."""
+def cpu_baseline(eq, shape, budget_s=10.0, max_steps=2000):
+    """FP64 C + OpenMP restatement of the oracle's step (oracle/fdtd3d_omp.c, checked
+    against the NumPy oracle in tests/test_oracle_c.py) on a bounded slab of the workload:
+    the same lateral size, 1/8 of the planes; a warm-up step and two timed steps to size
+    the sample, then as many steps as fit in about budget_s seconds in one call, on up to
+    16 host threads."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import fdtd3d
+    from oracle import fdtd3d, fdtd3d_c
     from wideanglepropagation_amd.workloads import kg_slab, schrodinger_gaussian_64
     nz, ny, nx = shape
     sub = (max(8, nz // 8), ny, nx)
@@ -165,21 +167,25 @@ def cpu_baseline(eq, shape, budget_s=10.0, max_steps=64):
         p, V, st, _ = schrodinger_gaussian_64("f64")
         sub = p.shape
     d = p.oracle_dict()
-    t = time.perf_counter()
-    nsteps = 0
-    while nsteps < max_steps and (nsteps == 0 or time.perf_counter() - t < budget_s):
-        st = fdtd3d.run(st, V, 1, d, step0=nsteps)
-        nsteps += 1
-    el = time.perf_counter() - t
-    vox = sub[0] * sub[1] * sub[2]
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
-    return {"value": vox * nsteps / el / 1e9, "unit": "Gvoxel-steps/s", "cores": 1,
+    threads = max(1, min(16, cores))
+    st = fdtd3d_c.run(st, V, 1, d, threads=threads)       # warm-up (library, pages)
+    t = time.perf_counter()
+    st = fdtd3d_c.run(st, V, 2, d, step0=1, threads=threads)
+    one = (time.perf_counter() - t) / 2
+    nsteps = max(1, min(max_steps, int(budget_s / max(one, 1e-6))))
+    t = time.perf_counter()
+    st = fdtd3d_c.run(st, V, nsteps, d, step0=3, threads=threads)
+    el = time.perf_counter() - t
+    vox = sub[0] * sub[1] * sub[2]
+    return {"value": vox * nsteps / el / 1e9, "unit": "Gvoxel-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"FP64 NumPy oracle, {eq} {sub[0]}x{sub[1]}x{sub[2]} (nz,ny,nx), "
-                      f"{nsteps} step(s), {el:.1f} s, 1 thread of {cores} host cores"}
+            "sample": f"FP64 C + OpenMP restatement of the oracle's RK4 step "
+                      f"(oracle/fdtd3d_omp.c), {eq} {sub[0]}x{sub[1]}x{sub[2]} (nz,ny,nx), "
+                      f"{nsteps} step(s) in {el:.1f} s, {threads} threads of {cores} host cores"}
 
 
 def main():
