@@ -18,6 +18,9 @@ extract   the notebooks' exit-wave / far-field recipe
           ``propagation.py:346-350``).
 spectral  the Fresnel / angular-spectrum multislice the notebooks compare FDTD
           exit waves against (``propagation.py:14-177, 305-352``), JAX-free.
+fdtd3d_c  ctypes front of ``fdtd3d_omp.c``: the same FP64 step restated in C +
+          OpenMP (one streaming sweep per RK4 stage), bench.py's multi-threaded
+          CPU baseline; coefficients come from ``fdtd3d``.
 
 Pinning (what checks each function, in ``tests/``):
   physics.*            reference-held golden scalars (notebook outputs and the
@@ -31,6 +34,10 @@ Pinning (what checks each function, in ``tests/``):
   fdtd3d (KG)          closed-form KG dispersion in a periodic box
                        (omega = V +- sqrt(c^2 k_d^2 + m^2)); charge
                        conservation without sponge.
+  fdtd3d_c.run         fdtd3d.run to 1e-12 (both equations, source profile, 2D
+                       zero-end rule, float32 clock, 1 and 3 threads), and a
+                       planted dropped sponge term that the check catches
+                       (``tests/test_oracle_c.py``).
   extract.*            synthetic snapshots with a known answer.
   spectral.*           textbook electron wavelengths, the refractive index in
                        60-digit decimal arithmetic, single Fourier modes
