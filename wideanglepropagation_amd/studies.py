@@ -1,6 +1,7 @@
-"""Workload drivers: precision study, batched probes, adjoint gradient, spectral comparison.
+"""Workload drivers: precision study, batched probes, adjoint gradient, spectral comparison,
+the API entry points.
 
-``python -m wideanglepropagation_amd.studies precision|batched|adjoint|spectral [--quick]``
+``python -m wideanglepropagation_amd.studies precision|batched|adjoint|spectral|api [--quick]``
 prints one JSON object per study.  Sizes follow the solver's standard
 workloads:
 
@@ -33,6 +34,12 @@ spectral   The plane-wave comparison of ``fdtd_3D_baseline_comparison.ipynb``:
            steps, at the notebook's dz = 2/127 A and at dz = 0.005 A; the exit wave
            against the Fresnel and angular spectrum multislices of the same
            potential.  Also the GPU multislice's slices/s at 512 x 512 x 256.
+api        The entry points around the stepper: the GPU atom-potential generator
+           (400 atoms on 256 x 256 x 512), the reference's RelativisticSolver3D.run
+           on that grid (Schrodinger, host V, 1000 steps, one snapshot) and the 2D
+           RelativisticSolver.run on 2048 x 4096 (1000 steps), and the
+           file-contract command (backend="cli") against the in-process run of a
+           64^3 job with 4 snapshots.
 """
 import argparse
 import json
@@ -484,13 +491,86 @@ def spectral_study(quick=False):
     return {"study": "spectral", "comparisons": rows, "multislice_throughput": thr}
 
 
+def api_study(quick=False):
+    """Throughput of the drop-in entry points around the stepper: the reference's Python
+    API (``RelativisticSolver3D.run`` and the 2D ``RelativisticSolver.run``, in process,
+    one snapshot), the file-contract command (``backend="cli"``) against the in-process
+    run on the same grid, and the GPU atom-potential generator."""
+    import contextlib
+    import io
+    from .fdtd_solver import RelativisticSolver, RelativisticSolver3D
+    from .workloads import atom_potential, centred_extent, random_slab
+    out = {"study": "api"}
+    quiet = contextlib.redirect_stdout(io.StringIO())   # the API prints like the reference
+
+    # the atom-potential generator on the benchmark grid (400 atoms, one launch)
+    shape = (128, 128, 256) if quick else (512, 256, 256)
+    ext = centred_extent(shape, 0.05, 0.005)
+    half = ext[5] - 20 * 0.005
+    pos, Zs, scr = random_slab(400, ((ext[0], ext[1] + 0.05), (ext[2], ext[3] + 0.05)),
+                               (-half, half), 0)
+    best = math.inf
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        V = atom_potential(shape, ext, pos, Zs, scr, r_soft=0.05, device="cuda")
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    nvox = shape[0] * shape[1] * shape[2]
+    out["atom_potential"] = {"grid_nz_ny_nx": list(shape), "atoms": 400, "ms": best * 1e3,
+                             "Gvoxels_per_s": nvox / best / 1e9}
+
+    # the reference's 3D API (Schrodinger, Angstrom / eV), host V in, one snapshot out
+    Vh = V.cpu().numpy()
+    nfr, spf = (4, 50) if quick else (10, 100)
+    with quiet:
+        t = time.perf_counter()
+        r = RelativisticSolver3D().run(Vh, 0.025, ext, n_frames=nfr, steps_per_frame=spf,
+                                       snapshot_count=1, pml_thick=0.08, k_source=18)
+        wall = time.perf_counter() - t
+    out["solver3d_run"] = {"grid_nz_ny_nx": list(shape), "steps": nfr * spf,
+                           "stepping_Gvoxel_steps_per_s": r["gvoxel_steps_per_s"],
+                           "wall_s": wall,
+                           "wall_Gvoxel_steps_per_s": nvox * nfr * spf / wall / 1e9}
+
+    # the 2D API (atomic units, y the propagation axis)
+    ny2, nx2 = (1024, 512) if quick else (4096, 2048)
+    V2 = np.zeros((ny2, nx2), dtype=np.float32)
+    with quiet:
+        t = time.perf_counter()
+        r = RelativisticSolver().run(V2, 0.05, (-20.0, 20.0, -40.0, 40.0), n_frames=nfr,
+                                     steps_per_frame=spf, snapshot_count=1)
+        wall = time.perf_counter() - t
+    out["solver2d_run"] = {"grid_ny_nx": [ny2, nx2], "steps": nfr * spf,
+                           "stepping_Gvoxel_steps_per_s": r["gvoxel_steps_per_s"],
+                           "wall_s": wall}
+
+    # the file contract (potential.bin -> fdtd_3d_bin -> snapshots.bin) against the
+    # in-process run of the same job
+    cshape = (64, 64, 64)
+    cext = centred_extent(cshape, 0.05)
+    Vc = np.zeros(cshape, dtype=np.float32)
+    times = {}
+    for backend in ("hip", "cli"):
+        with quiet:
+            t = time.perf_counter()
+            r = RelativisticSolver3D().run(Vc, 0.05, cext, n_frames=4, steps_per_frame=25,
+                                           snapshot_count=4, pml_thick=0.4, k_source=10,
+                                           backend=backend)
+            times[backend] = time.perf_counter() - t
+        assert r is not None and len(r["snapshots"]) == 4
+    out["cli_vs_inprocess"] = {"grid_nz_ny_nx": list(cshape), "steps": 100, "snapshots": 4,
+                               "inprocess_s": times["hip"], "cli_s": times["cli"]}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["precision", "batched", "adjoint", "spectral"])
+    ap.add_argument("study", choices=["precision", "batched", "adjoint", "spectral", "api"])
     ap.add_argument("--quick", action="store_true")
     a = ap.parse_args(argv)
     fn = {"precision": precision_study, "batched": batched_study, "adjoint": adjoint_study,
-          "spectral": spectral_study}
+          "spectral": spectral_study, "api": api_study}
     out = fn[a.study](quick=a.quick)
     print(json.dumps(out), flush=True)
 
