@@ -2339,7 +2339,10 @@ __global__ __launch_bounds__(256) void atom_potential_kernel(
       double ey = y - sa[a * 5 + 1];
       ex -= Lx * floor(ex / Lx + 0.5);   // nearest image, then neighbours within rc
       ey -= Ly * floor(ey / Ly + 0.5);
-      const int mx = (int)ceil(rc / Lx), my = (int)ceil(rc / Ly);
+      // |ex| <= Lx/2, so image s lies at least |s| Lx - Lx/2 away: beyond
+      // floor(rc/Lx + 1/2) periods every image fails the r <= rc test below (the
+      // 1e-9 keeps a boundary image in; an extra image only costs its test)
+      const int mx = (int)floor(rc / Lx + 0.5 + 1e-9), my = (int)floor(rc / Ly + 0.5 + 1e-9);
       const double zq = sa[a * 5 + 3];
       for (int sy = -my; sy <= my; ++sy) {
         const double fy = ey + sy * Ly;
