@@ -258,3 +258,13 @@ def test_adjoint_segment_plan():
     # 128 x 128 x 256 complex64 KG over 1500 steps fits whole in 140 GB: nothing recomputed
     v = 128 * 128 * 256
     assert plan_segment(1500, 16 * v, 16 * v, 40 * v, 140 << 30, 39) == 1500
+
+
+def test_exit_wave_rejects_no_steps():
+    # checked before any device work, so it runs without a GPU
+    import torch
+    from wideanglepropagation_amd.adjoint import exit_wave
+    from wideanglepropagation_amd.physics import klein_gordon_params
+    p = klein_gordon_params(0.05, (-0.4, 0.4, -0.4, 0.4, -0.5, 0.5), (12, 8, 8))
+    with pytest.raises(ValueError, match="nsteps"):
+        exit_wave(torch.zeros((12, 8, 8), dtype=torch.float32), p, 0, 6)

@@ -151,6 +151,8 @@ class ExitWave(torch.autograd.Function):
         p = opts["params"]
         nsteps = int(opts["nsteps"])
         k_exit = int(opts["k_exit"])
+        if nsteps < 1:
+            raise ValueError(f"exit_wave: nsteps must be >= 1 (got {nsteps})")
         samples = opts.get("samples") or {nsteps: 1.0}
         sim = FDTD3D(p, V=V.detach().contiguous(), source_profile=opts.get("source_profile"),
                      t0=opts.get("t0", 0.0))
