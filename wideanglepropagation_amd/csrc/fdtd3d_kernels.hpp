@@ -180,6 +180,23 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   return xcd * q + (xcd < rem ? xcd : rem) + pos;
 }
 
+// The order of a plane-walking pass's tiles on one XCD (applied to xcd_tile's index):
+// groups of G columns, each group's z chunks one after another, G the workgroups an XCD
+// holds at once.  Chunk c+1 of a column then starts as chunk c ends, and its lower halo
+// planes (the planes chunk c read last) come from the XCD's L2 instead of HBM; in the
+// column-major order the same chunk started 80+ tiles later, long after those planes
+// left the 4 MB L2 (fused KG step 274 -> 256 us at 256x256x512, 545 -> 493 us at
+// 512x512x256).  A bijection of [0, ncol * ntz); the last group may be narrower.
+constexpr int kXcdCus = 32;  // CUs per XCD
+template <int G>
+__device__ __forceinline__ int zgroup_tile(int o, int ncol, int ntz) {
+  const int g = o / (G * ntz);
+  const int base = g * G;
+  const int gw = min(G, ncol - base);
+  const int w = o - base * ntz;
+  return base + w % gw + ncol * (w / gw);
+}
+
 // Soft-source coefficients of one plane.  The band is a few planes, so
 // membership is uniform over a plane: it is evaluated once per plane (and
 // the kernarg arrays are selected without dynamic indexing).  ph = e^{-i w t}
@@ -1006,7 +1023,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // the store policy picks between two copies of the body (a branch per store would
   // split the stores into blocks of their own and cost most of the non-temporal gain)
   // (float64, which spills here already, keeps one copy)
-  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tile = zgroup_tile<kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty,
+                                        p.ntz);  // one 16-wave workgroup per CU
   if (sizeof(R) == 4 && p.stream_out)
     kg1_body<R, ROWS, kStream, KAHAN>(p, tile, p.src_ph);
   else
@@ -1245,7 +1263,10 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
 template <typename R, int ROWS, int PB>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass(
     const Schr2Args<R> p) {
-  schr2_body<R, ROWS, PB>(p, xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.src_ph);
+  // two 16-wave workgroups per CU
+  schr2_body<R, ROWS, PB>(
+      p, zgroup_tile<2 * kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty, p.ntz),
+      p.src_ph);
 }
 
 template <typename R, int ROWS, int PB>
@@ -1520,7 +1541,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   __shared__ C lfa[2][nps], laa[2][nps], lfb[2][nps], lab[2][nps];
   __shared__ C lzc[2 * (kKG1MaxTileZ + 6)];  // planes m_start - 2 .. m_end + 1
   __shared__ SrcCoef<R> lsrc[kMaxBand];
-  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tile = zgroup_tile<kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty,
+                                        p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
@@ -1864,7 +1886,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // by the same lane only, so one buffer and no barrier)
   __shared__ C lols[(ROWS + 1) * kWaveX], lolq[(ROWS + 1) * kWaveX];
   __shared__ R lov[(ROWS + 1) * kWaveX];
-  const int tile = xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz);
+  const int tile = zgroup_tile<kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty,
+                                        p.ntz);
   const int tx = tile % p.ntx;
   const int rest = tile / p.ntx;
   const int tyi = rest % p.nty;
