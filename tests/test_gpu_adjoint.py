@@ -308,3 +308,22 @@ def test_adjoint_forward_without_backward(rng):
         gc.collect()
         torch.cuda.synchronize()
         assert torch.cuda.memory_allocated() == before
+
+
+def test_exit_wave_sample_at_step_zero(rng):
+    # a sample at step 0 (the initial state) counts the same with and without a gradient,
+    # and its seed reaches the gradient
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.adjoint import exit_wave
+    print("case exit wave sample at step 0", flush=True)
+    p64, Vn = _setup("klein_gordon", "f64", rng)
+    kw = dict(samples={0: 0.5, 9: 1.0}, psi0=p64.init[0], pi0=p64.init[1])
+    V = torch.tensor(Vn, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        E0 = exit_wave(V, p64, 9, p64.nz - 5, **kw)
+    Vg = V.clone().requires_grad_(True)
+    E = exit_wave(Vg, p64, 9, p64.nz - 5, **kw)
+    assert torch.equal(E.detach(), E0)
+    E.abs().pow(2).sum().backward()
+    assert torch.isfinite(Vg.grad).all() and Vg.grad.abs().max() > 0

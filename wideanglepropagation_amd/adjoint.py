@@ -182,7 +182,8 @@ class ExitWave(torch.autograd.Function):
                 starts, last = [], nsteps
             # plain steps up to the last segment, from event to event (a checkpoint at a
             # segment start, a sample after step s)
-            events = sorted({s for s in samples if 0 < s <= last} | set(starts[:-1]) | {last})
+            # (a sample at step 0 is the initial state, with or without a gradient)
+            events = sorted({s for s in samples if 0 <= s <= last} | set(starts[:-1]) | {last})
             n = 0
             for e in events:
                 if e > n:
