@@ -13,17 +13,22 @@ Printed (one JSON line): Gvoxel-steps/s; the roofline of the dominant kernel
 solver's stream); the CPU baseline (the oracle's FP64 step restated in C +
 OpenMP, on a bounded slab of the same workload, timed on this host's cores); the cost of the gradient's reverse
 steps on the same grid (``adjoint``: a recorded forward step and an adjoint
-step against a plain step, per step).
+step against a plain step, per step, and the backward / forward ratio of a
+1500-step run checkpointed every ceil(sqrt(1500)) = 39 steps at those costs).
 
-Roofline bytes.  ``achieved`` counts the canonical bytes of an RK4 step that
-streams once per stage (Klein-Gordon complex64: 272 B per voxel-step,
+Roofline bytes.  ``achieved`` counts the algorithmic bytes of the schedule that
+runs: what the dominant kernel must read and write per voxel (the fused
+Klein-Gordon pass: read psi, pi, V, write psi', pi' = 36 B; float64 72 B; Kahan
+adds the compensation's read and write), times the voxels of one launch, over
+its mean duration; ``frac`` = achieved / 8 TB/s.  ``traffic`` is the HBM bytes per
+launch that rocprofv3 measured for the kernel on this workload (FETCH_SIZE +
+WRITE_SIZE, calibrated on a plain copy), read from the summary named in
+``traffic_source``.  ``canonical_rate`` restates the step at the bytes of an RK4
+step that streams once per stage (Klein-Gordon complex64: 272 B per voxel-step,
 Schrodinger complex64: 144 B; FP64 doubles them; Kahan adds the compensation
-buffer), apportioned to the dominant kernel by its share of its schedule's
-own bytes.  A schedule that fuses stages moves fewer bytes than that, so
-``frac`` may exceed 1.  ``compulsory_*`` counts the bytes the kernel itself
-must move (the fused Klein-Gordon pass: read psi, pi, V, write psi', pi' =
-36 B per voxel), ``traffic`` the HBM bytes rocprofv3 measured for it, and
-``measured_stream_*`` the best plain copy / read on the same device.
+buffer), the basis the throughput target is quoted on; a fused schedule moves
+fewer bytes than that, so its ``x_peak`` may exceed 1.  ``measured_stream_*`` is
+the best plain copy / read on the same device.
 """
 import argparse
 import json
@@ -69,8 +74,9 @@ def schedule_of(p):
 
 
 def profiled_traffic(kernel, p):
-    """HBM bytes per launch of this kernel on this workload, from the committed rocprofv3
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy); None if not profiled."""
+    """(HBM bytes per launch of this kernel on this workload, the summary's path) from the
+    committed rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy);
+    (None, None) if not profiled."""
     import re
     eq = "kg" if p.equation == "klein_gordon" else "schr"
     name = f"{kernel}_{eq}_{p.nx}x{p.ny}x{p.nz}_{p.precision}.md"
@@ -78,9 +84,9 @@ def profiled_traffic(kernel, p):
     try:
         with open(path) as f:
             m = re.search(r"traffic_bytes_per_launch\s*=\s*(\d+)", f.read())
-        return int(m.group(1)) if m else None
+        return (int(m.group(1)), f"profiles/{name}") if m else (None, None)
     except OSError:
-        return None
+        return None, None
 
 
 def parse():
@@ -244,7 +250,7 @@ def main():
     comp_bytes = pb[dom] * vox
     canon_bytes = int(round(CANONICAL_BYTES[(p.equation, p.precision)] * vox * pb[dom] / sum(pb)))
     dom_s = avg[dom] * 1e-3
-    achieved = canon_bytes / dom_s / 1e9
+    canon_gbs = canon_bytes / dom_s / 1e9
     comp_gbs = comp_bytes / dom_s / 1e9
     step_bytes = sum(pb) * vox
     step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
@@ -253,9 +259,16 @@ def main():
     if args.adjoint_steps > 0 and p.precision != "f32_kahan":
         rec_s, rev_s = adjoint_cost(sim, args.adjoint_steps)
         step_s = elapsed / args.steps
+        T = 1500
+        S = int(np.ceil(np.sqrt(T)))
         adjoint = {"steps": args.adjoint_steps, "step_ms": round(step_s * 1e3, 4),
                    "record_ms": round(rec_s * 1e3, 4), "adjoint_ms": round(rev_s * 1e3, 4),
                    "backward_over_forward": round((rec_s + rev_s) / step_s, 3),
+                   # T steps checkpointed every S: the backward recomputes the T - S steps
+                   # of every segment but the last (recorded in the forward) and reverses T
+                   "sqrt_t_plan": {"T": T, "S": S, "recomputed_steps": T - S,
+                                   "backward_over_forward": round(
+                                       ((T - S) * rec_s + T * rev_s) / (T * step_s), 3)},
                    "adjoint_kernel": ("schr_adj_pass (2 passes)" if sched == "schr_fused" else
                                       "kg1_adj_pass" if p.equation == "klein_gordon"
                                       and vox * 8 < 2 ** 31 else "adj_pass (4 stage passes)"),
@@ -269,8 +282,9 @@ def main():
     kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass",
               "schr_fused": f"schr2_pass_p{dom + 1}"}.get(sched, f"rk4_pass_s{dom + 1}")
     # the Schrodinger summary covers both passes and quotes pass B's bytes
-    traffic = (profiled_traffic("schr2_pass", p) if sched == "schr_fused" and dom == 1
-               else profiled_traffic(kernel, p))
+    # the Schrodinger summary covers both passes and quotes pass B's bytes
+    traffic, traffic_src = (profiled_traffic("schr2_pass", p) if sched == "schr_fused" and dom == 1
+                            else profiled_traffic(kernel, p))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p.equation, shape)
@@ -293,23 +307,29 @@ def main():
                                f"({p.precision}), 400-atom slab, PML 16 planes, 200 keV",
                    "grid": [nz, ny, nx], "precision": p.precision,
                    "equation": p.equation, "schedule": sched, "tile_y": ty, "tile_z": tz},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "achieved": round(comp_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
                                 "kg_fused": "kg1_pass",
                                 "schr_fused": f"schr2_pass pass {'AB'[dom]}"}.get(
                                     sched, f"rk4_pass stage {dom + 1}"),
-                     "basis": f"canonical {CANONICAL_BYTES[(p.equation, p.precision)]} B per "
-                              "voxel-step (one streaming pass per RK4 stage)",
-                     "bytes_per_launch": canon_bytes,
+                     "basis": f"algorithmic {pb[dom]} B per voxel of the executed {sched} "
+                              f"schedule's dominant kernel x {vox} voxels per launch",
+                     "bytes_per_launch": comp_bytes,
                      "mean_launch_ms": round(avg[dom], 5),
-                     "compulsory_bytes_per_launch": comp_bytes,
-                     "compulsory_achieved": round(comp_gbs, 1),
-                     "compulsory_frac": round(comp_gbs / HBM_PEAK_GBS, 4),
+                     "time_source": "HIP events on the solver's stream",
+                     "traffic_source": (f"{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE of "
+                                        "this kernel and workload, copy-calibrated)"
+                                        if traffic else None),
                      "traffic_achieved": (round(traffic / dom_s / 1e9, 1) if traffic else None),
                      "traffic_frac": (round(traffic / dom_s / 1e9 / HBM_PEAK_GBS, 4)
                                       if traffic else None)},
+        "canonical_rate": {"basis": f"{CANONICAL_BYTES[(p.equation, p.precision)]} B per "
+                                    "voxel-step (one streaming pass per RK4 stage), apportioned "
+                                    "to the dominant kernel by its share of the step's bytes",
+                           "bytes_per_launch": canon_bytes, "GBs": round(canon_gbs, 1),
+                           "x_peak": round(canon_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "detail": {"pass_ms": [round(a, 5) for a in avg],

@@ -56,7 +56,8 @@ extern "C" {
 
 /* time of step n */
 #define WAP_TIME_EXACT 0  /* t = t0 + (step0 + n) dt in float64 */
-#define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365 */
+#define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365;
+                             t0 must be 0 (WAP_ERR_ARG otherwise) */
 
 /* pass schedule of a Klein-Gordon step (Schrodinger always uses 4 stage passes) */
 #define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a field is under 2 GiB,

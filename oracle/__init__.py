@@ -32,8 +32,20 @@ Pinning (what checks each function, in ``tests/``):
                        box; bitwise determinism; the reference's five physics
                        invariants (``tests/test_fdtd_3d.py:203-323``).
   fdtd3d (KG)          closed-form KG dispersion in a periodic box
-                       (omega = V +- sqrt(c^2 k_d^2 + m^2)); charge
-                       conservation without sponge.
+                       (omega = V +- sqrt(c^2 k_d^2 + m^2), ``tests/test_oracle.py``);
+                       and in ``tests/test_oracle_kg.py``: the incident wave
+                       (pi = -i w psi) passing ``inject`` unchanged and, on the
+                       discrete dispersion relation, stepping by P4(-i w dt); the
+                       sponge's closed form and a step against P4(dt M) assembled
+                       from an independent sponge and source; the KG charge
+                       conserved to RK4 order without sponge and drained at the
+                       damping term's rate with it; the source's wave rising in
+                       phase along +z with the fully discrete wavenumber.  Each
+                       check fails on a mistake planted in the oracle (+i w, a
+                       dropped profile, r^(p+1), a dropped or flipped 2 i V pi,
+                       gamma pi for 2 gamma pi).
+  fdtd3d.ref_time      the int * float product of ``fdtd_3d.cu:365`` rounded to
+                       binary32 in exact rational arithmetic.
   fdtd3d_c.run         fdtd3d.run to 1e-12 (both equations, source profile, 2D
                        zero-end rule, float32 clock, 1 and 3 threads), and a
                        planted dropped sponge term that the check catches

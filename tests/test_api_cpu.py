@@ -268,3 +268,31 @@ def test_exit_wave_rejects_no_steps():
     p = klein_gordon_params(0.05, (-0.4, 0.4, -0.4, 0.4, -0.5, 0.5), (12, 8, 8))
     with pytest.raises(ValueError, match="nsteps"):
         exit_wave(torch.zeros((12, 8, 8), dtype=torch.float32), p, 0, 6)
+
+
+def test_f32_clock_rejects_a_time_origin():
+    # the reference's float32 clock (fdtd_3d.cu:365) has no t0: a handle on it refuses one
+    # before any device work, so source, demodulation and adjoint seeds share one clock
+    from wideanglepropagation_amd.engine import FDTD3D
+    from wideanglepropagation_amd.physics import schrodinger_params
+    p = schrodinger_params(0.05, (-1, 1, -1, 1, -1, 1), (8, 4, 4), reference_quantize=True)
+    assert p.time_mode == "f32"
+    with pytest.raises(ValueError, match="t0"):
+        FDTD3D(p, t0=1e-3)
+
+
+def test_exit_wave_rejects_planes_outside_the_grid():
+    import torch
+    from wideanglepropagation_amd.adjoint import exit_wave
+    from wideanglepropagation_amd.physics import klein_gordon_params
+    p = klein_gordon_params(0.05, (-0.4, 0.4, -0.4, 0.4, -0.5, 0.5), (12, 8, 8))
+    with pytest.raises(ValueError, match="planes"):
+        exit_wave(torch.zeros((12, 8, 8), dtype=torch.float32), p, 5, 6, k_monitor=12)
+
+
+def test_default_samples_are_the_recipe_window():
+    # the last 40 % of the frames (start = int(0.6 M)), equal weights, at frame ends
+    from wideanglepropagation_amd.adjoint import default_samples
+    s = default_samples(1500, 10)
+    assert sorted(s) == [1050, 1200, 1350, 1500] and all(w == 0.25 for w in s.values())
+    assert default_samples(100, None) == {100: 1.0}

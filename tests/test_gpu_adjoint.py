@@ -327,3 +327,50 @@ def test_exit_wave_sample_at_step_zero(rng):
     assert torch.equal(E.detach(), E0)
     E.abs().pow(2).sum().backward()
     assert torch.isfinite(Vg.grad).all() and Vg.grad.abs().max() > 0
+
+
+def test_exit_wave_recipe_against_oracle_and_its_gradient(rng):
+    # the notebook recipe as a differentiable function: the last 40 % of 10 frame ends,
+    # demodulated, averaged, divided by the monitor plane's incident amplitude
+    # (fdtd_3D_baseline_comparison.ipynb:267-290); forward against the oracle's states
+    # put through oracle/extract.exit_wave, gradient against central differences
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from oracle import extract, fdtd3d
+    from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
+    print("case exit wave recipe with monitor plane (f64 KG)", flush=True)
+    p, Vn = _setup("klein_gordon", "f64", rng)
+    nsteps, n_frames, k_exit, k_mon = 40, 10, p.nz - 5, 8
+    kw = dict(psi0=p.init[0], pi0=p.init[1], k_monitor=k_mon, n_frames=n_frames)
+    V = torch.tensor(Vn, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        E = exit_wave(V, p, nsteps, k_exit, **kw).cpu().numpy()
+    d = p.oracle_dict()
+    st = p.init
+    snaps = []
+    for f in range(n_frames):
+        st = fdtd3d.run(st, Vn, 4, d, step0=4 * f)
+        snaps.append((4 * (f + 1) * p.dt, st[0].real, st[0].imag))
+    want, inc = extract.exit_wave(snaps, p.e_sim, k_exit, k_mon, start_frac=0.6)
+    assert inc > 1e-9
+    assert np.linalg.norm(E - want) <= 1e-10 * np.linalg.norm(want)
+    target = 0.5 * far_field_intensity(torch.tensor(want, device="cuda"))
+
+    def loss(Vx):
+        return torch.sum((far_field_intensity(exit_wave(Vx, p, nsteps, k_exit, **kw)) - target) ** 2)
+
+    Vg = V.clone().requires_grad_(True)
+    loss(Vg).backward()
+    g = Vg.grad.cpu().numpy()
+    # voxels between the monitor and the exit plane, and the monitor plane itself
+    idx = [(int(rng.integers(k_mon + 1, k_exit)), int(rng.integers(0, p.ny)),
+            int(rng.integers(0, p.nx))) for _ in range(4)]
+    idx += [(k_mon, 2, 3), (k_mon - 1, 5, 1)]
+    h = 0.05
+    for ix in idx:
+        Vp, Vm = V.clone(), V.clone()
+        Vp[ix] += h
+        Vm[ix] -= h
+        with torch.no_grad():
+            fd = (loss(Vp).item() - loss(Vm).item()) / (2 * h)
+        assert abs(g[ix] - fd) / abs(fd) < 1e-4, (ix, g[ix], fd)

@@ -97,6 +97,37 @@ def test_graph_replay_matches_eager_steps(rng):
         s.close()
 
 
+def test_replay_after_longer_eager_steps_matches_eager(rng):
+    # eager steps longer than the captured graph regrow the eager phase table; the graph
+    # keeps its own table, so a replay afterwards still runs the right source phases
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.batched import BatchedFDTD
+    print("case capture(4) step(8) replay()", flush=True)
+    cases = _cases(rng, "schrodinger", "f32", n=3) + _cases(rng, "klein_gordon", "f32", n=2)
+    eager = _make(cases)
+    eb = BatchedFDTD(eager)
+    graph = _make(cases)
+    gb = BatchedFDTD(graph)
+    gb.capture(4)
+    gb.step(8)
+    eb.step(8)
+    gb.replay()
+    eb.step(4)
+    gb.replay()
+    eb.step(4)
+    gb.synchronize()
+    eb.synchronize()
+    for e, g in zip(eager, graph):
+        a, c = e.get_state(np.complex64), g.get_state(np.complex64)
+        for u, v in zip(a if isinstance(a, tuple) else (a,), c if isinstance(c, tuple) else (c,)):
+            assert np.array_equal(u, v)
+    eb.close()
+    gb.close()
+    for s in eager + graph:
+        s.close()
+
+
 def test_batch_rejects_mixed_precision(rng):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")

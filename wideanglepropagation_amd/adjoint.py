@@ -7,8 +7,14 @@ and returns the demodulated exit wave
 
 (the notebooks' average of demodulated exit planes,
 ``FDTDmethod/fdtd_3D_baseline_comparison.ipynb:267-290``, as a linear
-functional of the states psi_s after s steps, t_s = t0 + s dt).  It is a
-``torch.autograd.Function``: the backward is the discrete adjoint of the
+functional of the states psi_s after s steps, t_s the solver's clock:
+t0 + s dt, or the reference's float32 time).  With ``n_frames`` the weights
+w_s are the recipe's window (equal weights on the last 40 % of the frame ends);
+with ``k_monitor`` E is divided by sqrt(mean |M|^2), M the same average of plane
+k_monitor, when that exceeds 1e-9 (the recipe's incident-intensity
+normalisation; differentiable, since M is another plane of the same forward).
+``ExitWave`` is a ``torch.autograd.Function`` of V returning such averages for a
+list of planes: the backward is the discrete adjoint of the
 same RK4 steps (hand-written HIP kernels, ``wap_adjoint_*`` in the C ABI),
 with checkpointing: the T steps are cut into segments of S >= ceil(sqrt(T))
 steps, counted back from step T.  The forward keeps the state at the start of
@@ -150,25 +156,31 @@ class ExitWave(torch.autograd.Function):
     def forward(ctx, V, opts):
         p = opts["params"]
         nsteps = int(opts["nsteps"])
-        k_exit = int(opts["k_exit"])
+        # k_exit: one plane (E is (ny, nx)) or a list of planes (E is (len, ny, nx))
+        planes = opts["k_exit"]
+        single = not isinstance(planes, (list, tuple))
+        planes = [int(planes)] if single else [int(k) for k in planes]
         if nsteps < 1:
             raise ValueError(f"exit_wave: nsteps must be >= 1 (got {nsteps})")
+        if not planes or any(k < 0 or k >= p.nz for k in planes):
+            raise ValueError(f"exit_wave: planes {planes} outside 0..{p.nz - 1}")
         samples = opts.get("samples") or {nsteps: 1.0}
         sim = FDTD3D(p, V=V.detach().contiguous(), source_profile=opts.get("source_profile"),
                      t0=opts.get("t0", 0.0))
         if opts.get("psi0") is not None:
             sim.set_state(opts["psi0"], opts.get("pi0"))
-        E = torch.zeros((p.ny, p.nx), dtype=torch.complex128, device="cuda")
-        tmp = torch.zeros_like(E)
+        E = torch.zeros((len(planes), p.ny, p.nx), dtype=torch.complex128, device="cuda")
+        tmp = torch.zeros_like(E[0])
         ckpt = {}
         keep = bool(ctx.needs_input_grad[0])   # no checkpoints when no gradient is wanted
         cdt = np.complex128 if p.precision == "f64" else np.complex64
 
         def sample(n):
             if n in samples:
-                tmp.zero_()
-                sim.accumulate_plane(k_exit, p.e_sim * (sim.t0 + n * p.dt), tmp)
-                E.add_(samples[n] * tmp)
+                for i, k in enumerate(planes):
+                    tmp.zero_()
+                    sim.accumulate_plane(k, p.e_sim * sim.time_of_step(n), tmp)
+                    E[i].add_(samples[n] * tmp)
 
         adj = None
         try:
@@ -202,18 +214,19 @@ class ExitWave(torch.autograd.Function):
                 adj.close()
             sim.close()
             raise
+        ctx.single = single
         if not keep:
             sim.close()
             ctx.sim = None
             ctx.ckpt = None
-            return E
+            return E[0] if single else E
         ctx.sim = sim
         ctx.adj = adj
         ctx.ckpt = ckpt
         ctx.starts = starts
-        ctx.opts = dict(opts, samples=samples)
+        ctx.opts = dict(opts, samples=samples, planes=planes)
         ctx.v_dtype = V.dtype
-        return E
+        return E[0] if single else E
 
     @staticmethod
     def backward(ctx, gE):
@@ -222,17 +235,18 @@ class ExitWave(torch.autograd.Function):
             raise RuntimeError("exit_wave: backward called twice")
         p = sim.params
         o = ctx.opts
-        nsteps, k_exit = int(o["nsteps"]), int(o["k_exit"])
+        nsteps, planes = int(o["nsteps"]), o["planes"]
         samples = o["samples"]
         starts = ctx.starts
-        g = gE.detach().to(torch.complex128).contiguous()
+        g = gE.detach().to(torch.complex128).reshape(len(planes), p.ny, p.nx).contiguous()
         adj = ctx.adj
         try:
             def seed(s):
                 if s in samples:
-                    t = sim.t0 + s * p.dt
+                    t = sim.time_of_step(s)
                     c = samples[s] * complex(math.cos(p.e_sim * t), math.sin(p.e_sim * t))
-                    adj.seed(k_exit, g, c)
+                    for i, k in enumerate(planes):
+                        adj.seed(k, g[i], c)
             seed(nsteps)
             ends = starts[1:] + [nsteps]
             for i in reversed(range(len(starts))):
@@ -263,13 +277,28 @@ FDTDAdjointFunction = ExitWave
 
 
 def exit_wave(V, params, nsteps, k_exit, samples=None, checkpoint_every=None, psi0=None,
-              pi0=None, source_profile=None, t0=0.0, memory_budget=None):
-    """Differentiable exit wave E(V) (complex128 (ny, nx) on the GPU).  memory_budget caps
-    the device bytes of the backward's records and checkpoints."""
-    opts = {"params": params, "nsteps": nsteps, "k_exit": k_exit, "samples": samples,
+              pi0=None, source_profile=None, t0=0.0, memory_budget=None, k_monitor=None,
+              n_frames=None, start_frac=0.6):
+    """Differentiable exit wave E(V) (complex128 (ny, nx) on the GPU).
+
+    samples: {step: weight} of the average; without it, n_frames gives the recipe's
+    window (default_samples(nsteps, n_frames, start_frac)), and without either E is the
+    demodulated plane after the last step.  k_monitor: divide by the incident amplitude
+    sqrt(mean |M|^2) of that plane's average when its intensity exceeds 1e-9.
+    memory_budget caps the device bytes of the backward's records and checkpoints."""
+    if samples is None and n_frames is not None:
+        samples = default_samples(int(nsteps), int(n_frames), start_frac)
+    planes = int(k_exit) if k_monitor is None else [int(k_exit), int(k_monitor)]
+    opts = {"params": params, "nsteps": nsteps, "k_exit": planes, "samples": samples,
             "checkpoint_every": checkpoint_every, "psi0": psi0, "pi0": pi0,
             "source_profile": source_profile, "t0": t0, "memory_budget": memory_budget}
-    return ExitWave.apply(V, opts)
+    E = ExitWave.apply(V, opts)
+    if k_monitor is None:
+        return E
+    inc = torch.mean(torch.abs(E[1]) ** 2)
+    if float(inc) > 1e-9:
+        return E[0] / torch.sqrt(inc)
+    return E[0]
 
 
 def far_field_intensity(E):

@@ -222,6 +222,15 @@ int build_tables(wap_handle *h) {
   return upload_src_tab<float>(h, mi_w);
 }
 
+// The float32 clock of fdtd_3d.cu:365 has no time origin: callers pass t0 = 0 there
+// (check_t0), so every consumer of step times (source, demodulation, adjoint seeds)
+// agrees on t_n.
+int check_t0(const wap_handle *h, double t0) {
+  if (h->d.time_mode == WAP_TIME_F32 && t0 != 0.0)
+    return fail(WAP_ERR_ARG, "t0 must be 0 with the float32 clock (WAP_TIME_F32)");
+  return WAP_OK;
+}
+
 double step_time(const wap_handle *h, int64_t step0, int64_t n, double t0) {
   if (h->d.time_mode == WAP_TIME_F32) {
     const float sf = (float)(step0 + n);
@@ -984,6 +993,7 @@ int wap_step(wap_handle *h, int64_t nsteps, int64_t step0, double t0) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
   if (nsteps < 0) return fail(WAP_ERR_ARG, "nsteps < 0");
+  if (int rc = check_t0(h, t0)) return rc;
   return dispatch_steps(h, nsteps, step0, t0, nullptr, nullptr);
 }
 
@@ -991,6 +1001,7 @@ int wap_step_profiled(wap_handle *h, int64_t nsteps, int64_t step0, double t0, d
                       double *total_ms) {
   if (!h || !pass_ms || !total_ms) return fail(WAP_ERR_ARG, "NULL argument");
   if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  if (int rc = check_t0(h, t0)) return rc;
   hipEvent_t ev[5], t_start, t_stop;
   for (int s = 0; s < 5; ++s) WAP_HIP(hipEventCreate(&ev[s]));
   WAP_HIP(hipEventCreate(&t_start));
@@ -1123,8 +1134,13 @@ struct wap_batch {
   bool own_stream = false;
   void *d_args[4] = {nullptr, nullptr, nullptr, nullptr};
   int *d_tile_begin = nullptr;
-  void *d_phase = nullptr;  // [steps][ninst] complex, e^{-i w_i t_n}
+  // [steps][ninst] complex e^{-i w_i t_n}: one table for eager steps (regrown on demand)
+  // and one the captured graph reads, sized at capture and never moved while the graph
+  // lives (the graph's kernel arguments hold its address)
+  void *d_phase = nullptr;
   int64_t phase_steps = 0;
+  void *d_phase_graph = nullptr;
+  int64_t phase_graph_steps = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
@@ -1140,11 +1156,13 @@ void batch_free(wap_batch *b) {
     if (p) { (void)hipFree(p); p = nullptr; }
   if (b->d_tile_begin) (void)hipFree(b->d_tile_begin);
   if (b->d_phase) (void)hipFree(b->d_phase);
+  if (b->d_phase_graph) (void)hipFree(b->d_phase_graph);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
   b->exec = nullptr;
   b->graph = nullptr;
   b->d_tile_begin = nullptr;
   b->d_phase = nullptr;
+  b->d_phase_graph = nullptr;
 }
 
 // batched launches read the per-step phasor from a device table instead of src_ph
@@ -1285,18 +1303,21 @@ int batch_refresh(wap_batch *b) {
   return WAP_OK;
 }
 
-// e^{-i w_i t_n} for n in [0, nsteps) into the device table (synchronous upload)
+// e^{-i w_i t_n} for n in [0, nsteps) into the device table *table of *cap steps, grown
+// when nsteps exceeds it (synchronous upload).  A table a live graph reads is never
+// passed here with more steps than it holds (the graph's table is sized at capture).
 template <typename R>
-int batch_phases(wap_batch *b, int64_t nsteps, int64_t step0) {
+int batch_phases(wap_batch *b, int64_t nsteps, int64_t step0, void **table, int64_t *cap) {
   const int n = (int)b->hs.size();
-  if (nsteps > b->phase_steps) {
-    if (b->d_phase) {
+  if (nsteps > *cap) {
+    if (*table) {
       WAP_HIP(hipStreamSynchronize(b->stream));
-      (void)hipFree(b->d_phase);
-      b->d_phase = nullptr;
+      (void)hipFree(*table);
+      *table = nullptr;
+      *cap = 0;
     }
-    WAP_HIP(hipMalloc(&b->d_phase, (size_t)nsteps * n * sizeof(wap::cplx<R>)));
-    b->phase_steps = nsteps;
+    WAP_HIP(hipMalloc(table, (size_t)nsteps * n * sizeof(wap::cplx<R>)));
+    *cap = nsteps;
   }
   std::vector<wap::cplx<R>> ph((size_t)nsteps * n);
   for (int64_t t = 0; t < nsteps; ++t)
@@ -1306,7 +1327,7 @@ int batch_phases(wap_batch *b, int64_t nsteps, int64_t step0) {
       ph[(size_t)t * n + i] = wap::cplx<R>{(R)z.real(), (R)z.imag()};
     }
   WAP_HIP(hipStreamSynchronize(b->stream));
-  WAP_HIP(hipMemcpy(b->d_phase, ph.data(), ph.size() * sizeof(wap::cplx<R>),
+  WAP_HIP(hipMemcpy(*table, ph.data(), ph.size() * sizeof(wap::cplx<R>),
                     hipMemcpyHostToDevice));
   return WAP_OK;
 }
@@ -1357,13 +1378,13 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
 }
 
 template <typename R, int EQ, bool K>
-int batch_launch(wap_batch *b, int64_t nsteps) {
+int batch_launch(wap_batch *b, int64_t nsteps, const void *phase) {
   using C = wap::cplx<R>;
   const int n = (int)b->hs.size();
   const dim3 block(wap::kWaveX, b->tile_y);
   const dim3 grid((unsigned)b->total_tiles);
   for (int64_t t = 0; t < nsteps; ++t) {
-    const C *ph = (const C *)b->d_phase + (size_t)t * n;
+    const C *ph = (const C *)phase + (size_t)t * n;
     if (batch_fused(b)) {
       batch_launch_fused<R>(b, t, ph);
     } else if (b->kg2) {
@@ -1388,21 +1409,21 @@ int batch_launch(wap_batch *b, int64_t nsteps) {
   return WAP_OK;
 }
 
-int batch_dispatch(wap_batch *b, int64_t nsteps) {
+int batch_dispatch(wap_batch *b, int64_t nsteps, const void *ph) {
   const bool kg = b->equation == WAP_EQ_KLEIN_GORDON;
   switch (b->precision) {
     case WAP_PREC_F64:
-      return kg ? batch_launch<double, 1, false>(b, nsteps) : batch_launch<double, 0, false>(b, nsteps);
+      return kg ? batch_launch<double, 1, false>(b, nsteps, ph) : batch_launch<double, 0, false>(b, nsteps, ph);
     case WAP_PREC_F32:
-      return kg ? batch_launch<float, 1, false>(b, nsteps) : batch_launch<float, 0, false>(b, nsteps);
+      return kg ? batch_launch<float, 1, false>(b, nsteps, ph) : batch_launch<float, 0, false>(b, nsteps, ph);
     default:
-      return kg ? batch_launch<float, 1, true>(b, nsteps) : batch_launch<float, 0, true>(b, nsteps);
+      return kg ? batch_launch<float, 1, true>(b, nsteps, ph) : batch_launch<float, 0, true>(b, nsteps, ph);
   }
 }
 
-int batch_phase_dispatch(wap_batch *b, int64_t nsteps, int64_t step0) {
-  return b->precision == WAP_PREC_F64 ? batch_phases<double>(b, nsteps, step0)
-                                      : batch_phases<float>(b, nsteps, step0);
+int batch_phase_dispatch(wap_batch *b, int64_t nsteps, int64_t step0, void **table, int64_t *cap) {
+  return b->precision == WAP_PREC_F64 ? batch_phases<double>(b, nsteps, step0, table, cap)
+                                      : batch_phases<float>(b, nsteps, step0, table, cap);
 }
 
 }  // namespace
@@ -1480,9 +1501,9 @@ int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0) {
   if (nsteps == 0) return WAP_OK;
   int rc = batch_refresh(b);
   if (rc) return rc;
-  rc = batch_phase_dispatch(b, nsteps, step0);
+  rc = batch_phase_dispatch(b, nsteps, step0, &b->d_phase, &b->phase_steps);
   if (rc) return rc;
-  rc = batch_dispatch(b, nsteps);
+  rc = batch_dispatch(b, nsteps, b->d_phase);
   if (!rc) batch_flip(b, nsteps);
   return rc;
 }
@@ -1490,6 +1511,9 @@ int wap_batch_step(wap_batch *b, int64_t nsteps, int64_t step0) {
 int wap_batch_capture(wap_batch *b, int64_t nsteps) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   if (nsteps < 1) return fail(WAP_ERR_ARG, "nsteps < 1");
+  // a replay of the previous graph may still be running: let it end before its graph
+  // and phase table go
+  if (b->exec) WAP_HIP(hipStreamSynchronize(b->stream));
   if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   if (!b->stream) {
@@ -1497,13 +1521,20 @@ int wap_batch_capture(wap_batch *b, int64_t nsteps) {
     WAP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->own_stream = true;
   }
-  // allocate the phase table before capture; its contents are written per replay
+  // the graph's own phase table, sized here (the previous graph, its only reader, is
+  // gone); its contents are written per replay
   int rc = batch_refresh(b);
   if (rc) return rc;
-  rc = batch_phase_dispatch(b, nsteps, 0);
+  if (b->phase_graph_steps != nsteps && b->d_phase_graph) {
+    WAP_HIP(hipStreamSynchronize(b->stream));
+    (void)hipFree(b->d_phase_graph);
+    b->d_phase_graph = nullptr;
+    b->phase_graph_steps = 0;
+  }
+  rc = batch_phase_dispatch(b, nsteps, 0, &b->d_phase_graph, &b->phase_graph_steps);
   if (rc) return rc;
   WAP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  rc = batch_dispatch(b, nsteps);
+  rc = batch_dispatch(b, nsteps, b->d_phase_graph);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc) {
@@ -1522,7 +1553,8 @@ int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
   if (!b->exec) return fail(WAP_ERR_STATE, "no captured graph (wap_batch_capture)");
   int rc = batch_refresh(b);
   if (rc) return rc;
-  rc = batch_phase_dispatch(b, b->graph_steps, step0);
+  // graph_steps == phase_graph_steps: the table is rewritten in place, never regrown
+  rc = batch_phase_dispatch(b, b->graph_steps, step0, &b->d_phase_graph, &b->phase_graph_steps);
   if (rc) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms) {
@@ -1570,7 +1602,9 @@ struct wap_adjoint {
   void *kba[2] = {nullptr, nullptr};
   void *kbb[2] = {nullptr, nullptr};
   double *vbar = nullptr;
-  void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type
+  void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type,
+                         // folded into vbar every kGradFlush reverse steps
+  int since_flush = 0;
   bool schr = false;     // fused Schrodinger (one state component; u3, v3, S in kba, kbb, acc)
   bool owns = true;      // false: the buffers are carved from a caller's workspace
   void *orig[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [comp]: the handle's
@@ -1823,6 +1857,31 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   return WAP_OK;
 }
 
+// The fused adjoints add each step's dL/dV into vacc (the handle's real type); every
+// kGradFlush steps vacc is added into the float64 vbar and cleared, so a float32
+// accumulator never sums more than that many steps (a run of 1500 reverse steps would
+// otherwise carry 1500 float32 roundings of a growing partial sum per voxel).
+constexpr int kGradFlush = 16;
+
+int adjoint_flush(wap_adjoint *a) {
+  if (!a->vacc) return WAP_OK;
+  const size_t n = a->h->n;
+  const dim3 grid(grid_1d(n)), block(256);
+  if (is_f64(a->h))
+    hipLaunchKernelGGL(wap::flush_grad<double>, grid, block, 0, a->h->stream,
+                       (double *)a->vacc, a->vbar, n);
+  else
+    hipLaunchKernelGGL(wap::flush_grad<float>, grid, block, 0, a->h->stream,
+                       (float *)a->vacc, a->vbar, n);
+  WAP_HIP(hipGetLastError());
+  a->since_flush = 0;
+  return WAP_OK;
+}
+
+int adjoint_count_step(wap_adjoint *a) {
+  return ++a->since_flush >= kGradFlush ? adjoint_flush(a) : WAP_OK;
+}
+
 // Reverse of one recorded Schrodinger 2-pass step: schr_adj_pass I then II.
 int adjoint_step_schr(wap_adjoint *ad, int slot) {
   using R = float;
@@ -1979,6 +2038,7 @@ int wap_adjoint_reset(wap_adjoint *a) {
     for (int c = 0; c < a->ncomp; ++c) WAP_HIP(hipMemsetAsync(a->lam(c), 0, fb, a->h->stream));
   WAP_HIP(hipMemsetAsync(a->vbar, 0, a->h->n * sizeof(double), a->h->stream));
   if (a->vacc) WAP_HIP(hipMemsetAsync(a->vacc, 0, a->h->n * a->h->rs, a->h->stream));
+  a->since_flush = 0;
   return WAP_OK;
 }
 
@@ -1986,6 +2046,7 @@ int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
   if (!a->h->v_set) return fail(WAP_ERR_STATE, "potential not set (wap_set_potential)");
+  if (int rc = check_t0(a->h, t0)) return rc;
   if (a->fused) return adjoint_record_fused(a, step, t0, slot);
   const bool kg = is_kg(a->h);
   if (is_f64(a->h))
@@ -1996,8 +2057,15 @@ int wap_adjoint_record(wap_adjoint *a, int64_t step, double t0, int slot) {
 int wap_adjoint_step(wap_adjoint *a, int slot) {
   if (!a) return fail(WAP_ERR_ARG, "adjoint is NULL");
   if (slot < 0 || slot >= a->slots) return fail(WAP_ERR_ARG, "slot out of range");
-  if (a->schr) return adjoint_step_schr(a, slot);
-  if (a->fused) return is_f64(a->h) ? adjoint_step_fused<double>(a, slot) : adjoint_step_fused<float>(a, slot);
+  if (a->schr) {
+    const int rc = adjoint_step_schr(a, slot);
+    return rc ? rc : adjoint_count_step(a);
+  }
+  if (a->fused) {
+    const int rc = is_f64(a->h) ? adjoint_step_fused<double>(a, slot)
+                                : adjoint_step_fused<float>(a, slot);
+    return rc ? rc : adjoint_count_step(a);
+  }
   const bool kg = is_kg(a->h);
   if (is_f64(a->h)) return kg ? adjoint_step_t<double, 1>(a, slot) : adjoint_step_t<double, 0>(a, slot);
   return kg ? adjoint_step_t<float, 1>(a, slot) : adjoint_step_t<float, 0>(a, slot);
@@ -2023,17 +2091,7 @@ int wap_adjoint_seed(wap_adjoint *a, int k, const void *g, double c_re, double c
 
 int wap_adjoint_get_grad(wap_adjoint *a, void *out, int on_device) {
   if (!a || !out) return fail(WAP_ERR_ARG, "NULL argument");
-  if (a->vacc) {  // the fused path accumulates in the handle's real type
-    const size_t n = a->h->n;
-    const dim3 grid(grid_1d(n)), block(256);
-    if (is_f64(a->h))
-      hipLaunchKernelGGL((wap::import_real<double, double>), grid, block, 0, a->h->stream,
-                         (const double *)a->vacc, a->vbar, n);
-    else
-      hipLaunchKernelGGL((wap::import_real<float, double>), grid, block, 0, a->h->stream,
-                         (const float *)a->vacc, a->vbar, n);
-    WAP_HIP(hipGetLastError());
-  }
+  if (int rc = adjoint_flush(a)) return rc;  // the fused path's unflushed steps
   WAP_HIP(hipMemcpyAsync(out, a->vbar, a->h->n * sizeof(double),
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->h->stream));
   WAP_HIP(hipStreamSynchronize(a->h->stream));

@@ -2269,6 +2269,18 @@ __global__ void import_real(const RI *in, RO *out, size_t n) {
   for (; t < n; t += stride) out[t] = (RO)in[t];
 }
 
+// vbar += (double) vacc; vacc = 0: the fused adjoints accumulate dL/dV in the handle's
+// real type for a few steps, then fold it into the float64 gradient
+template <typename R>
+__global__ void flush_grad(R *vacc, double *vbar, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) {
+    vbar[t] += (double)vacc[t];
+    vacc[t] = (R)0;
+  }
+}
+
 template <typename R>
 __global__ void accumulate_plane(const cplx<R> *psi, const cplx<R> *cmp, size_t off, int n, double c,
                                  double s, double2 *acc) {

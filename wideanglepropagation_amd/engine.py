@@ -57,6 +57,9 @@ class FDTD3D:
     """One device-resident grid of the RK4 FDTD stepper."""
 
     def __init__(self, params, V=None, psi0=None, pi0=None, source_profile=None, t0=0.0):
+        if params.time_mode == "f32" and float(t0) != 0.0:
+            # the reference's float32 clock (fdtd_3d.cu:365) has no time origin
+            raise ValueError("t0 must be 0 with time_mode='f32'")
         _require_gpu()
         self.params = params
         self._L = lib()

@@ -25,8 +25,12 @@ What is new (only through keyword arguments)
   dt, pml_theta, pml_power (the reference never forwards these),
   equation ("schrodinger" | "klein_gordon"), psi0 / pi0 (initial state),
   source (bool; the reference always injects), source_profile (ny, nx),
-  exit_plane (k: a GPU-accumulated demodulated average of plane k over the
-  saved frames, returned as "exit_wave"), backend ("hip" | "cli": run the
+  exit_plane (k: the notebooks' exit wave, ``fdtd_3D_baseline_comparison.ipynb:267-290``,
+  accumulated on the GPU: plane k demodulated with e^{+i E_sim t} and averaged over
+  the saved snapshots from int(exit_start_frac * M) on, default 0.6, i.e. the last
+  40 %; with monitor_plane=k_m it is divided by sqrt(mean |average of plane k_m|^2)
+  when that exceeds 1e-9; returned as "exit_wave", and "exit_incident"),
+  backend ("hip" | "cli": run the
   ``fdtd_3d_bin`` command through the file contract), return_timing,
   schedule ("auto" | "stages" | "schr_fused" | "kg_2pass" | "kg_fused": how a step
   is split into passes; "auto" fuses where the precision allows), tile_y, tile_z.
@@ -228,10 +232,19 @@ class RelativisticSolver:
             save_start = max(0, n_frames - M)
             saved = 0
             snaps = []
+            # the notebooks' exit-wave recipe (fdtd_3D_baseline_comparison.ipynb:267-290) on
+            # device accumulators: the demodulated exit plane (and monitor plane) averaged
+            # over the snapshots from int(start_frac * M) on, M the snapshots saved
             exit_k = kw.get("exit_plane")
-            acc = None
+            mon_k = kw.get("monitor_plane")
+            acc = mon = None
+            n_saved = min(M, n_frames)
+            win0 = int(float(kw.get("exit_start_frac", 0.6)) * n_saved)
+            n_acc = 0
             if exit_k is not None:
                 acc = torch.zeros((params.ny, params.nx), dtype=torch.complex128, device="cuda")
+                if mon_k is not None:
+                    mon = torch.zeros_like(acc)
             quant = params.time_mode == "f32"
             dt32 = np.float32(params.dt)
             e_sim32 = np.float32(params.e_sim)
@@ -247,8 +260,11 @@ class RelativisticSolver:
                         t_snap = np.float32(np.float32(frame * steps_per_frame) * dt32)
                     else:
                         t_snap = np.float32((frame + 1) * steps_per_frame * params.dt)
-                    if acc is not None:
+                    if acc is not None and saved >= win0:
                         sim.accumulate_plane(exit_k, float(e_sim32) * float(t_snap), acc)
+                        if mon is not None:
+                            sim.accumulate_plane(mon_k, float(e_sim32) * float(t_snap), mon)
+                        n_acc += 1
                     if not kw.get("no_snapshots", False):
                         st = sim.get_state(np.complex64)
                         if isinstance(st, tuple):
@@ -264,7 +280,15 @@ class RelativisticSolver:
             out["elapsed_s"] = t_run
             out["gvoxel_steps_per_s"] = vox_steps / t_run / 1e9 if t_run > 0 else float("nan")
             if acc is not None:
-                out["exit_wave"] = (acc / max(1, saved)).cpu().numpy()
+                ex = (acc / max(1, n_acc)).cpu().numpy()
+                if mon is not None:
+                    # normalised by the incident intensity at the monitor plane when
+                    # it exceeds 1e-9 (the notebook's rule)
+                    inc = float(np.mean(np.abs((mon / max(1, n_acc)).cpu().numpy()) ** 2))
+                    out["exit_incident"] = inc
+                    if inc > 1e-9:
+                        ex = ex / np.sqrt(inc)
+                out["exit_wave"] = ex
             if kw.get("return_state", False):
                 out["state"] = sim.get_state(np.complex128)
             print(f"Loaded {len(snaps)} snapshots.")
