@@ -10,6 +10,16 @@
 import numpy as np
 
 
+def demod_phase(e_sim, t):
+    """E t of the notebook's phasor (``fdtd_3D_baseline_comparison.ipynb:242, 277``): there
+    E_sim and the snapshot time are both numpy float32 (the file's types), so their
+    product is rounded to float32; float64 otherwise.  The phasor itself is formed in
+    float64 (NumPy 1.x promotes ``1j * float32 scalar`` to complex128)."""
+    if isinstance(t, np.float32):
+        return float(np.float32(e_sim) * t)
+    return float(e_sim) * float(t)
+
+
 def demodulated_average(snapshots, e_sim, k, start_frac=0.6):
     """Average over snapshots[int(start_frac*M):] of plane k of (phi+i psi) e^{i E t}."""
     m = len(snapshots)
@@ -18,7 +28,7 @@ def demodulated_average(snapshots, e_sim, k, start_frac=0.6):
     count = 0
     for t, phi, psi in snapshots[start:]:
         field = (np.asarray(phi[k], np.float64) + 1j * np.asarray(psi[k], np.float64))
-        field = field * np.exp(1j * e_sim * t)
+        field = field * np.exp(1j * demod_phase(e_sim, t))
         acc = field if acc is None else acc + field
         count += 1
     if acc is None:

@@ -10,7 +10,9 @@ S = ceil(sqrt(1500)) = 39 steps.
 dL/dV is checked against central differences of the FP64 forward at 32 voxels drawn
 uniformly (seed 4) from the planes the wave crosses between the source and the exit
 plane (the frozen end planes and the sponge carry no gradient); the step is
-h = 1e-3 max |V|; the error is |g - fd| / |fd| per voxel, with no floor.  The float64
+h = 1e-3 max |V|; the error is |g - fd| / |fd| per voxel, with no floor.  A central
+difference cannot resolve a gradient below 1e4 times its own rounding (4 eps |L| / 2h):
+such a voxel is drawn again, and the count is printed.  The float64
 adjoint must agree to 1e-4 at every voxel.  The float32 adjoint (complex64 fields,
 float32 V: the workload's precision) computes the gradient of the float32 forward;
 it is held to 1e-4 at the median voxel and 2e-3 at the worst one.
@@ -56,29 +58,38 @@ def test_adjoint_workload_against_central_differences():
     g = {}
     for name, p, Vx in (("f32", p32, V32), ("f64", p64, V32.double())):
         Vg = Vx.clone().requires_grad_(True)
-        loss(Vg, p).backward()
+        L0 = loss(Vg, p)
+        L0.backward()
         assert last_plan["segment"] == S and last_plan["recomputed_steps"] > 0
         g[name] = Vg.grad.double().cpu().numpy()
-        print(f"{name} gradient done, plan {dict(last_plan)}", flush=True)
+        print(f"{name} gradient done, L = {L0.item():.6e}, plan {dict(last_plan)}", flush=True)
     rng = np.random.default_rng(4)
     k_lo, k_hi = p32.src_k + 3, k_exit
-    idx = [(int(rng.integers(k_lo, k_hi + 1)), int(rng.integers(0, SHAPE[1])),
-            int(rng.integers(0, SHAPE[2]))) for _ in range(N_FD)]
     h = 1e-3 * float(V32.abs().max())
+    # a central difference resolves a gradient to 1e-4 only where |fd| exceeds 1e4 times
+    # its own float64 rounding, 4 eps |L| / 2h; a voxel below that is drawn again
+    resolvable = 1e4 * 4 * np.finfo(np.float64).eps * abs(L0.item()) / (2 * h)
     base = V32.double()
-    err64, err32 = [], []
-    for n, ix in enumerate(idx):
+    err64, err32, redrawn = [], [], 0
+    while len(err64) < N_FD:
+        ix = (int(rng.integers(k_lo, k_hi + 1)), int(rng.integers(0, SHAPE[1])),
+              int(rng.integers(0, SHAPE[2])))
         Vp, Vm = base.clone(), base.clone()
         Vp[ix] += h
         Vm[ix] -= h
         with torch.no_grad():
             fd = (loss(Vp, p64).item() - loss(Vm, p64).item()) / (2 * h)
+        if abs(fd) < resolvable:
+            redrawn += 1
+            assert redrawn <= N_FD, "most voxels below the difference's resolution"
+            continue
         err64.append(abs(g["f64"][ix] - fd) / abs(fd))
         err32.append(abs(g["f32"][ix] - fd) / abs(fd))
-        if (n + 1) % 8 == 0:
-            print(f"{n + 1}/{N_FD} differences: worst f64 {max(err64):.2e}, "
-                  f"f32 {max(err32):.2e}", flush=True)
+        if len(err64) % 8 == 0:
+            print(f"{len(err64)}/{N_FD} differences ({redrawn} redrawn): worst f64 "
+                  f"{max(err64):.2e}, f32 {max(err32):.2e}", flush=True)
     print(f"f64 adjoint vs FD: max {max(err64):.3e} median {np.median(err64):.3e}; "
-          f"f32: max {max(err32):.3e} median {np.median(err32):.3e}", flush=True)
+          f"f32: max {max(err32):.3e} median {np.median(err32):.3e}; "
+          f"{redrawn} voxels redrawn (|fd| < {resolvable:.2e})", flush=True)
     assert max(err64) < 1e-4, err64
     assert np.median(err32) < 1e-4 and max(err32) < 2e-3, err32

@@ -166,12 +166,14 @@ def test_exit_plane_accumulator_matches_notebook_recipe(tmp_path):
     rng = np.random.default_rng(7)
     V = (100 * rng.standard_normal((24, 16, 16))).astype(np.float32)
     solver = _solver(tmp_path)
+    # a random initial state: every plane carries signal from the first step
+    psi0 = rng.standard_normal((24, 16, 16)) + 1j * rng.standard_normal((24, 16, 16))
     res = solver.run(V, 0.1, (-1.0, 1.0, -1.0, 1.0, -1.5, 1.5), n_frames=10, steps_per_frame=4,
                      snapshot_count=8, pml_thick=0.3, k_source=4, exit_plane=15,
-                     monitor_plane=6, precision="f64")
+                     monitor_plane=6, precision="f64", psi0=psi0)
     assert len(res["snapshots"]) == 8
     want, inc = extract.exit_wave(res["snapshots"], float(res["E_sim"]), 15, 6, start_frac=0.6)
-    assert inc > 1e-9 and res["exit_incident"] == pytest.approx(inc, rel=1e-6)
+    assert inc > 0.1 and res["exit_incident"] == pytest.approx(inc, rel=1e-6)
     assert np.linalg.norm(res["exit_wave"] - want) <= 1e-6 * np.linalg.norm(want)
     # the window matters: the average over every snapshot differs
     every, _ = extract.exit_wave(res["snapshots"], float(res["E_sim"]), 15, 6, start_frac=0.0)
@@ -186,19 +188,20 @@ def test_exit_plane_recipe_against_the_fp64_oracle(tmp_path):
     rng = np.random.default_rng(8)
     shape, extent = (24, 16, 16), (-1.0, 1.0, -1.0, 1.0, -1.5, 1.5)
     V = (100 * rng.standard_normal(shape)).astype(np.float32)
+    psi0 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
     solver = _solver(tmp_path)
     res = solver.run(V, 0.1, extent, n_frames=10, steps_per_frame=4, snapshot_count=10,
                      pml_thick=0.3, k_source=4, exit_plane=15, monitor_plane=2,
-                     precision="f64", no_snapshots=True)
+                     precision="f64", no_snapshots=True, psi0=psi0)
     p = schrodinger_params(0.1, extent, shape, pml_thick=0.3, k_source=4,
                            reference_quantize=True)
     d = p.oracle_dict()
-    st = fdtd3d.zero_state(d)
+    st = psi0
     snaps = []
     for f in range(10):
         st = fdtd3d.run(st, V.astype(np.float64), 4, d, step0=4 * f, f32_time=True)
         t = np.float32(np.float32(4 * f) * np.float32(p.dt))
         snaps.append((t, st.real, st.imag))
     want, inc = extract.exit_wave(snaps, float(np.float32(p.e_sim)), 15, 2, start_frac=0.6)
-    assert res["exit_incident"] == pytest.approx(inc, rel=1e-10)
+    assert inc > 0.1 and res["exit_incident"] == pytest.approx(inc, rel=1e-10)
     assert np.linalg.norm(res["exit_wave"] - want) <= 1e-10 * np.linalg.norm(want)

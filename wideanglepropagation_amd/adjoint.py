@@ -296,7 +296,7 @@ def exit_wave(V, params, nsteps, k_exit, samples=None, checkpoint_every=None, ps
     if k_monitor is None:
         return E
     inc = torch.mean(torch.abs(E[1]) ** 2)
-    if float(inc) > 1e-9:
+    if float(inc.detach()) > 1e-9:
         return E[0] / torch.sqrt(inc)
     return E[0]
 

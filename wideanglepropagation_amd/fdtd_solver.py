@@ -261,9 +261,12 @@ class RelativisticSolver:
                     else:
                         t_snap = np.float32((frame + 1) * steps_per_frame * params.dt)
                     if acc is not None and saved >= win0:
-                        sim.accumulate_plane(exit_k, float(e_sim32) * float(t_snap), acc)
+                        # E_sim t as the notebook forms it from the snapshot's float32
+                        # values: a float32 product (fdtd_3D_baseline_comparison.ipynb:242, 277)
+                        ph = float(np.float32(e_sim32 * np.float32(t_snap)))
+                        sim.accumulate_plane(exit_k, ph, acc)
                         if mon is not None:
-                            sim.accumulate_plane(mon_k, float(e_sim32) * float(t_snap), mon)
+                            sim.accumulate_plane(mon_k, ph, mon)
                         n_acc += 1
                     if not kw.get("no_snapshots", False):
                         st = sim.get_state(np.complex64)
