@@ -258,6 +258,19 @@ int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
                        const double *extent, double r_soft, double cutoff, void *out, int dtype,
                        void *stream);
 
+/* As wap_atom_potential, allocating nothing: the atom records are staged in a static
+ * device array of the library (WAP_STATIC_MAX_ATOMS records, 160 KiB per GPU, held until
+ * the process ends) instead of a per-call device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code that a caller captures
+ * into a HIP graph, tight serving loops, harnesses that forbid hipMalloc).  n_atoms above
+ * WAP_STATIC_MAX_ATOMS is rejected (WAP_ERR_ARG).  Every call on one GPU shares that GPU's
+ * staging array: calls that overlap on one GPU (other threads, other streams) race;
+ * calls from one thread are ordered, since each synchronises `stream` before it returns. */
+#define WAP_STATIC_MAX_ATOMS 4096
+int wap_atom_potential_static(const double *atoms, int n_atoms, int nx, int ny, int nz,
+                              const double *extent, double r_soft, double cutoff, void *out,
+                              int dtype, void *stream);
+
 /* Phase grating of one potential slice of a spectral multislice: psi *= exp(i 2 pi
  * (n(v) - 1) dz / lambda), n the relativistic electron refractive index at the
  * electrostatic potential v [V] for kinetic energy energy_ev (rest energy

@@ -86,6 +86,8 @@ SIGNATURES = {
                                     _c_dbl, _vp]),
     "wap_atom_potential": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_dbl, _c_dbl, _vp,
                                     _c_int, _vp]),
+    "wap_atom_potential_static": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_dbl,
+                                           _c_dbl, _vp, _c_int, _vp]),
 }
 
 _LIB = None

@@ -12,6 +12,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -2148,23 +2149,30 @@ extern "C" int wap_stream_bandwidth(int64_t bytes, int reps, double *copy_gbs, d
   return rc;
 }
 
-extern "C" int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
-                                  const double *extent, double r_soft, double cutoff, void *out,
-                                  int dtype, void *stream) {
+namespace {
+
+int atom_args_ok(const double *atoms, int n_atoms, int nx, int ny, int nz, const double *extent,
+                 double cutoff, void *out, int dtype) {
   if (!atoms || n_atoms < 0 || !extent || !out) return fail(WAP_ERR_ARG, "NULL argument");
   if (nx < 2 || ny < 2 || nz < 2) return fail(WAP_ERR_ARG, "grid must be at least 2 x 2 x 2");
   if (dtype != WAP_DT_F32 && dtype != WAP_DT_F64)
     return fail(WAP_ERR_ARG, "V dtype must be WAP_DT_F32 or WAP_DT_F64");
   if (!(cutoff > 0.0)) return fail(WAP_ERR_ARG, "cutoff must be > 0");
+  return WAP_OK;
+}
+
+// Copy the host atom records into the device staging d_atoms, run the atom-sum kernel on
+// `st` and wait for it (the caller may reuse d_atoms afterwards).
+hipError_t atom_potential_run(double *d_atoms, const double *atoms, int n_atoms, int nx, int ny,
+                              int nz, const double *extent, double r_soft, double cutoff,
+                              void *out, int dtype, hipStream_t st) {
   const double dx = (extent[1] - extent[0]) / (nx - 1.0);
   const double dy = (extent[3] - extent[2]) / (ny - 1.0);
   const double dz = (extent[5] - extent[4]) / (nz - 1.0);
-  double *d_atoms = nullptr;
-  const size_t ab = (size_t)(n_atoms > 0 ? n_atoms : 1) * 5 * sizeof(double);
-  WAP_HIP(hipMalloc((void **)&d_atoms, ab));
-  hipStream_t st = (hipStream_t)stream;
-  hipError_t e = hipMemcpyAsync(d_atoms, atoms, (size_t)n_atoms * 5 * sizeof(double),
-                                hipMemcpyHostToDevice, st);
+  hipError_t e = hipSuccess;
+  if (n_atoms > 0)
+    e = hipMemcpyAsync(d_atoms, atoms, (size_t)n_atoms * 5 * sizeof(double),
+                       hipMemcpyHostToDevice, st);
   const size_t total = (size_t)nx * ny * nz;
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
   const double ke2 = 14.399645;  // e^2 / (4 pi eps0) in eV A
@@ -2180,8 +2188,72 @@ extern "C" int wap_atom_potential(const double *atoms, int n_atoms, int nx, int 
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
+
+}  // namespace
+
+extern "C" int wap_atom_potential(const double *atoms, int n_atoms, int nx, int ny, int nz,
+                                  const double *extent, double r_soft, double cutoff, void *out,
+                                  int dtype, void *stream) {
+  if (int rc = atom_args_ok(atoms, n_atoms, nx, ny, nz, extent, cutoff, out, dtype)) return rc;
+  double *d_atoms = nullptr;
+  const size_t ab = (size_t)(n_atoms > 0 ? n_atoms : 1) * 5 * sizeof(double);
+  WAP_HIP(hipMalloc((void **)&d_atoms, ab));
+  const hipError_t e = atom_potential_run(d_atoms, atoms, n_atoms, nx, ny, nz, extent, r_soft,
+                                          cutoff, out, dtype, (hipStream_t)stream);
   (void)hipFree(d_atoms);
   if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("wap_atom_potential: ") + hipGetErrorString(e));
+  return WAP_OK;
+}
+
+// ------------------------------------------- allocation-free atom potential
+// The atom records of wap_atom_potential_static are staged in this file-scope device
+// array instead of a per-call hipMalloc: each GPU holds one copy of it (160 KiB) until
+// the process ends.  Host code never uses the array's name as an address (on the host it
+// is only a stand-in for the symbol): the device address comes from hipGetSymbolAddress,
+// once per device, and is kept in g_scratch_dev.
+constexpr int kStaticAtoms = 4096;  // WAP_STATIC_MAX_ATOMS
+static_assert(kStaticAtoms == WAP_STATIC_MAX_ATOMS, "header and library disagree");
+static __device__ __align__(256) double scratch[kStaticAtoms * 5];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch` (looked up on the first call on that device).
+int scratch_address(double **out) {
+  int dev = -1;
+  WAP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(WAP_ERR_ARG, "device index beyond the static scratch table (64 devices)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    WAP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(WAP_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return WAP_OK;
+}
+}  // namespace
+
+extern "C" int wap_atom_potential_static(const double *atoms, int n_atoms, int nx, int ny,
+                                         int nz, const double *extent, double r_soft,
+                                         double cutoff, void *out, int dtype, void *stream) {
+  if (int rc = atom_args_ok(atoms, n_atoms, nx, ny, nz, extent, cutoff, out, dtype)) return rc;
+  if (n_atoms > kStaticAtoms)
+    return fail(WAP_ERR_ARG, "wap_atom_potential_static: " + std::to_string(n_atoms) +
+                                 " atoms exceed the static staging of " +
+                                 std::to_string(kStaticAtoms) + " (use wap_atom_potential)");
+  double *d_atoms = nullptr;
+  if (int rc = scratch_address(&d_atoms)) return rc;
+  const hipError_t e = atom_potential_run(d_atoms, atoms, n_atoms, nx, ny, nz, extent, r_soft,
+                                          cutoff, out, dtype, (hipStream_t)stream);
+  if (e != hipSuccess)
+    return fail(WAP_ERR_HIP, std::string("wap_atom_potential_static: ") + hipGetErrorString(e));
   return WAP_OK;
 }
 
