@@ -59,6 +59,8 @@ PASS_BYTES = {
     ("kg_fused", "klein_gordon", "f32_kahan"): (68,),   # + the compensation, read and written
     # Schrodinger RK4 polynomial in Horner form: 2 passes of 2 stencil levels
     ("schr_fused", "schrodinger", "f32"): (20, 28),
+    ("schr_fused", "schrodinger", "f64"): (40, 56),
+    ("schr_fused", "schrodinger", "f32_kahan"): (20, 44),  # pass B + the compensation, R/W
 }
 
 

@@ -206,6 +206,24 @@ int wap_batch_replay(wap_batch *b, int64_t step0, double *ms);
 int wap_batch_synchronize(wap_batch *b);
 
 /*
+ * Batch sets: one hipGraph over several batches, e.g. one per equation of a mixed
+ * workload (a batch holds one equation, precision and schedule).  Inside the graph each
+ * batch's steps run on a stream of their own, forked from and joined into the set's
+ * launch stream, so independent groups overlap on the device.  The set keeps its own
+ * per-step phase tables; the batches and their handles must outlive it, and no other
+ * work may step the same handles while a replay runs.
+ */
+typedef struct wap_batch_set wap_batch_set;
+int wap_batch_set_create(wap_batch *const *batches, int n, wap_batch_set **out);
+int wap_batch_set_destroy(wap_batch_set *s);
+/* Record nsteps steps of every batch into the set's graph. */
+int wap_batch_set_capture(wap_batch_set *s, int64_t nsteps);
+/* Run the graph for steps step0 .. step0+nsteps-1 of every batch; ms != NULL returns its
+ * device time (HIP events on the launch stream) and synchronises. */
+int wap_batch_set_replay(wap_batch_set *s, int64_t step0, double *ms);
+int wap_batch_set_synchronize(wap_batch_set *s);
+
+/*
  * Discrete adjoint (no reference counterpart: the reference has no gradient).
  * Gradient of a real loss of exit-plane values with respect to V, through the
  * RK4 steps, the PML / sponge and the soft source.  Use: run the forward with

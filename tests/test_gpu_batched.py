@@ -205,16 +205,22 @@ def test_tune_batch_picks_a_chunk_per_class():
         mk = schrodinger_params if i % 2 else klein_gordon_params
         ps.append(mk(0.025, centred_extent(shape, 0.05), shape, pml_thick=0.15, k_source=4,
                      precision="f32"))
-    choice, ms, table = autotune.tune_batch(ps, tile_zs=(8, 16), steps=2, warmup=1)
+    choice, ms, table = autotune.tune_batch(ps, tile_zs=(8, 16), tile_ys=(8, 16), steps=2,
+                                            warmup=1)
     assert set(choice) == {autotune._class_of(p) for p in ps}
-    assert set(choice.values()) <= {8, 16}
+    assert {tz for _, tz in choice.values()} <= {8, 16}
+    assert {ty for ty, _ in choice.values()} <= {8, 16}
+    # the grids of one launch group share its waves
+    for c in choice:
+        assert all(choice[d][0] == choice[c][0] for d in choice if d[:3] == c[:3])
     assert ms == min(m for _, m in table) and ms > 0
-    # 2 uniform choices, then one alternative per class
-    assert len(table) == 2 + len(choice)
+    # 2 uniform chunks, one alternative waves per group (2), one alternative chunk per class
+    assert len(table) == 2 + 2 + len(choice)
 
 
 def test_batched_study_quick_runs_finite():
     from wideanglepropagation_amd.studies import batched_study
     r = batched_study(quick=True)
     assert r["all_finite"] and r["gvoxel_steps_per_s"] > 0
-    assert r["autotuned"] and all(v["tile_z"] in (16, 32, 64) for v in r["autotuned"].values())
+    assert r["autotuned"] and all(v["tile_z"] in (16, 32, 64) and v["waves"] in (8, 16)
+                                  for v in r["autotuned"].values())

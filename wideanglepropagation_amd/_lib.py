@@ -72,6 +72,11 @@ SIGNATURES = {
     "wap_batch_capture": (_c_int, [_vp, _c_i64]),
     "wap_batch_replay": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_dbl)]),
     "wap_batch_synchronize": (_c_int, [_vp]),
+    "wap_batch_set_create": (_c_int, [ctypes.POINTER(_vp), _c_int, ctypes.POINTER(_vp)]),
+    "wap_batch_set_destroy": (_c_int, [_vp]),
+    "wap_batch_set_capture": (_c_int, [_vp, _c_i64]),
+    "wap_batch_set_replay": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_dbl)]),
+    "wap_batch_set_synchronize": (_c_int, [_vp]),
     "wap_adjoint_create": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp)]),
     "wap_adjoint_workspace_bytes": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_i64)]),
     "wap_adjoint_create_in": (_c_int, [_vp, _c_int, _vp, _c_i64, ctypes.POINTER(_vp)]),

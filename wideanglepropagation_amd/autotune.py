@@ -10,12 +10,13 @@ size and returns the fastest; results are cached per (equation, precision,
 schedule, nz, ny, nx) in a JSON file (``$WAP_AUTOTUNE_CACHE`` or
 ``~/.cache/wideanglepropagation_amd/autotune.json``).
 
-``tune_batch`` tunes the z chunk per size class for grids that run together in
-one ``BatchedFDTD``: there the batch fills the device whatever the chunk, so a
-single grid's best chunk (short, to make enough workgroups on its own) is the
-wrong one (48 uneven grids: 82 Gvoxel-steps/s with single-grid choices against
-105 with 32-plane chunks).  It times the whole batch, first with one chunk for
-every class, then class by class.
+``tune_batch`` tunes the z chunk per size class, and the workgroup waves per
+launch group, for grids that run together in one ``BatchedFDTD``: there the
+batch fills the device whatever the chunk, so a single grid's best chunk (short,
+to make enough workgroups on its own) is the wrong one (48 uneven grids: 82
+Gvoxel-steps/s with single-grid choices against 105 with 32-plane chunks).  It
+times the whole batch as it runs (its captured graph), first with one chunk for
+every class, then the waves of each group, then class by class.
 """
 import json
 import os
@@ -108,7 +109,9 @@ def tuned(params, **kw):
 
 
 def measure_batch(param_list, steps=4, warmup=2):
-    """Device ms per step of one BatchedFDTD over zero-potential grids of param_list."""
+    """Device ms per step of one BatchedFDTD over zero-potential grids of param_list, run
+    the way a batch runs: its steps captured into one hipGraph (groups on forked streams),
+    timed over a replay after a warm-up replay."""
     from .batched import BatchedFDTD
     sims = []
     try:
@@ -117,15 +120,11 @@ def measure_batch(param_list, steps=4, warmup=2):
             sims.append(FDTD3D(p, V=torch.zeros(p.shape, dtype=dt, device="cuda")))
         b = BatchedFDTD(sims)
         try:
-            b.step(warmup)
+            b.capture(steps)
+            for _ in range(max(1, -(-warmup // steps))):
+                b.replay()
             b.synchronize()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(b.stream)
-            b.step(steps)
-            e1.record(b.stream)
-            e1.synchronize()
-            return e0.elapsed_time(e1) / steps
+            return b.replay(timed=True) / steps
         finally:
             b.close()
     finally:
@@ -137,35 +136,54 @@ def _class_of(p):
     return (p.equation, p.precision, p.resolved_schedule(), p.shape)
 
 
-def tune_batch(param_list, tile_zs=(16, 32, 64), tile_y=16, steps=4, warmup=2, verbose=False):
-    """z chunk per size class for grids batched together.  Returns ({class: tile_z},
-    ms per batched step, table of (choice, ms)).  A class is (equation, precision,
-    schedule, shape)."""
-    classes = sorted({_class_of(p) for p in param_list}, key=str)
+def _group_of(p):
+    return (p.equation, p.precision, p.resolved_schedule())
 
-    def run(choice):
-        ps = [p.with_(tile_y=tile_y, tile_z=choice[_class_of(p)]) for p in param_list]
+
+def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, steps=4, warmup=2,
+               verbose=False):
+    """Launch shape for grids batched together: the z chunk per size class and the
+    workgroup waves per launch group (the grids of one batched launch share them).
+    Coordinate descent on the timed batch: one chunk for every class, then the waves of
+    each group, then each class's chunk with the others fixed.  Returns
+    ({class: (tile_y, tile_z)}, ms per batched step, table of (choice, ms)).  A class is
+    (equation, precision, schedule, shape); a group (equation, precision, schedule)."""
+    classes = sorted({_class_of(p) for p in param_list}, key=str)
+    groups = sorted({_group_of(p) for p in param_list}, key=str)
+
+    def label(choice, waves):
+        return {"/".join((c[0], "x".join(map(str, c[3])))): (waves[c[:3]], tz)
+                for c, tz in choice.items()}
+
+    def run(choice, waves):
+        ps = [p.with_(tile_y=waves[_group_of(p)], tile_z=choice[_class_of(p)])
+              for p in param_list]
         ms = measure_batch(ps, steps, warmup)
-        table.append(({"/".join(map(str, c[:1] + ("x".join(map(str, c[3])),))): tz
-                       for c, tz in choice.items()}, ms))
+        table.append((label(choice, waves), ms))
         if verbose:
             print(f"[autotune] batch {table[-1][0]}: {ms:.3f} ms/step", flush=True)
         return ms
 
     table = []
+    waves = {g: tile_y for g in groups}
     best, best_ms = None, float("inf")
     for tz in tile_zs:   # one chunk for every class
         choice = {c: tz for c in classes}
-        ms = run(choice)
+        ms = run(choice, waves)
         if ms < best_ms:
             best, best_ms = choice, ms
+    for g in groups:     # the waves of each launch group
+        for ty in [t for t in tile_ys if t != waves[g]]:
+            trial = dict(waves)
+            trial[g] = ty
+            ms = run(best, trial)
+            if ms < best_ms:
+                waves, best_ms = trial, ms
     for c in classes:    # then class by class
-        for tz in tile_zs:
-            if tz == best[c]:
-                continue
+        for tz in [t for t in tile_zs if t != best[c]]:
             choice = dict(best)
             choice[c] = tz
-            ms = run(choice)
+            ms = run(choice, waves)
             if ms < best_ms:
                 best, best_ms = choice, ms
-    return best, best_ms, table
+    return {c: (waves[c[:3]], tz) for c, tz in best.items()}, best_ms, table

@@ -6,8 +6,9 @@ probe positions, beam energies or wave packets is a set of independent runs.
 and, per group, issues one kernel launch per pass whose workgroups are the
 union of every grid's column tiles (equal-sized work units, so uneven grids
 balance by construction).  ``capture(n)`` records n steps of every group into
-a single hipGraph per group; ``replay(step0)`` rewrites the per-step source
-phasors and launches it.
+one hipGraph (several groups, e.g. the two equations of a mixed workload, run on
+forked streams inside it, so their launches overlap); ``replay()`` rewrites the
+per-step source phasors and launches it.
 """
 import ctypes
 
@@ -40,6 +41,7 @@ class BatchedFDTD:
             self.groups.append((key, members, b))
         self.step_index = 0
         self.captured_steps = 0
+        self._set = None   # the graph over all groups (wap_batch_set) when there are several
 
     @property
     def voxels(self):
@@ -61,9 +63,17 @@ class BatchedFDTD:
             s.step_index = self.step_index
 
     def capture(self, nsteps):
-        """Record nsteps steps of every group as hipGraphs (one per group)."""
-        for _, _, b in self.groups:
-            check(self._L.wap_batch_capture(b, int(nsteps)), "wap_batch_capture")
+        """Record nsteps steps of every group into one hipGraph."""
+        if len(self.groups) == 1:
+            check(self._L.wap_batch_capture(self.groups[0][2], int(nsteps)), "wap_batch_capture")
+        else:
+            if self._set is None:
+                arr = (ctypes.c_void_p * len(self.groups))(*[b.value for _, _, b in self.groups])
+                h = ctypes.c_void_p()
+                check(self._L.wap_batch_set_create(arr, len(self.groups), ctypes.byref(h)),
+                      "wap_batch_set_create")
+                self._set = h
+            check(self._L.wap_batch_set_capture(self._set, int(nsteps)), "wap_batch_set_capture")
         self.captured_steps = int(nsteps)
 
     def replay(self, timed=False):
@@ -71,10 +81,11 @@ class BatchedFDTD:
         if not self.captured_steps:
             raise RuntimeError("capture() first")
         total = 0.0
-        for _, _, b in self.groups:
+        targets = ([(self._L.wap_batch_set_replay, self._set)] if self._set is not None else
+                   [(self._L.wap_batch_replay, b) for _, _, b in self.groups])
+        for fn, h in targets:
             ms = ctypes.c_double()
-            check(self._L.wap_batch_replay(b, int(self.step_index),
-                                           ctypes.byref(ms) if timed else None), "replay")
+            check(fn(h, int(self.step_index), ctypes.byref(ms) if timed else None), "replay")
             total += ms.value
         self.step_index += self.captured_steps
         for s in self.sims:
@@ -82,10 +93,15 @@ class BatchedFDTD:
         return total if timed else None
 
     def synchronize(self):
+        if self._set is not None:
+            check(self._L.wap_batch_set_synchronize(self._set), "wap_batch_set_synchronize")
         for _, _, b in self.groups:
             check(self._L.wap_batch_synchronize(b), "wap_batch_synchronize")
 
     def close(self):
+        if self._set is not None:
+            self._L.wap_batch_set_destroy(self._set)
+            self._set = None
         for _, _, b in self.groups:
             if b:
                 self._L.wap_batch_destroy(b)

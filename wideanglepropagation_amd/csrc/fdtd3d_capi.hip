@@ -7,6 +7,7 @@
 // voxel on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <complex>
@@ -378,11 +379,12 @@ void split_hi_lo(double x, R &hi, R &lo) {
   lo = (R)(x - (double)hi);
 }
 
-// the Schrodinger fused schedule: f32, frozen end planes (the 2D rule zeroes the stage
-// inputs on the end rows, which the RK4 polynomial form does not express), fields < 2 GiB
+// the Schrodinger fused schedule (every precision): frozen end planes (the 2D rule zeroes
+// the stage inputs on the end rows, which the RK4 polynomial form does not express),
+// fields < 2 GiB
 bool schr_fused_ok(const wap_desc &d) {
-  return d.equation == WAP_EQ_SCHRODINGER && d.precision == WAP_PREC_F32 &&
-         d.end_planes == WAP_ENDS_FROZEN && fused_plane_fits(d);
+  return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
+         fused_plane_fits(d);
 }
 
 int sched_of(const wap_handle *h) {
@@ -402,12 +404,12 @@ bool kg_two_pass(const wap_handle *h) {
 }
 
 // Fused-pass workgroups: tile_y = 4, 8 or 16 waves (default 16), i.e. rows = waves - 1
-// rows of 64 columns (two of them halo rows) plus the side-column wave.  float64 takes at
-// most 8 (its fused pass needs ~180 VGPRs: 2 waves per SIMD).
+// rows of 64 columns (two of them halo rows) plus the side-column wave.  The float64 KG
+// pass takes at most 8 (it needs ~180 VGPRs: 2 waves per SIMD).
 int kg1_waves(const wap_handle *h) {
   const int ty = h->d.tile_y;
   const int w = (ty == 4 || ty == 8 || ty == 16) ? ty : 16;
-  return (is_f64(h) && w > 8) ? 8 : w;
+  return (is_f64(h) && is_kg(h) && w > 8) ? 8 : w;
 }
 int kg1_ntx(const wap_handle *h) { return (h->d.nx + wap::kWaveX - 1) / wap::kWaveX; }
 int kg1_nty(const wap_handle *h, int waves) {
@@ -520,8 +522,8 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 }
 
 // the launch-invariant arguments of schr2_pass; pass 0 (A) or 1 (B) sets the level factors
-wap::Schr2Args<float> schr2_args(const wap_handle *h, int waves, int pass) {
-  using R = float;
+template <typename R>
+wap::Schr2Args<R> schr2_args(const wap_handle *h, int waves, int pass) {
   using C = wap::cplx<R>;
   wap::Schr2Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
@@ -537,25 +539,28 @@ wap::Schr2Args<float> schr2_args(const wap_handle *h, int waves, int pass) {
   a.c1 = (R)(pass == 0 ? dt / 4.0 : dt / 2.0);
   a.c2 = (R)(pass == 0 ? dt / 3.0 : dt);
   a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
+  a.cmp = (pass == 1 && is_kahan(h)) ? (C *)h->cmp_psi : nullptr;
   return a;
 }
 
-// Schrodinger f32 step in two fused passes (schr2_pass): A psi -> u3 (ya), B (u3, psi)
-// -> psi' (yb), then the state buffer and yb swap.
+// Schrodinger step in two fused passes (schr2_pass): A psi -> u3 (ya), B (u3, psi) ->
+// psi' (yb), then the state buffer and yb swap; K: f32_kahan (pass B also updates the
+// compensation in place)
+template <typename R, bool K>
 int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                     double *pass_ms) {
-  using R = float;
   using C = wap::cplx<R>;
   const int waves = kg1_waves(h);
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)fused_tiles(h, waves));
-  wap::Schr2Args<R> pa = schr2_args(h, waves, 0), pb = schr2_args(h, waves, 1);
+  wap::Schr2Args<R> pa = schr2_args<R>(h, waves, 0), pb = schr2_args<R>(h, waves, 1);
   auto launch = [&](auto pass, const wap::Schr2Args<R> &a) {
     constexpr int PB = decltype(pass)::value;
+    constexpr bool KB = K && PB == 1;
     switch (waves) {
-      case 4: hipLaunchKernelGGL((wap::schr2_pass<R, 3, PB>), grid, block, 0, h->stream, a); break;
-      case 8: hipLaunchKernelGGL((wap::schr2_pass<R, 7, PB>), grid, block, 0, h->stream, a); break;
-      default: hipLaunchKernelGGL((wap::schr2_pass<R, 15, PB>), grid, block, 0, h->stream, a); break;
+      case 4: hipLaunchKernelGGL((wap::schr2_pass<R, 3, PB, KB>), grid, block, 0, h->stream, a); break;
+      case 8: hipLaunchKernelGGL((wap::schr2_pass<R, 7, PB, KB>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL((wap::schr2_pass<R, 15, PB, KB>), grid, block, 0, h->stream, a); break;
     }
   };
   for (int64_t n = 0; n < nsteps; ++n) {
@@ -655,7 +660,13 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
-  if (sched_of(h) == WAP_SCHED_SCHR_FUSED) return run_steps_schr2(h, nsteps, step0, t0, ev, pass_ms);
+  if (sched_of(h) == WAP_SCHED_SCHR_FUSED) {
+    switch (h->d.precision) {
+      case WAP_PREC_F64: return run_steps_schr2<double, false>(h, nsteps, step0, t0, ev, pass_ms);
+      case WAP_PREC_F32: return run_steps_schr2<float, false>(h, nsteps, step0, t0, ev, pass_ms);
+      default: return run_steps_schr2<float, true>(h, nsteps, step0, t0, ev, pass_ms);
+    }
+  }
   if (sched_of(h) == WAP_SCHED_KG_FUSED) {
     if (h->d.precision == WAP_PREC_F64) return run_steps_kg1<double>(h, nsteps, step0, t0, ev, pass_ms);
     return run_steps_kg1<float>(h, nsteps, step0, t0, ev, pass_ms);
@@ -735,7 +746,7 @@ int validate(const wap_desc *d) {
   if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_SCHR_FUSED)
     return fail(WAP_ERR_ARG, "unknown schedule");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
-    return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, f32, "
+    return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
                              "frozen end planes and fields under 2 GiB");
   if ((d->schedule == WAP_SCHED_KG_2PASS || d->schedule == WAP_SCHED_KG_FUSED) &&
       d->equation != WAP_EQ_KLEIN_GORDON)
@@ -1202,13 +1213,13 @@ int batch_upload_fused(wap_batch *b) {
         a[i].stream_out = stream;
       }
       WAP_HIP(hipMemcpy(b->d_args[par], a.data(), n * sizeof(a[0]), hipMemcpyHostToDevice));
-    } else if constexpr (std::is_same<R, float>::value) {
-      std::vector<wap::Schr2Args<float>> pa(n), pb(n);
+    } else {
+      std::vector<wap::Schr2Args<R>> pa(n), pb(n);
       for (int i = 0; i < n; ++i) {
         const wap_handle *h = b->hs[i];
         const C *cur = (const C *)(par ? h->yb_psi : h->psi);
-        pa[i] = schr2_args(h, b->waves, 0);
-        pb[i] = schr2_args(h, b->waves, 1);
+        pa[i] = schr2_args<R>(h, b->waves, 0);
+        pb[i] = schr2_args<R>(h, b->waves, 1);
         timeless_source(h, pa[i]);
         timeless_source(h, pb[i]);
         pa[i].in = cur;
@@ -1289,7 +1300,7 @@ int batch_build(wap_batch *b) {
     one = sizeof(wap::KG1Args<R>);
     nargs = 2;
   } else if (b->sched == WAP_SCHED_SCHR_FUSED) {
-    one = sizeof(wap::Schr2Args<float>);
+    one = sizeof(wap::Schr2Args<R>);
     nargs = 4;
   }
   for (int s = 0; s < nargs; ++s) WAP_HIP(hipMalloc(&b->d_args[s], n * one));
@@ -1358,24 +1369,28 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
     }
     return;
   }
-  if constexpr (std::is_same<R, float>::value) {
-    const wap::Schr2Args<float> *pa = (const wap::Schr2Args<float> *)b->d_args[2 * par];
-    const wap::Schr2Args<float> *pb = (const wap::Schr2Args<float> *)b->d_args[2 * par + 1];
+  const wap::Schr2Args<R> *pa = (const wap::Schr2Args<R> *)b->d_args[2 * par];
+  const wap::Schr2Args<R> *pb = (const wap::Schr2Args<R> *)b->d_args[2 * par + 1];
+  auto two = [&](auto rows, auto kahan) {
+    constexpr int RW = decltype(rows)::value;
+    constexpr bool KB = decltype(kahan)::value;
+    hipLaunchKernelGGL((wap::schr2_pass_batched<R, RW, 0, false>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
+    hipLaunchKernelGGL((wap::schr2_pass_batched<R, RW, 1, KB>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
+  };
+  auto rows = [&](auto kahan) {
     switch (b->waves) {
-      case 4:
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 3, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 3, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
-        break;
-      case 8:
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 7, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 7, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
-        break;
-      default:
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 15, 0>), grid, block, 0, b->stream, pa, b->d_tile_begin, n, ph);
-        hipLaunchKernelGGL((wap::schr2_pass_batched<float, 15, 1>), grid, block, 0, b->stream, pb, b->d_tile_begin, n, ph);
-        break;
+      case 4: two(std::integral_constant<int, 3>{}, kahan); break;
+      case 8: two(std::integral_constant<int, 7>{}, kahan); break;
+      default: two(std::integral_constant<int, 15>{}, kahan); break;
+    }
+  };
+  if constexpr (sizeof(R) == 4) {
+    if (b->precision == WAP_PREC_F32_KAHAN) {
+      rows(std::true_type{});
+      return;
     }
   }
+  rows(std::false_type{});
 }
 
 template <typename R, int EQ, bool K>
@@ -1457,7 +1472,7 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
   b->sched = s0;
   b->kg2 = kg_two_pass(h0) && !fused;
   b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
-  if (is_f64(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
+  if (is_f64(h0) && is_kg(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;
@@ -1580,6 +1595,190 @@ int wap_batch_replay(wap_batch *b, int64_t step0, double *ms) {
 int wap_batch_synchronize(wap_batch *b) {
   if (!b) return fail(WAP_ERR_ARG, "batch is NULL");
   WAP_HIP(hipStreamSynchronize(b->stream));
+  return WAP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- batch sets
+// One hipGraph over several batches (one per equation / precision / schedule group): the
+// graph forks to one stream per batch, runs each batch's steps there and joins them, so
+// the groups' launches overlap where the CUs allow.  The set owns a phase table per batch
+// (sized at capture), separate from the batches' own eager and graph tables.
+struct wap_batch_set {
+  std::vector<wap_batch *> bs;
+  std::vector<hipStream_t> streams;  // [0] the launch stream; one per batch
+  std::vector<hipEvent_t> events;    // [0] the fork, [i > 0] batch i's join
+  hipEvent_t done = nullptr;         // the end of a replay, which the handles' streams wait for
+  std::vector<hipStream_t> handle_streams;  // the distinct streams of the batches' handles
+  std::vector<void *> phase;         // per batch: [steps][ninst] phasors
+  std::vector<int64_t> phase_steps;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_steps = 0;
+};
+
+namespace {
+
+void batch_set_free(wap_batch_set *s) {
+  if (!s->streams.empty() && s->streams[0]) (void)hipStreamSynchronize(s->streams[0]);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->exec = nullptr;
+  s->graph = nullptr;
+  for (void *&p : s->phase)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  for (hipStream_t &st : s->streams)
+    if (st) { (void)hipStreamDestroy(st); st = nullptr; }
+  for (hipEvent_t &ev : s->events)
+    if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
+  if (s->done) (void)hipEventDestroy(s->done);
+  s->done = nullptr;
+}
+
+// The steps of every batch, each on its stream, forked from and joined into streams[0]
+// (called inside stream capture).
+int batch_set_record(wap_batch_set *s, int64_t nsteps) {
+  const size_t n = s->bs.size();
+  const hipEvent_t fork = s->events[0];
+  int rc = WAP_OK;
+  hipError_t e = hipEventRecord(fork, s->streams[0]);
+  for (size_t i = 0; i < n && e == hipSuccess && !rc; ++i) {
+    wap_batch *b = s->bs[i];
+    if (i > 0) e = hipStreamWaitEvent(s->streams[i], fork, 0);
+    if (e != hipSuccess) break;
+    hipStream_t own = b->stream;
+    b->stream = s->streams[i];  // the batch's launches go to the set's stream
+    rc = batch_dispatch(b, nsteps, s->phase[i]);
+    b->stream = own;
+    if (!rc && i > 0) e = hipEventRecord(s->events[i], s->streams[i]);
+  }
+  for (size_t i = 1; i < n && e == hipSuccess && !rc; ++i)
+    e = hipStreamWaitEvent(s->streams[0], s->events[i], 0);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("batch set capture: ") + hipGetErrorString(e));
+  return WAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wap_batch_set_create(wap_batch *const *batches, int n, wap_batch_set **out) {
+  if (!out || !batches || n < 1) return fail(WAP_ERR_ARG, "need at least one batch");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i)
+    if (!batches[i]) return fail(WAP_ERR_ARG, "NULL batch");
+  wap_batch_set *s = new (std::nothrow) wap_batch_set();
+  if (!s) return fail(WAP_ERR_OOM, "host allocation failed");
+  s->bs.assign(batches, batches + n);
+  s->streams.assign(n, nullptr);
+  s->events.assign(n, nullptr);
+  s->phase.assign(n, nullptr);
+  s->phase_steps.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    // default priority, non-blocking (capture needs streams of its own)
+    hipError_t e = hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->events[i], hipEventDisableTiming);
+    if (e == hipSuccess && i == 0) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      batch_set_free(s);
+      delete s;
+      return fail(WAP_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
+  }
+  for (wap_batch *b : s->bs)
+    for (wap_handle *h : b->hs)
+      if (std::find(s->handle_streams.begin(), s->handle_streams.end(), h->stream) ==
+          s->handle_streams.end())
+        s->handle_streams.push_back(h->stream);
+  *out = s;
+  return WAP_OK;
+}
+
+int wap_batch_set_destroy(wap_batch_set *s) {
+  if (!s) return WAP_OK;
+  batch_set_free(s);
+  delete s;
+  return WAP_OK;
+}
+
+int wap_batch_set_capture(wap_batch_set *s, int64_t nsteps) {
+  if (!s) return fail(WAP_ERR_ARG, "batch set is NULL");
+  if (nsteps < 1) return fail(WAP_ERR_ARG, "nsteps < 1");
+  WAP_HIP(hipStreamSynchronize(s->streams[0]));  // a replay of the previous graph
+  if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  for (size_t i = 0; i < s->bs.size(); ++i) {
+    wap_batch *b = s->bs[i];
+    // eager work of the batch (its own stream) ends before its arguments are rewritten
+    WAP_HIP(hipStreamSynchronize(b->stream));
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+    if (s->phase_steps[i] != nsteps && s->phase[i]) {
+      (void)hipFree(s->phase[i]);
+      s->phase[i] = nullptr;
+      s->phase_steps[i] = 0;
+    }
+    rc = batch_phase_dispatch(b, nsteps, 0, &s->phase[i], &s->phase_steps[i]);
+    if (rc) return rc;
+  }
+  WAP_HIP(hipStreamBeginCapture(s->streams[0], hipStreamCaptureModeThreadLocal));
+  int rc = batch_set_record(s, nsteps);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s->streams[0], &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(WAP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  s->graph = g;
+  WAP_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+  s->graph_steps = nsteps;
+  return WAP_OK;
+}
+
+int wap_batch_set_replay(wap_batch_set *s, int64_t step0, double *ms) {
+  if (!s) return fail(WAP_ERR_ARG, "batch set is NULL");
+  if (!s->exec) return fail(WAP_ERR_STATE, "no captured graph (wap_batch_set_capture)");
+  WAP_HIP(hipStreamSynchronize(s->streams[0]));  // the previous replay reads the tables
+  // earlier work on the handles (eager steps, state uploads) ends before the graph starts
+  for (hipStream_t hs : s->handle_streams) WAP_HIP(hipStreamSynchronize(hs));
+  for (size_t i = 0; i < s->bs.size(); ++i) {
+    wap_batch *b = s->bs[i];
+    WAP_HIP(hipStreamSynchronize(b->stream));
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+    // phase_steps == graph_steps: rewritten in place, never regrown
+    rc = batch_phase_dispatch(b, s->graph_steps, step0, &s->phase[i], &s->phase_steps[i]);
+    if (rc) return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    WAP_HIP(hipEventCreate(&e0));
+    WAP_HIP(hipEventCreate(&e1));
+    WAP_HIP(hipEventRecord(e0, s->streams[0]));
+  }
+  WAP_HIP(hipGraphLaunch(s->exec, s->streams[0]));
+  // later work on the handles' streams waits for the graph
+  WAP_HIP(hipEventRecord(s->done, s->streams[0]));
+  for (hipStream_t hs : s->handle_streams) WAP_HIP(hipStreamWaitEvent(hs, s->done, 0));
+  for (wap_batch *b : s->bs) batch_flip(b, s->graph_steps);
+  if (ms) {
+    WAP_HIP(hipEventRecord(e1, s->streams[0]));
+    WAP_HIP(hipEventSynchronize(e1));
+    float f = 0.f;
+    WAP_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return WAP_OK;
+}
+
+int wap_batch_set_synchronize(wap_batch_set *s) {
+  if (!s) return fail(WAP_ERR_ARG, "batch set is NULL");
+  WAP_HIP(hipStreamSynchronize(s->streams[0]));
   return WAP_OK;
 }
 
@@ -1798,7 +1997,7 @@ int adjoint_record_fused(wap_adjoint *ad, int64_t step, double t0, int slot) {
   int rc;
   if (ad->schr) {
     // the 2-pass step leaves psi_n in the buffer its output swapped with
-    rc = run_steps_schr2(h, 1, step, t0, nullptr, nullptr);
+    rc = run_steps_schr2<float, false>(h, 1, step, t0, nullptr, nullptr);
     if (rc) return rc;
     std::swap(h->yb_psi, sp[0]);
   } else if (sched_of(h) == WAP_SCHED_KG_FUSED) {
@@ -1943,7 +2142,8 @@ int adjoint_new(wap_handle *h, int slots, wap_adjoint **out) {
   a->h = h;
   a->slots = slots;
   a->ncomp = is_kg(h) ? 2 : 1;
-  a->schr = !is_kg(h) && sched_of(h) == WAP_SCHED_SCHR_FUSED;
+  // the fused Schrodinger adjoint is float32 (float64 handles reverse stage by stage)
+  a->schr = !is_kg(h) && sched_of(h) == WAP_SCHED_SCHR_FUSED && h->d.precision == WAP_PREC_F32;
   a->fused = (is_kg(h) && fused_plane_fits(h->d)) || a->schr;
   if (a->fused) {
     a->st.assign((size_t)slots * a->ncomp, nullptr);

@@ -1080,6 +1080,7 @@ struct Schr2Args {
   const SrcCoef<R> *src_tab;
   C src_ph;
   int stream_out;  // non-temporal result stores
+  C *cmp;          // B, f32_kahan: the state's compensation, read and rewritten in place
 };
 
 // H y from the lateral neighbour sums sx, sy and the z neighbours zp, zm
@@ -1100,9 +1101,15 @@ __device__ __forceinline__ cplx<R> horner(cplx<R> b, R f, cplx<R> hy) {
   return mkc<R>(b.x + f * hy.y, b.y - f * hy.x);
 }
 
-template <typename R, int ROWS, int PB>
+// KAHAN (pass B only): the state is (hi, comp) with value hi - comp, and pass B adds the
+// step's increment h M u2 to b = inject(hi) with the compensated update of the stage passes:
+// y = inc - keep comp, t = b + y, comp' = (t - b) - y (keep: the soft source's factor on
+// its band planes, which scales the state and so its compensation); on the frozen planes
+// hi' = b and comp' = keep comp.  The compensation of plane k is loaded one plane ahead.
+template <typename R, int ROWS, int PB, bool KAHAN>
 __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
+  static_assert(!KAHAN || (PB == 1 && sizeof(R) == 4), "Kahan: float32 pass B");
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
   constexpr int pw = kWaveX + 4;
   constexpr int nps = pw * (ROWS + 2);
@@ -1131,6 +1138,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rbs = buf_rsrc(PB ? p.base : p.in);
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const __amdgpu_buffer_rsrc_t rcm = buf_rsrc(p.cmp);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;
   int oo = -1, hs = ss;
@@ -1209,10 +1217,21 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   __syncthreads();
 
   C uprev = zero, umm = zero;  // level-1 outputs at m-1 and m-2 (this column)
+  // the compensation of plane m (own column; planes outside the chunk are not loaded)
+  auto load_cmp = [&](int m, C &c) {
+    const uint32_t bc = (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(C);
+    c = buf_ld<C>(rcm, (m >= k0 && m < k1) ? ow : kBufSkip, bc);
+  };
+  auto keep_of = [&](int k) {
+    const int bnd = k - p.src_k0;
+    return (bnd >= 0 && bnd < p.src_n) ? lsrc[bnd].keep : R(1);
+  };
 
-  // one plane: ra, rb hold planes m-1, m (prepared), rc plane m+1 (raw), pre receives m+2
-  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre) {
+  // one plane: ra, rb hold planes m-1, m (prepared), rc plane m+1 (raw), pre receives m+2;
+  // Kahan: ck holds the compensation of plane m-1 (the level-2 plane), cn receives plane m's
+  auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre, C &ck, C &cn) {
     const int rd = m & 1, wr = rd ^ 1;
+    if constexpr (KAHAN) load_cmp(m, cn);
     load_plane(m + 2, pre);
     inject_plane(m + 1, rc);
     // ---- level 1 at m
@@ -1232,9 +1251,24 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
       const C sx = cadd<R>(lu[ss + 1], lu[ss - 1]), sy = cadd<R>(lu[ss + pw], lu[ss - pw]);
       const int t = max(k, m_start) - m_start;
       const C hyv = schr_h<R>(p, ra.v, lzc[2 * t], lzc[2 * t + 1], uprev, sx, sy, ucur, umm);
-      const C o = horner<R>(base_of(ra), k == 0 ? R(0) : p.c2, hyv);
       const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
-      buf_st_pol<C>(stream, rout, (k >= k0) ? ow : kBufSkip, bc, o);
+      const uint32_t o_at = (k >= k0) ? ow : kBufSkip;
+      if constexpr (KAHAN) {
+        const C b = base_of(ra);
+        const C c = cscale<R>(ck, keep_of(k));
+        C o = b, cn2 = c;  // the frozen plane k = 0: hi' = b, comp' = keep comp
+        if (k != 0) {
+          const R f = p.c2;
+          const C y = csub<R>(mkc<R>(f * hyv.y, -f * hyv.x), c);  // inc = f (-i H u2)
+          o = cadd<R>(b, y);
+          cn2 = csub<R>(csub<R>(o, b), y);
+        }
+        buf_st_pol<C>(stream, rout, o_at, bc, o);
+        buf_st_pol<C>(stream, rcm, o_at, bc, cn2);
+      } else {
+        const C o = horner<R>(base_of(ra), k == 0 ? R(0) : p.c2, hyv);
+        buf_st_pol<C>(stream, rout, o_at, bc, o);
+      }
     }
     lub[wr][ss] = ucur;
     store_state(wr, rc);
@@ -1243,40 +1277,47 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
     uprev = ucur;
   };
 
+  C c0 = zero, c1 = zero;  // (plane m_start - 1 is never stored: its compensation is unused)
   int m = m_start;
   for (; m + 3 <= m_end; m += 4) {
-    plane_step(m, r0, r1, r2, r3);
-    plane_step(m + 1, r1, r2, r3, r0);
-    plane_step(m + 2, r2, r3, r0, r1);
-    plane_step(m + 3, r3, r0, r1, r2);
+    plane_step(m, r0, r1, r2, r3, c0, c1);
+    plane_step(m + 1, r1, r2, r3, r0, c1, c0);
+    plane_step(m + 2, r2, r3, r0, r1, c0, c1);
+    plane_step(m + 3, r3, r0, r1, r2, c1, c0);
   }
-  if (m <= m_end) plane_step(m, r0, r1, r2, r3);
-  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0);
-  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1);
+  if (m <= m_end) plane_step(m, r0, r1, r2, r3, c0, c1);
+  if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0, c1, c0);
+  if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, c0, c1);
   // the frozen top plane nz-1 of the last chunk: its level-1 output is b exactly
   if (k1 == nz) {
     const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
     buf_st_pol<C>(stream, rout, ow, bc, uprev);
+    if constexpr (KAHAN) {
+      C c;
+      load_cmp(nz - 1, c);
+      buf_st_pol<C>(stream, rcm, ow, bc, cscale<R>(c, keep_of(nz - 1)));
+    }
   }
 }
 
-template <typename R, int ROWS, int PB>
+template <typename R, int ROWS, int PB, bool KAHAN>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass(
     const Schr2Args<R> p) {
-  // two 16-wave workgroups per CU
-  schr2_body<R, ROWS, PB>(
-      p, zgroup_tile<2 * kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty, p.ntz),
+  // float32: two 16-wave workgroups per CU (42 KB of LDS each); float64: one (82 KB)
+  constexpr int G = sizeof(R) == 4 ? 2 * kXcdCus : kXcdCus;
+  schr2_body<R, ROWS, PB, KAHAN>(
+      p, zgroup_tile<G>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty, p.ntz),
       p.src_ph);
 }
 
-template <typename R, int ROWS, int PB>
+template <typename R, int ROWS, int PB, bool KAHAN>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr2_pass_batched(
     const Schr2Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
     const cplx<R> *__restrict__ ph) {
   const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
   const int inst = find_instance(tile_begin, ninst, t);
   const Schr2Args<R> a = args[inst];
-  schr2_body<R, ROWS, PB>(a, t - tile_begin[inst], ph[inst]);
+  schr2_body<R, ROWS, PB, KAHAN>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ------------------------------------------------------------------ adjoint

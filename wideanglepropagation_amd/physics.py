@@ -140,8 +140,7 @@ class FDTDParams:
         fits = self.nx * self.ny * self.nz * (16 if self.precision == "f64" else 8) < 2 ** 31
         if self.equation != "klein_gordon":
             if self.schedule == "auto":
-                ok = self.precision == "f32" and self.end_planes == "frozen" and fits
-                return "schr_fused" if ok else "stages"
+                return "schr_fused" if self.end_planes == "frozen" and fits else "stages"
             return self.schedule
         if self.schedule == "auto":
             return "kg_fused" if fits else "kg_2pass"

@@ -16,9 +16,10 @@ batched    48 instances, sizes from {96^3, 128^3, 160x160x256, 192^3}, beam
            energies U[60, 300] keV (seed 1), convergent Gaussian probes at
            random lateral positions over a shared 200-atom slab; alternating
            Schrodinger / Klein-Gordon, complex64 (the fused schedules), 1000
-           steps in hipGraphs of 100 steps per equation group.  Reports
-           aggregate Gvoxel-steps/s and the z chunk the autotuner picked per
-           size class.
+           steps replayed as one hipGraph of 100 steps (the two equation
+           groups on forked streams inside it).  Reports aggregate
+           Gvoxel-steps/s and the launch shape the autotuner picked: waves per
+           equation group, z chunk per size class.
 adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
            L = || |fft2(exit)|^2 - I_target ||^2 with I_target from a copy of the
            150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
@@ -167,7 +168,8 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
     if autotune:
         # the fused schedules (the float32 default): workgroups of 16 waves; the z chunk
         # per size class, timed on the whole batch
-        tuned, tuned_ms, _ = tune_batch(params, tile_zs=(16, 32, 64), tile_y=16)
+        tuned, tuned_ms, _ = tune_batch(params, tile_zs=(16, 32, 64), tile_ys=(8, 16),
+                                        tile_y=16)
         print(f"[batched] tuned z chunks: {tuned_ms:.3f} ms per batched step", file=sys.stderr,
               flush=True)
     for i, (shape, p) in enumerate(zip(shapes, params)):
@@ -177,7 +179,7 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
         ext = centred_extent(shape, spacing, spacing_z)
         cls = (p.equation, p.precision, p.resolved_schedule(), p.shape)
         if cls in tuned:
-            p = p.with_(tile_y=16, tile_z=tuned[cls])
+            p = p.with_(tile_y=tuned[cls][0], tile_z=tuned[cls][1])
         V = atom_potential(shape, ext, pos, Zs, scr, r_soft=spacing, cutoff=4.0, device="cuda")
         x0, y0 = rng.uniform(ext[0], ext[1]), rng.uniform(ext[2], ext[3])
         prof = probe_profile(shape, ext, x0, y0, lam)
@@ -209,7 +211,7 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
            "total_voxels": vox, "gvoxel_steps_per_s": vox * done / wall / 1e9,
            "gvoxel_steps_per_s_device": vox * done / (dev_ms * 1e-3) / 1e9,
            "graph_steps_per_launch": chunk, "launch_groups": b.tiles(),
-           "autotuned": {f"{k[0]}/{'x'.join(map(str, k[3]))}": {"tile_y": 16, "tile_z": v}
+           "autotuned": {f"{k[0]}/{'x'.join(map(str, k[3]))}": {"waves": v[0], "tile_z": v[1]}
                          for k, v in tuned.items()},
            "autotuned_ms_per_batched_step": tuned_ms,
            "all_finite": bool(np.all(np.isfinite(norms))),
