@@ -650,6 +650,12 @@ __global__ __launch_bounds__(512) void kg2_pass_batched(const KG2Args<R> *__rest
 // float32 V): read psi, pi, V (20) + write psi', pi' (16) = 36 B, plus the
 // halo columns / planes re-read through L2.
 constexpr int kKG1MaxRows = 16;
+// LDS row pitch of the haloed planes of the fused passes (in complex values): the 68
+// columns -2 .. 65 of a tile plus one unused slot, so that the side wave, whose lanes walk
+// down one column, meets a different pair of banks in every row (a pitch of 136 dwords,
+// 8 mod 32, repeats the banks every 4 rows; 138, 10 mod 32, spreads 15 rows over 30 banks:
+// the fused KG adjoint 1-2 % faster, the forward passes unchanged)
+constexpr int kLdsPitch = kWaveX + 5;
 constexpr int kKG1MaxTileZ = 256;  // planes per workgroup chunk (the sponge table lives in LDS)
 
 template <typename R>
@@ -714,7 +720,7 @@ __device__ __forceinline__ cplx<R> kg1_damp_t(R dg, R dv, cplx<R> x) {
 // follow it.
 template <typename R, int ROWS>
 struct KG1Lds {
-  static constexpr int pw = kWaveX + 4;  // LDS row pitch: columns -2 .. 65 of the tile
+  static constexpr int pw = kLdsPitch;  // LDS row pitch: columns -2 .. 65 of the tile
   static constexpr int nps = pw * (ROWS + 2);
   cplx<R> lq3b[2][nps];  // Y3 (only the tile's columns and side columns are used)
   cplx<R> lq4b[2][nps];  // Y4
@@ -1111,7 +1117,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   using C = cplx<R>;
   static_assert(!KAHAN || (PB == 1 && sizeof(R) == 4), "Kahan: float32 pass B");
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
-  constexpr int pw = kWaveX + 4;
+  constexpr int pw = kLdsPitch;
   constexpr int nps = pw * (ROWS + 2);
   __shared__ C lsb[2][nps];  // level-1 input of the plane (A: b, B: u3) with the halo ring
   __shared__ C lub[2][nps];  // level-1 output (A: u4, B: u2): tile and side columns
@@ -1575,7 +1581,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     const SchrAdjArgs<R> p) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
-  constexpr int pw = kWaveX + 4;
+  constexpr int pw = kLdsPitch;
   constexpr int nps = pw * (ROWS + 2);
   // level-1 inputs with the halo ring (I: b and lam^; II: u3 and v3^) and level-1
   // outputs for level 2 (I: u4 and v2^; II: v4^), double-buffered
@@ -1912,7 +1918,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     const KG1AdjArgs<R> p) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
-  constexpr int pw = kWaveX + 4;
+  constexpr int pw = kLdsPitch;
   constexpr int nps = pw * (ROWS + 2);
   __shared__ C lssb[2][nps];  // forward state psi (injected)
   __shared__ C lsqb[2][nps];  // forward state pi

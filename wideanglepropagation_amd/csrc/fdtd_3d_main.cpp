@@ -94,7 +94,7 @@ int main(int argc, char **argv) {
   }
   auto d = read_defines(params);
   if (d.empty()) {
-    printf("Error: no #define parameters in %s.\n", params.empty() ? "(none found)" : params.c_str());
+    printf("fdtd_3d_bin: no #define parameters in %s\n", params.empty() ? "(none found)" : params.c_str());
     return 1;
   }
   const bool two_d = d.find("NZ") == d.end();
@@ -150,7 +150,7 @@ int main(int argc, char **argv) {
   std::vector<float> V(n);
   FILE *fp = fopen("potential.bin", "rb");
   if (!fp) {
-    printf("Error: potential.bin not found.\n");
+    printf("fdtd_3d_bin: cannot open potential.bin\n");
     return 1;
   }
   const int nhdr = two_d ? 2 : 3;
@@ -161,19 +161,19 @@ int main(int argc, char **argv) {
   const bool hdr_ok = two_d ? (hdr[0] == desc.nx && hdr[1] == desc.nz)
                             : (hdr[0] == desc.nx && hdr[1] == desc.ny && hdr[2] == desc.nz);
   if (got_h != (size_t)nhdr || got != n || !hdr_ok) {
-    printf("Error: potential.bin holds %zu floats for a [%d %d %d] header; expected %zu.\n",
+    printf("fdtd_3d_bin: potential.bin has %zu values after its [%d %d %d] header, not %zu\n",
            got, hdr[0], hdr[1], two_d ? 0 : hdr[2], n);
     return 1;
   }
 
   if (two_d) {
-    printf("Initializing 2D Simulation (Top/Bottom PML, X periodic)...\n");
-    printf("Grid: %d x %d\n", desc.nx, desc.nz);
-    printf("E_SIM: %.4f, P_INF=%.4f\n", e_sim, k_beam);
+    printf("2D grid, periodic in x, absorbing layers at both ends of the propagation axis\n");
+    printf("nx %d, ny %d\n", desc.nx, desc.nz);
+    printf("carrier %.4f, wavenumber %.4f\n", e_sim, k_beam);
   } else {
-    printf("Initializing 3D Simulation (X, Y Periodic; Z PML)...\n");
-    printf("Grid: %d x %d x %d\n", desc.nx, desc.ny, desc.nz);
-    printf("E_SIM: %.4f (LapCoeff=%.4f)\n", e_sim, lap);
+    printf("3D grid, periodic in x and y, absorbing layers at both z ends\n");
+    printf("nx %d, ny %d, nz %d\n", desc.nx, desc.ny, desc.nz);
+    printf("carrier %.4f, Laplacian coefficient %.4f\n", e_sim, lap);
   }
 
   wap_handle *h = nullptr;
@@ -182,7 +182,7 @@ int main(int argc, char **argv) {
 
   FILE *fs = fopen("snapshots.bin", "wb");
   if (!fs) {
-    printf("Error opening snapshots.bin\n");
+    printf("fdtd_3d_bin: cannot create snapshots.bin\n");
     wap_destroy(h);
     return 1;
   }
@@ -204,7 +204,7 @@ int main(int argc, char **argv) {
   int saved = 0;
   for (int frame = 0; frame < n_frames; ++frame) {
     if (wap_step(h, spf, (int64_t)frame * spf, 0.0)) return die("wap_step");
-    if (frame % 10 == 0) printf("Frame %d / %d\n", frame, n_frames);
+    if (frame % 10 == 0) printf("frame %d of %d\n", frame, n_frames);
     if (frame >= save_start && saved < m_snap) {
       if (wap_get_state(h, st.data(), nullptr, WAP_DT_C64, 0)) return die("wap_get_state");
       for (size_t t = 0; t < n; ++t) {
@@ -214,7 +214,7 @@ int main(int argc, char **argv) {
       const float t_snap = (float)(frame * spf) * dt;
       if (fwrite(&t_snap, sizeof(float), 1, fs) != 1 || fwrite(phi.data(), sizeof(float), n, fs) != n ||
           fwrite(psi.data(), sizeof(float), n, fs) != n) {
-        fprintf(stderr, "Error: Disk write failed at frame %d.\n", frame);
+        fprintf(stderr, "fdtd_3d_bin: writing snapshots.bin failed at frame %d\n", frame);
         fclose(fs);
         wap_destroy(h);
         return 1;
@@ -224,6 +224,6 @@ int main(int argc, char **argv) {
   }
   fclose(fs);
   wap_destroy(h);
-  printf("Done. Saved %d snapshots.\n", saved);
+  printf("%d snapshots written\n", saved);
   return 0;
 }
