@@ -14,10 +14,14 @@ def test_adjoint_study_quick():
     r = adjoint_study(quick=True)
     assert r["fd_voxels"] == 6
     assert r["max_rel_err_f64_adjoint_vs_fd"] < 1e-4
-    plan = r["segment_plan"]
-    assert plan["segment"] >= math.ceil(math.sqrt(r["steps"]))
+    s = math.ceil(math.sqrt(r["steps"]))
+    sq = r["cost"]["sqrt_t"]["segment_plan"]
+    assert sq["segment"] == s and sq["recomputed_steps"] == r["steps"] - s
+    plan = r["cost"]["memory_budget"]["segment_plan"]
+    assert plan["segment"] >= s
     assert plan["recomputed_steps"] == r["steps"] - plan["segment"] or plan["segment"] == r["steps"]
-    assert r["t_forward_s"] > 0 and r["t_forward_backward_s"] > r["t_forward_s"]
+    for c in r["cost"].values():
+        assert c["t_forward_s"] > 0 and c["t_forward_backward_s"] > c["t_forward_s"]
 
 
 def test_api_study_quick():

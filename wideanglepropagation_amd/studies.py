@@ -23,11 +23,13 @@ batched    48 instances, sizes from {96^3, 128^3, 160x160x256, 192^3}, beam
 adjoint    Klein-Gordon 128 x 128 x 256, complex64 fields, f32 V, 1500 steps;
            L = || |fft2(exit)|^2 - I_target ||^2 with I_target from a copy of the
            150-atom slab (seed 2) with positions perturbed by N(0, 0.05 A)
-           (seed 3).  Checkpointing in segments of >= sqrt(T) steps sized to half the
-           free device memory.  Checks dL/dV against central
-           differences of the f64 forward at 32 random voxels (seed 4) and reports
-           backward / forward time; the same loss and step count are also timed
-           on the 512 x 512 x 256 grid of the precision study (timing only).
+           (seed 3).  Reports backward / forward time with sqrt(T) checkpoints
+           (segments of ceil(sqrt(T)) steps, every one but the last recomputed) and
+           with the default plan (segments as long as half the free device memory
+           holds), and checks dL/dV against central differences of the f64
+           forward at 32 voxels drawn uniformly (seed 4) between the source and the
+           exit plane; the same loss and step count are also timed on the
+           512 x 512 x 256 grid of the precision study (timing only).
 spectral   The plane-wave comparison of ``fdtd_3D_baseline_comparison.ipynb``:
            lambda = 0.05 A, 64 x 64 over [-1, 1]^2 A, one Z = 79 atom (screened
            Coulomb, 0.3 A) in the ROI z in [-1, 1] A, FDTD (Schrodinger, drop-in
@@ -244,55 +246,66 @@ def adjoint_study(quick=False, n_fd=32):
     spf = nsteps // n_frames
     samples = {(f + 1) * spf: 1.0 / 4 for f in range(6, 10)}
     V32 = V.to(torch.float32)
+    sqrt_t = math.ceil(math.sqrt(nsteps))
     with torch.no_grad():
         It = far_field_intensity(exit_wave(Vt.to(torch.float32), p32, nsteps, k_exit,
                                            samples=samples))
 
-    def loss_of(Vx, p):
-        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples)
+    def loss_of(Vx, p, **kw):
+        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples, **kw)
         return torch.sum((far_field_intensity(E) - It) ** 2)
 
     with torch.no_grad():
         L0 = loss_of(V32, p32)
-    t_fwd, t_fb, Vg = _time_forward_backward(lambda Vx: loss_of(Vx, p32), V32)
-    plan = dict(adjoint_plan)
+    # the cost with sqrt(T) checkpoints (every segment but the last recomputed), and with the
+    # default plan (the longest segments half the free device memory holds)
+    costs = {}
+    for plan_name, kw in (("sqrt_t", {"checkpoint_every": sqrt_t}), ("memory_budget", {})):
+        t_fwd, t_fb, Vg = _time_forward_backward(lambda Vx: loss_of(Vx, p32, **kw), V32)
+        costs[plan_name] = {"segment_plan": dict(adjoint_plan), "t_forward_s": t_fwd,
+                            "t_forward_backward_s": t_fb,
+                            "backward_over_forward": (t_fb - t_fwd) / t_fwd}
+        print(f"[adjoint] {plan_name}: forward {t_fwd:.3f} s, forward+backward {t_fb:.3f} s",
+              file=sys.stderr, flush=True)
     g32 = Vg.grad.double().cpu().numpy()
-    print(f"[adjoint] forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s", file=sys.stderr,
-          flush=True)
     # f64 adjoint for the same loss
     p64 = p32.with_(precision="f64")
     Vd = V32.double().clone().requires_grad_(True)
-    L64 = loss_of(Vd, p64)
+    L64 = loss_of(Vd, p64, checkpoint_every=sqrt_t)
     L64.backward()
     g64 = Vd.grad.cpu().numpy()
-    # central differences of the f64 forward at random voxels (and the largest-gradient ones)
+    # central differences of the f64 forward at voxels drawn uniformly (seed 4) from the
+    # planes the wave crosses between the source and the exit plane; a voxel whose
+    # difference cannot resolve 1e-4 (|fd| below 1e4 times its float64 rounding,
+    # 4 eps |L| / 2h) is drawn again
     rng = np.random.default_rng(4)
-    idx = [tuple(int(rng.integers(0, s)) for s in shape) for _ in range(n_fd // 2)]
-    top = np.argsort(np.abs(g64).ravel())[-(n_fd - len(idx)):]
-    idx += [tuple(int(v) for v in np.unravel_index(t, shape)) for t in top]
     hstep = 1e-3 * float(V32.abs().max())
-    rows = []
+    resolvable = 1e4 * 4 * np.finfo(np.float64).eps * abs(L64.item()) / (2 * hstep)
+    rows, redrawn = [], 0
     Vbase = V32.double()
-    for ix in idx:
+    while len(rows) < n_fd and redrawn <= n_fd:
+        ix = (int(rng.integers(p32.src_k + 3, k_exit + 1)), int(rng.integers(0, shape[1])),
+              int(rng.integers(0, shape[2])))
         Vp = Vbase.clone()
         Vp[ix] += hstep
         Vm = Vbase.clone()
         Vm[ix] -= hstep
         with torch.no_grad():
             fd = (loss_of(Vp, p64).item() - loss_of(Vm, p64).item()) / (2 * hstep)
+        if abs(fd) < resolvable:
+            redrawn += 1
+            continue
         rows.append((ix, fd, float(g64[ix]), float(g32[ix])))
         if len(rows) % 8 == 0:
-            print(f"[adjoint] {len(rows)}/{len(idx)} finite differences", file=sys.stderr,
+            print(f"[adjoint] {len(rows)}/{n_fd} finite differences", file=sys.stderr,
                   flush=True)
-    gmax = float(np.abs(g64).max())
     big = None if quick else _adjoint_timing((256, 512, 512), nsteps, samples)
-    err64 = [abs(r[2] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
-    err32 = [abs(r[3] - r[1]) / max(abs(r[1]), 1e-3 * gmax) for r in rows]
+    err64 = [abs(r[2] - r[1]) / abs(r[1]) for r in rows]
+    err32 = [abs(r[3] - r[1]) / abs(r[1]) for r in rows]
     return {"study": "adjoint", "grid_nz_ny_nx": list(shape), "steps": nsteps,
-            "segment_plan": plan, "loss": float(L0),
-            "t_forward_s": t_fwd, "t_forward_backward_s": t_fb,
-            "backward_over_forward": (t_fb - t_fwd) / t_fwd,
-            "fd_step": hstep, "fd_voxels": len(rows),
+            "loss": float(L0), "cost": costs,
+            "backward_over_forward": costs["sqrt_t"]["backward_over_forward"],
+            "fd_step": hstep, "fd_voxels": len(rows), "fd_redrawn": redrawn,
             "max_rel_err_f64_adjoint_vs_fd": float(max(err64)),
             "median_rel_err_f64_adjoint_vs_fd": float(np.median(err64)),
             "max_rel_err_f32_adjoint_vs_fd": float(max(err32)),
@@ -327,7 +340,7 @@ def _time_forward_backward(loss_of, V, reps=3):
 
 def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
     """Forward and forward + backward time of the adjoint study's loss on another grid
-    (random target: timing only)."""
+    (random target: timing only), with sqrt(T) checkpoints and with the default plan."""
     from .adjoint import exit_wave, far_field_intensity
     from .adjoint import last_plan as adjoint_plan
     p, V = kg_slab(shape, precision="f32", device="cuda", n_atoms=n_atoms, seed=2)
@@ -338,19 +351,22 @@ def _adjoint_timing(shape, nsteps, samples, n_atoms=150):
     It = torch.rand((p.ny, p.nx), dtype=torch.float64, device="cuda",
                     generator=torch.Generator(device="cuda").manual_seed(5))
 
-    def loss_of(Vx):
-        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples)
+    def loss_of(Vx, **kw):
+        E = exit_wave(Vx, p, nsteps, k_exit, samples=samples, **kw)
         return torch.sum((far_field_intensity(E) - It) ** 2)
 
     with torch.no_grad():  # warm-up (allocations, library load)
         loss_of(V32)
-    t_fwd, t_fb, _ = _time_forward_backward(loss_of, V32)
-    plan = dict(adjoint_plan)
-    print(f"[adjoint] {shape}: forward {t_fwd:.2f} s, forward+backward {t_fb:.2f} s",
-          file=sys.stderr, flush=True)
-    return {"grid_nz_ny_nx": list(shape), "steps": nsteps,
-            "segment_plan": plan, "t_forward_s": t_fwd,
-            "t_forward_backward_s": t_fb, "backward_over_forward": (t_fb - t_fwd) / t_fwd}
+    out = {"grid_nz_ny_nx": list(shape), "steps": nsteps}
+    for plan_name, kw in (("sqrt_t", {"checkpoint_every": math.ceil(math.sqrt(nsteps))}),
+                          ("memory_budget", {})):
+        t_fwd, t_fb, _ = _time_forward_backward(lambda Vx: loss_of(Vx, **kw), V32)
+        out[plan_name] = {"segment_plan": dict(adjoint_plan), "t_forward_s": t_fwd,
+                          "t_forward_backward_s": t_fb,
+                          "backward_over_forward": (t_fb - t_fwd) / t_fwd}
+        print(f"[adjoint] {shape} {plan_name}: forward {t_fwd:.2f} s, forward+backward "
+              f"{t_fb:.2f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def _notebook_grid(dz_roi):
