@@ -2048,10 +2048,18 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
   fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
   const dim3 block(wap::kWaveX, waves);
   const dim3 grid((unsigned)(ntx * nty * ntz));
-  if (sizeof(R) == 4 && waves == 16)
-    hipLaunchKernelGGL((wap::kg1_adj_pass<R, sizeof(R) == 4 ? 15 : 7>), grid, block, 0, h->stream, a);
-  else
-    hipLaunchKernelGGL((wap::kg1_adj_pass<R, 7>), grid, block, 0, h->stream, a);
+  constexpr int kRows = sizeof(R) == 4 ? 15 : 7;
+  if (sizeof(R) == 4 && waves == 16) {
+    if (a.stream_out)
+      hipLaunchKernelGGL((wap::kg1_adj_pass<R, kRows, true>), grid, block, 0, h->stream, a);
+    else
+      hipLaunchKernelGGL((wap::kg1_adj_pass<R, kRows, false>), grid, block, 0, h->stream, a);
+  } else {
+    if (a.stream_out)
+      hipLaunchKernelGGL((wap::kg1_adj_pass<R, 7, true>), grid, block, 0, h->stream, a);
+    else
+      hipLaunchKernelGGL((wap::kg1_adj_pass<R, 7, false>), grid, block, 0, h->stream, a);
+  }
   WAP_HIP(hipGetLastError());
   ad->cur ^= 1;
   return WAP_OK;

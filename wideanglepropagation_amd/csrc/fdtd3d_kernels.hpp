@@ -91,6 +91,12 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, kBufRange, kBufFlags);
 }
+// Pins a register value at this point of the program: an empty asm statement that reads
+// and rewrites it (no instruction is emitted), so nothing computed from the value can be
+// moved above it.  Placed where a prefetched value is consumed, it keeps the compiler's
+// wait for the load there instead of right after the load is issued.
+__device__ __forceinline__ void pin(float &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 template <typename T>
 __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
   if constexpr (sizeof(T) == 4) {
@@ -121,8 +127,10 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, u
   }
 }
 
-// a result store whose cache policy the launch picks (stream: a wave-uniform flag; both
-// sides issue one store, so the load-counter accounting is the same either way)
+// a result store whose cache policy the launch picks (stream: a wave-uniform flag).  Each
+// side issues one store, but the compiler's memory-counter accounting merges the two
+// paths conservatively: a later wait for an older load then waits for these stores too.
+// Loops that prefetch across iterations take the policy as a template parameter instead.
 template <typename T>
 __device__ __forceinline__ void buf_st_pol(bool stream, __amdgpu_buffer_rsrc_t r, uint32_t off,
                                            uint32_t soff, T v) {
@@ -1913,7 +1921,11 @@ struct KG1AdjArgs {
   int stream_out;  // non-temporal result stores
 };
 
-template <typename R, int ROWS>
+// NT: non-temporal result stores.  A compile-time policy keeps the stores in one basic
+// block, so the loop's memory-counter waits can count them (a run-time choice between two
+// store instructions made every wait at the loop latch drain them, 6 % of the kernel's
+// time at 512 x 512 x 256).
+template <typename R, int ROWS, bool NT>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_adj_pass(
     const KG1AdjArgs<R> p) {
   using C = cplx<R>;
@@ -1981,7 +1993,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t owl = writer ? ol : kBufSkip;
   const uint32_t owd = writer ? (uint32_t)ij * sizeof(R) : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
-  const bool stream = p.stream_out != 0;
+  constexpr bool stream = NT;
   const C ph = p.src_ph;
   const C *prof = p.prof;
   const int k0 = tz * p.tile_z;
@@ -2055,13 +2067,11 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     a4 = cscale<R>(lq, h6);
     a3 = caxpy<R>(cscale<R>(lq, h3), h, yb4q);
   };
-  // plane m into LDS buffer b: own column (prepared) and the outer neighbour
-  auto store_plane = [&](auto dgt, int b, int m, C s_, C q_, C a4, C a3, Halo &o) {
-    lssb[b][ss] = s_;
-    lsqb[b][ss] = q_;
-    la4b[b][ss] = a4;
-    la3b[b][ss] = a3;
+  // the outer neighbour of plane m (prepared) into LDS buffer b
+  auto store_halo = [&](auto dgt, int b, int m, Halo &o) {
     if (has_o) {
+      pin(o.s.x); pin(o.s.y); pin(o.q.x); pin(o.q.y);
+      pin(o.ls.x); pin(o.ls.y); pin(o.lq.x); pin(o.lq.y); pin(o.v);
       inject(m, o.s, o.q, oo);
       C oa4, oa3;
       point_a(dgt, m, o.ls, o.lq, o.v, oa4, oa3);
@@ -2070,6 +2080,14 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       la4b[b][hs] = oa4;
       la3b[b][hs] = oa3;
     }
+  };
+  // plane m into LDS buffer b: own column (prepared) and the outer neighbour
+  auto store_plane = [&](auto dgt, int b, int m, C s_, C q_, C a4, C a3, Halo &o) {
+    lssb[b][ss] = s_;
+    lsqb[b][ss] = q_;
+    la4b[b][ss] = a4;
+    la3b[b][ss] = a3;
+    store_halo(dgt, b, m, o);
   };
   // A x with the operator coefficient dl = d0_lo - V^2 of the center voxel
   auto apply = [&](R dl, C x, C sx, C sy, C sz) {
@@ -2086,7 +2104,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   auto crs = [](C k, C y, R acc) { return acc + k.x * y.y - k.y * y.x; };
 
   Raw r1, r2, r3;
-  Halo hn;
+  Halo hn;  // the prologue's outer neighbours of plane m_start
   load_plane(m_start, r1);
   load_plane(m_start + 1, r2);
   load_halo(m_start, hn);
@@ -2119,7 +2137,6 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     lols[own] = r1.ls;
     lolq[own] = r1.lq;
     lov[own] = r1.v;
-    load_halo(m_start + 1, hn);  // the outer neighbours of the first loop plane's m + 1
   }
   __syncthreads();
 
@@ -2143,11 +2160,20 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
                         const R vb, R &vb_next) {
     constexpr bool DG = decltype(dgt)::value;
     const int rd = m & 1, wr = rd ^ 1;
-    // loads: vbar for the next iteration's pass-2 plane, then the prefetch of plane m+2
-    // (older loads are the ones waited for); hn, the outer neighbours of plane m+1, was
-    // loaded at the end of the previous iteration
-    vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
-                        (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+    // loads, oldest first, so that a wait for one never waits for a younger prefetch: the
+    // outer neighbours of plane m+1 (halo waves; stored after this iteration's pass-1) or
+    // vbar of the next iteration's pass-2 plane (writer waves), then the prefetch of plane
+    // m+2.  The scheduling barrier keeps the compiler from interleaving the two groups
+    // (interleaved, the halo values were copied, and so waited for, right after issue).
+    Halo hn;
+    if (wave_writes) {
+      vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
+                          (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+    } else {
+      vb_next = R(0);
+      load_halo(m + 1, hn);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     load_plane(m + 2, pre);
     inject(m + 1, rc.s, rc.q, ij);
     C ca4, ca3;
@@ -2228,6 +2254,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       lns = cscale<R>(cadd<R>(prev.ps, cadd<R>(T2, T1)), kp);
       lnq = cscale<R>(caxpy<R>(prev.pq, in_k ? h2 : R(0), T2), kp);
       if (in_k) g = prev.g1 + R(2) * (a4m.x * wv4.y - a4m.y * wv4.x);
+    } else {
+      // the halo waves (exactly the waves that write no pass-2 result): the outer
+      // neighbours of plane m+1 into buffer wr, which no lane reads laterally this iteration
+      store_halo(dgt, wr, m + 1, hn);
     }
     {
       const uint32_t b = (uint32_t)max(k, 0) * plane;
@@ -2241,11 +2271,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     lols[own] = rc.ls;
     lolq[own] = rc.lq;
     lov[own] = rc.v;
-    store_plane(dgt, wr, m + 1, rc.s, rc.q, ca4, ca3, hn);
-    // the outer neighbours of the next iteration's plane m+2, issued a whole iteration
-    // before their use (issued at the top of the iteration, the halo lanes waited for
-    // them at its end: 8 % of the kernel's time at 512 x 512 x 256)
-    load_halo(m + 2, hn);
+    lssb[wr][ss] = rc.s;
+    lsqb[wr][ss] = rc.q;
+    la4b[wr][ss] = ca4;
+    la3b[wr][ss] = ca3;
     __syncthreads();
   };
 
