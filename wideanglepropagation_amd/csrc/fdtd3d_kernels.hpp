@@ -1584,7 +1584,9 @@ __device__ __forceinline__ R imcj(cplx<R> a, cplx<R> b) {
   return a.x * b.y - a.y * b.x;
 }
 
-template <typename R, int ROWS, int PASS>
+// NT: non-temporal result stores, a compile-time policy so that the loop's memory-counter
+// waits can count the stores (see kg1_adj_pass)
+template <typename R, int ROWS, int PASS, bool NT>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr_adj_pass(
     const SchrAdjArgs<R> p) {
   using C = cplx<R>;
@@ -1596,6 +1598,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   __shared__ C lfa[2][nps], laa[2][nps], lfb[2][nps], lab[2][nps];
   __shared__ C lzc[2 * (kKG1MaxTileZ + 6)];  // planes m_start - 2 .. m_end + 1
   __shared__ SrcCoef<R> lsrc[kMaxBand];
+  // the source profile at this lane's column and at its outer neighbour, staged once (a
+  // profile load inside the plane loop left a full memory-counter wait on every plane)
+  __shared__ C lprof[2][(ROWS + 1) * kWaveX];
   const int tile = zgroup_tile<kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty,
                                         p.ntz);
   const int tx = tile % p.ntx;
@@ -1604,6 +1609,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int tz = rest / p.nty;
   const int lx = threadIdx.x;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int own = wv * kWaveX + lx;
   const bool side = wv == ROWS;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const int sl = lx % (2 * ROWS);
@@ -1638,10 +1644,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
   const uint32_t owr = writer ? ov : kBufSkip;
-  const C zero = mkc<R>(R(0), R(0));
-  const bool stream = p.stream_out != 0;
+  constexpr bool stream = NT;
   const C ph = p.src_ph;
   const C *prof = p.prof;
+  const bool has_prof = prof != nullptr;  // the loop keeps this flag, not the pointer
   const int k0 = tz * p.tile_z;
   const int k1 = min(nz, k0 + p.tile_z);
   const R h = p.h, h2 = p.h * R(0.5), h3 = p.h / R(3), h4 = p.h * R(0.25);
@@ -1654,10 +1660,15 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     for (int t = tid; t <= m_end + 1 - zlo; t += kWaveX * (ROWS + 1)) {
       const int k = zlo + t;
       const bool in = k >= 0 && k < nz;
-      lzc[2 * t] = in ? p.zc[2 * k] : zero;
-      lzc[2 * t + 1] = in ? p.zc[2 * k + 1] : zero;
+      lzc[2 * t] = in ? p.zc[2 * k] : mkc<R>(R(0), R(0));
+      lzc[2 * t + 1] = in ? p.zc[2 * k + 1] : mkc<R>(R(0), R(0));
     }
     if (tid < p.src_n) lsrc[tid] = p.src_tab[tid];
+    // the planes this chunk injects into: m_start - 1 .. m_end + 1
+    if (prof && m_start - 1 < p.src_k0 + p.src_n && m_end + 1 >= p.src_k0) {
+      lprof[0][own] = prof[ij];
+      lprof[1][own] = has_o ? prof[oo] : mkc<R>(R(1), R(0));
+    }
   }
   auto interior = [&](int m) { return m > 0 && m < nz - 1; };
   auto zw = [&](int k, int which) { return lzc[2 * (k - zlo) + which]; };  // 0: cp, 1: cm
@@ -1665,15 +1676,13 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     const int bnd = k - p.src_k0;
     return (bnd >= 0 && bnd < p.src_n && k < nz) ? lsrc[bnd].keep : R(1);
   };
-  auto inject = [&](int m, C &x, int at) {
+  // the soft source on plane m; outer: at this lane's outer neighbour
+  auto inject = [&](int m, C &x, bool outer) {
     const int bnd = m - p.src_k0;
     if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
-    SrcPlane<R> sp;
-    sp.band = true;
-    sp.keep = lsrc[bnd].keep;
-    sp.c = cmul<R>(lsrc[bnd].c, ph);
-    sp.cq = sp.c;
-    x = inj<R>(x, sp, sp.c, prof, at);
+    const C c = cmul<R>(lsrc[bnd].c, ph);
+    const C inc = has_prof ? cmul<R>(c, lprof[outer ? 1 : 0][own]) : c;
+    x = caxpy<R>(cscale<R>(x, lsrc[bnd].keep), R(1), inc);
   };
   // forward H (level weights of plane k) and adjoint c M^H (weights of k-1, k, k+1)
   auto hfwd = [&](int k, R v, C y, C sx, C sy, C zp, C zm) {
@@ -1688,7 +1697,16 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     sx = cadd<R>(L[ss + 1], L[ss - 1]);
     sy = cadd<R>(L[ss + pw], L[ss - pw]);
   };
-  auto mask = [&](int m, C x) { return interior(m) ? x : zero; };
+  // x on interior planes, else 0, selected by value.  This kernel names no shared zero
+  // object: a select between x and such an object compiled to a select of their
+  // addresses, which kept the zero in scratch memory and put a scratch load, and with it
+  // a full memory-counter wait, in every plane.
+  auto mask = [&](int m, C x) {
+    const bool in = interior(m);
+    x.x = in ? x.x : R(0);
+    x.y = in ? x.y : R(0);
+    return x;
+  };
 
   struct Raw {  // one plane of this lane: the two level-1 inputs, own and outer, V and
     C f, of, a, oa, b;  // (pass II) the base b
@@ -1723,10 +1741,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // b = inject(psi_n): pass I's forward input (own and outer neighbour), pass II's base
   auto prepare = [&](int m, Raw &w) {
     if (PASS == 1) {
-      inject(m, w.f, ij);
-      if (has_o) inject(m, w.of, oo);
+      inject(m, w.f, false);
+      if (has_o) inject(m, w.of, true);
     } else {
-      inject(m, w.b, ij);
+      inject(m, w.b, false);
     }
   };
   // level-1 inputs of plane m into LDS buffer bb (the adjoint one masked to the interior)
@@ -1753,7 +1771,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 
   // level-1 outputs of this column at m-1 and m-2: forward (I: u4; II: u2), adjoint
   // (I: v2; II: v4), unmasked
-  C fprev = zero, fmm = zero, aprev = zero, amm = zero;
+  C fprev = mkc<R>(R(0), R(0)), fmm = mkc<R>(R(0), R(0)), aprev = mkc<R>(R(0), R(0)), amm = mkc<R>(R(0), R(0));
 
   auto plane_step = [&](const int m, Raw &ra, Raw &rb, Raw &rc, Raw &pre, Pt &qk, Pt &qn) {
     const int rd = m & 1, wr = rd ^ 1;
@@ -1780,7 +1798,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     // every wave issues the stores (skipped off the writer lanes): with the same
     // vector-memory count on every wave the load-counter waits stay exact (1.5 % faster
     // than stores inside the branch)
-    C u3 = zero, v3 = zero, sk = zero, ln = zero;
+    C u3 = mkc<R>(R(0), R(0)), v3 = mkc<R>(R(0), R(0)), sk = mkc<R>(R(0), R(0)), ln = mkc<R>(R(0), R(0));
     R g = R(0);
     if (wave_writes) {
       const bool ik = interior(k);
