@@ -132,6 +132,12 @@ int wap_set_stream(wap_handle *h, void *stream);
  * pointer is device memory.  Must be called before wap_step. */
 int wap_set_potential(wap_handle *h, const void *v, int dtype, int on_device);
 
+/* on_device of wap_set_state / wap_get_state: 0 host memory, 1 device memory (the call
+ * returns when the copy is done), WAP_STREAM_ORDERED device memory copied on the
+ * handle's stream without waiting: the caller keeps the memory alive and orders its own
+ * accesses after the handle's stream (checkpoints that only this handle reads back). */
+#define WAP_STREAM_ORDERED 2
+
 /* Set the state (replaces the zero init of fdtd_3d.cu:322-326).  psi (and pi for
  * Klein-Gordon; NULL means 0) are complex (nz,ny,nx) of dtype WAP_DT_C64 or C128.
  * psi == NULL resets the whole state to zero. */

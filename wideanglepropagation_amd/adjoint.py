@@ -202,8 +202,8 @@ class ExitWave(torch.autograd.Function):
                     sim.step(e - n)
                     n = e
                 sample(n)
-                if n in starts[:-1]:
-                    ckpt[n] = sim.get_state(cdt, device=True)
+                if n in starts[:-1]:   # stream-ordered: no host wait per checkpoint
+                    ckpt[n] = sim.get_state(cdt, device=True, stream_ordered=True)
             if keep:   # the last segment: each step recorded for the backward
                 adj = _Adjoint(sim, nsteps - last)
                 for m in range(last, nsteps):
@@ -252,11 +252,14 @@ class ExitWave(torch.autograd.Function):
             for i in reversed(range(len(starts))):
                 start, end = starts[i], ends[i]
                 if i < len(starts) - 1:   # recompute (the last segment was recorded forward)
+                    # the copy runs on the solver's stream (torch's stream when the
+                    # solver was made), so the checkpoint's memory may go back to torch's
+                    # allocator right away
                     st = ctx.ckpt.pop(start)
                     if isinstance(st, tuple):
-                        sim.set_state(st[0], st[1])
+                        sim.set_state(st[0], st[1], stream_ordered=True)
                     else:
-                        sim.set_state(st)
+                        sim.set_state(st, stream_ordered=True)
                     for m in range(start, end):
                         adj.record(m, m - start)
                 for m in reversed(range(start, end)):

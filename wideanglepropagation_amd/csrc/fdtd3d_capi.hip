@@ -903,8 +903,9 @@ int import_one(wap_handle *h, const void *src, int dtype, int on_device, void *d
     }
   }
   WAP_HIP(hipGetLastError());
-  // the staging buffer is reused by the next import: finish this one first
-  WAP_HIP(hipStreamSynchronize(h->stream));
+  // the staging buffer is reused by the next import, and a device source may be released
+  // by the caller: finish this one first, unless the caller orders that itself
+  if (on_device != WAP_STREAM_ORDERED) WAP_HIP(hipStreamSynchronize(h->stream));
   return WAP_OK;
 }
 
@@ -942,7 +943,7 @@ int export_one(wap_handle *h, void *dst, int dtype, int on_device, const void *s
     if (!on_device)
       WAP_HIP(hipMemcpyAsync(dst, h->staging, h->n * out_b, hipMemcpyDeviceToHost, h->stream));
   }
-  WAP_HIP(hipStreamSynchronize(h->stream));
+  if (on_device != WAP_STREAM_ORDERED) WAP_HIP(hipStreamSynchronize(h->stream));
   return WAP_OK;
 }
 }  // namespace
@@ -2080,8 +2081,10 @@ int adjoint_step_fused(wap_adjoint *ad, int slot) {
 // The fused adjoints add each step's dL/dV into vacc (the handle's real type); every
 // kGradFlush steps vacc is added into the float64 vbar and cleared, so a float32
 // accumulator never sums more than that many steps (a run of 1500 reverse steps would
-// otherwise carry 1500 float32 roundings of a growing partial sum per voxel).
-constexpr int kGradFlush = 16;
+// otherwise carry 1500 float32 roundings of a growing partial sum per voxel).  32 steps
+// bound the partial sum's rounding near 32 ulp while a flush (read vacc, read and write
+// the float64 vbar, clear vacc: 24 B per voxel) costs under 1 % of the reverse sweep.
+constexpr int kGradFlush = 32;
 
 int adjoint_flush(wap_adjoint *a) {
   if (!a->vacc) return WAP_OK;

@@ -2257,21 +2257,24 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     C lns = prev.ps, lnq = prev.pq;
     R g = R(0);
     if (wave_writes) {
-      const bool in_k = interior(k);
-      const R dg = DG ? R(-2) * gam_of(k) : R(0);
-      C sx, sy;
-      lat(ly3b[rd], sx, sy);
-      const C k3 = cadd<R>(apply(prev.dl, ly3b[rd][ss], sx, sy, cadd<R>(cy3, ly3b[wr][ss])),
-                           kg1_damp_t<DG, R>(dg, prev.dv, prev.z3));
-      const C wv4 = caxpy<R>(qm, h, k3);
-      lat(la2b[rd], sx, sy);
-      const C T2 = apply(prev.dl, la2b[rd][ss], sx, sy, cadd<R>(ca2, la2b[wr][ss]));
-      lat(la1b[rd], sx, sy);
-      const C T1 = apply(prev.dl, la1b[rd][ss], sx, sy, cadd<R>(ca1, la1b[wr][ss]));
-      const R kp = keep_of(k);
-      lns = cscale<R>(cadd<R>(prev.ps, cadd<R>(T2, T1)), kp);
-      lnq = cscale<R>(caxpy<R>(prev.pq, in_k ? h2 : R(0), T2), kp);
-      if (in_k) g = prev.g1 + R(2) * (a4m.x * wv4.y - a4m.y * wv4.x);
+      // a chunk's first planes (k < k0) only feed the next ones: no pass-2 there
+      if (k >= k0) {
+        const bool in_k = interior(k);
+        const R dg = DG ? R(-2) * gam_of(k) : R(0);
+        C sx, sy;
+        lat(ly3b[rd], sx, sy);
+        const C k3 = cadd<R>(apply(prev.dl, ly3b[rd][ss], sx, sy, cadd<R>(cy3, ly3b[wr][ss])),
+                             kg1_damp_t<DG, R>(dg, prev.dv, prev.z3));
+        const C wv4 = caxpy<R>(qm, h, k3);
+        lat(la2b[rd], sx, sy);
+        const C T2 = apply(prev.dl, la2b[rd][ss], sx, sy, cadd<R>(ca2, la2b[wr][ss]));
+        lat(la1b[rd], sx, sy);
+        const C T1 = apply(prev.dl, la1b[rd][ss], sx, sy, cadd<R>(ca1, la1b[wr][ss]));
+        const R kp = keep_of(k);
+        lns = cscale<R>(cadd<R>(prev.ps, cadd<R>(T2, T1)), kp);
+        lnq = cscale<R>(caxpy<R>(prev.pq, in_k ? h2 : R(0), T2), kp);
+        if (in_k) g = prev.g1 + R(2) * (a4m.x * wv4.y - a4m.y * wv4.x);
+      }
     } else {
       // the halo waves (exactly the waves that write no pass-2 result): the outer
       // neighbours of plane m+1 into buffer wr, which no lane reads laterally this iteration

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib, make_desc, DT_F32, DT_F64, DT_C64, DT_C128
+from ._lib import check, lib, make_desc, DT_F32, DT_F64, DT_C64, DT_C128, WAP_STREAM_ORDERED
 
 
 def _require_gpu():
@@ -91,7 +91,9 @@ class FDTD3D:
         check(self._L.wap_set_potential(self.handle, ptr, code, dev), "wap_set_potential")
         del keep
 
-    def set_state(self, psi, pi=None):
+    def set_state(self, psi, pi=None, stream_ordered=False):
+        """stream_ordered: device tensors are copied on the solver's stream (torch's stream
+        when the solver was made) without waiting for the copy."""
         n = self.params.voxels
         if psi is None:
             check(self._L.wap_set_state(self.handle, None, None, DT_C128, 0), "wap_set_state")
@@ -105,6 +107,8 @@ class FDTD3D:
                 raise ValueError("psi and pi must have the same dtype and device")
         else:
             kq, pq = None, None
+        if stream_ordered and dev:
+            dev = WAP_STREAM_ORDERED
         check(self._L.wap_set_state(self.handle, pp, pq, code, dev), "wap_set_state")
         del kp, kq
 
@@ -142,8 +146,9 @@ class FDTD3D:
         return self.t0 + n * p.dt
 
     # --------------------------------------------------------------- output
-    def get_state(self, dtype=np.complex128, device=False):
-        """psi (and pi for KG) as numpy arrays (nz,ny,nx) or, device=True, torch tensors."""
+    def get_state(self, dtype=np.complex128, device=False, stream_ordered=False):
+        """psi (and pi for KG) as numpy arrays (nz,ny,nx) or, device=True, torch tensors
+        (stream_ordered: filled on the solver's stream without waiting)."""
         p = self.params
         shape = p.shape
         kg = p.equation == "klein_gordon"
@@ -154,7 +159,8 @@ class FDTD3D:
             code = DT_C128 if tdt == torch.complex128 else DT_C64
             check(self._L.wap_get_state(self.handle, ctypes.c_void_p(psi.data_ptr()),
                                         ctypes.c_void_p(pi.data_ptr()) if kg else None,
-                                        code, 1), "wap_get_state")
+                                        code, WAP_STREAM_ORDERED if stream_ordered else 1),
+                  "wap_get_state")
             return (psi, pi) if kg else psi
         psi = np.empty(shape, dtype=dtype)
         pi = np.empty(shape, dtype=dtype) if kg else None
