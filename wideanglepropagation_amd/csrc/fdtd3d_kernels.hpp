@@ -1930,7 +1930,10 @@ struct KG1AdjArgs {
   int tile_z;
   int ntx, nty, ntz;
   // hz: the z weight rounded once (the gradient does not propagate a systematic
-  // dispersion shift); d0 = 2 (hx + hy + hz) + m^2 as d0_hi + d0_lo
+  // dispersion shift); d0 = 2 (hx + hy + hz) + m^2 as d0_hi + d0_lo, summed with -V^2
+  // into one diagonal coefficient per voxel (d0_hi + (d0_lo - V^2), rounded once: the
+  // float32 gradient's distance to the float64 one moved from 1.412e-5 to 1.414e-5, and
+  // each stencil saves a multiply-add)
   R hx, hy, hz, d0_hi, d0_lo, h;
   int gz_lo, gz_hi;  // gamma is 0 on planes gz_lo..gz_hi (none if gz_lo > gz_hi)
   int src_k0, src_n;
@@ -2107,10 +2110,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     la3b[b][ss] = a3;
     store_halo(dgt, b, m, o);
   };
-  // A x with the operator coefficient dl = d0_lo - V^2 of the center voxel
-  auto apply = [&](R dl, C x, C sx, C sy, C sz) {
-    return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz * sz.x - p.d0_hi * x.x - dl * x.x,
-                  p.hx * sx.y + p.hy * sy.y + p.hz * sz.y - p.d0_hi * x.y - dl * x.y);
+  // A x with the diagonal coefficient dd = d0_hi + (d0_lo - V^2) of the center voxel
+  auto apply = [&](R dd, C x, C sx, C sy, C sz) {
+    return mkc<R>(p.hx * sx.x + p.hy * sy.x + p.hz * sz.x - dd * x.x,
+                  p.hx * sx.y + p.hy * sy.y + p.hz * sz.y - dd * x.y);
   };
   auto lat = [&](const C *L, C &sx, C &sy) {
     sx = cadd<R>(L[ss + 1], L[ss - 1]);
@@ -2135,7 +2138,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     inject(m_start - 1, w.s, w.q, ij);
     C a4, a3;
     point_a(std::true_type{}, m_start - 1, w.ls, w.lq, w.v, a4, a3);
-    const int b = (m_start - 1) & 1;
+    const int b = 1;  // buffer parity relative to m_start: plane m_start - 1 is odd
     lssb[b][ss] = w.s;
     lsqb[b][ss] = w.q;
     la4b[b][ss] = a4;
@@ -2143,15 +2146,15 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     // y3, a2, a1 of plane m_start - 1 (the Y values of plane p live in buffer (p + 1) & 1;
     // read as the z- neighbours of the first stored pass-2 when m_start = 0: the plane
     // below the grid, 0)
-    ly3b[m_start & 1][ss] = zero;
-    la2b[m_start & 1][ss] = zero;
-    la1b[m_start & 1][ss] = zero;
+    ly3b[0][ss] = zero;
+    la2b[0][ss] = zero;
+    la1b[0][ss] = zero;
   }
   inject(m_start, r1.s, r1.q, ij);
   {
     C a4, a3;
     point_a(std::true_type{}, m_start, r1.ls, r1.lq, r1.v, a4, a3);
-    store_plane(std::true_type{}, m_start & 1, m_start, r1.s, r1.q, a4, a3, hn);
+    store_plane(std::true_type{}, 0, m_start, r1.s, r1.q, a4, a3, hn);
     lols[own] = r1.ls;
     lolq[own] = r1.lq;
     lov[own] = r1.v;
@@ -2160,7 +2163,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 
   struct Pass1 {  // pass-1 results of one voxel that pass-2 of the same plane needs
     C z3, ps, pq;   // (its y3, a2, a1, q and a4 are read back from the LDS planes)
-    R dl, dv, g1;
+    R dl, dv, g1;   // dl: the diagonal coefficient dd of apply
   };
   Pass1 p0, p1;
   p0.z3 = p0.ps = p0.pq = zero;
@@ -2174,10 +2177,10 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // (state, a4, a3) and m-2 (y3, a2, a1) until this lane overwrites its slot at the end
   // of the iteration.  prev/cur and vb/vb_next alternate between two copies.
   // DG: gamma may be nonzero on planes m-1..m+1 (else the damping is the V term alone)
-  auto plane_step = [&](auto dgt, const int m, Raw &rc, Raw &pre, const Pass1 &prev, Pass1 &cur,
-                        const R vb, R &vb_next) {
+  auto plane_step = [&](auto dgt, auto par, const int m, Raw &rc, Raw &pre, const Pass1 &prev,
+                        Pass1 &cur, const R vb, R &vb_next) {
     constexpr bool DG = decltype(dgt)::value;
-    const int rd = m & 1, wr = rd ^ 1;
+    constexpr int rd = decltype(par)::value, wr = rd ^ 1;  // (m - m_start) & 1
     // loads, oldest first, so that a wait for one never waits for a younger prefetch: the
     // outer neighbours of plane m+1 (halo waves; stored after this iteration's pass-1) or
     // vbar of the next iteration's pass-2 plane (writer waves), then the prefetch of plane
@@ -2200,7 +2203,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     C cy3, ca2, ca1;  // this plane's y3, a2, a1 (LDS and the z+ neighbours of pass-2)
     const bool in_m = interior(m);
     const R v = lov[own];
-    cur.dl = p.d0_lo - v * v;
+    cur.dl = p.d0_hi + (p.d0_lo - v * v);
     cur.dv = R(-2) * v;
     const C qm = lsqb[wr][ss], a4m = la4b[wr][ss];  // plane m-1 (also pass-2's center)
     {
@@ -2299,17 +2302,21 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     __syncthreads();
   };
 
-  // a plain loop with copies (unrolling the register rings spills at 128 VGPRs; so does
-  // a variant specialised to interior planes, which then runs 2x slower)
-  for (int m = m_start; m <= m_end; ++m) {
+  // unrolled by two: the register rings rotate by renaming and every LDS address is a
+  // constant offset from the lane's slot (the buffer parity is a compile-time value)
+  auto step = [&](auto par, int m, Raw &rc, Raw &pre, const Pass1 &prev, Pass1 &cur, R vb,
+                  R &vb_next) {
     if (m - 1 >= p.gz_lo && m + 1 <= p.gz_hi)  // no sponge on planes m-1..m+1
-      plane_step(std::false_type{}, m, r2, r3, p0, p1, vb0, vb1);
+      plane_step(std::false_type{}, par, m, rc, pre, prev, cur, vb, vb_next);
     else
-      plane_step(std::true_type{}, m, r2, r3, p0, p1, vb0, vb1);
-    r2 = r3;
-    p0 = p1;
-    vb0 = vb1;
+      plane_step(std::true_type{}, par, m, rc, pre, prev, cur, vb, vb_next);
+  };
+  int m = m_start;
+  for (; m + 1 <= m_end; m += 2) {
+    step(std::integral_constant<int, 0>{}, m, r2, r3, p0, p1, vb0, vb1);
+    step(std::integral_constant<int, 1>{}, m + 1, r3, r2, p1, p0, vb1, vb0);
   }
+  if (m <= m_end) step(std::integral_constant<int, 0>{}, m, r2, r3, p0, p1, vb0, vb1);
 }
 
 // lam[(off + t) stride] += conj(c) g[t]  (an exit-plane seed; stride 2 addresses the psi
