@@ -61,6 +61,8 @@ PASS_BYTES = {
     ("schr_fused", "schrodinger", "f32"): (20, 28),
     ("schr_fused", "schrodinger", "f64"): (40, 56),
     ("schr_fused", "schrodinger", "f32_kahan"): (20, 44),  # pass B + the compensation, R/W
+    # the four Horner levels in one pass: read psi_n, V; write psi_{n+1}
+    ("schr_1pass", "schrodinger", "f32"): (20,),
 }
 
 
@@ -104,7 +106,7 @@ def parse():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--schedule", default="auto",
-                    choices=["auto", "stages", "kg_2pass", "kg_fused", "schr_fused"])
+                    choices=["auto", "stages", "kg_2pass", "kg_fused", "schr_fused", "schr_1pass"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration)")
     ap.add_argument("--adjoint-steps", type=int, default=16,
@@ -281,9 +283,8 @@ def main():
     from wideanglepropagation_amd._lib import stream_bandwidth
     copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
 
-    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass",
+    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass", "schr_1pass": "schr1_pass",
               "schr_fused": f"schr2_pass_p{dom + 1}"}.get(sched, f"rk4_pass_s{dom + 1}")
-    # the Schrodinger summary covers both passes and quotes pass B's bytes
     # the Schrodinger summary covers both passes and quotes pass B's bytes
     traffic, traffic_src = (profiled_traffic("schr2_pass", p) if sched == "schr_fused" and dom == 1
                             else profiled_traffic(kernel, p))
@@ -313,7 +314,7 @@ def main():
                      "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
-                                "kg_fused": "kg1_pass",
+                                "kg_fused": "kg1_pass", "schr_1pass": "schr1_pass",
                                 "schr_fused": f"schr2_pass pass {'AB'[dom]}"}.get(
                                     sched, f"rk4_pass stage {dom + 1}"),
                      "basis": f"algorithmic {pb[dom]} B per voxel of the executed {sched} "

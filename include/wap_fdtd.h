@@ -59,10 +59,11 @@ extern "C" {
 #define WAP_TIME_F32 1    /* t = float((float)(step0 + n) * (float)dt), as fdtd_3d.cu:365;
                              t0 must be 0 (WAP_ERR_ARG otherwise) */
 
-/* pass schedule of a Klein-Gordon step (Schrodinger always uses 4 stage passes) */
+/* pass schedule of a step */
 #define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a field is under 2 GiB,
-                               * else KG_2PASS; Schrodinger: SCHR_FUSED for f32 (3D), else
-                               * STAGES */
+                               * else KG_2PASS; Schrodinger with frozen end planes and fields
+                               * under 2 GiB: SCHR_1PASS in f32, SCHR_FUSED in f64 and
+                               * f32_kahan; else STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
 #define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state;
@@ -70,6 +71,8 @@ extern "C" {
 #define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger f32: the RK4 polynomial in Horner form, two passes
                                  * of two stencil levels each (AUTO picks it for f32 with
                                  * frozen end planes) */
+#define WAP_SCHED_SCHR_1PASS 5  /* Schrodinger f32 (not f32_kahan), frozen end planes: the four
+                                 * Horner levels in one pass (20 B per voxel-step) */
 
 /* planes k = 0 and nz-1 (never stepped) */
 #define WAP_ENDS_FROZEN 0  /* stage inputs there = the injected state (fdtd_3d.cu:184-190) */

@@ -206,7 +206,7 @@ def test_schr_fused_adjoint_close_to_f64(rng):
     print("case schr fused adjoint f32 vs f64", flush=True)
     p64, Vn = _setup("schrodinger", "f64", rng)
     p32 = p64.with_(precision="f32")
-    assert p32.resolved_schedule() == "schr_fused"
+    assert p32.resolved_schedule() == "schr_1pass"   # the records run the one-pass step
     nsteps, k_exit = 25, p64.nz - 5
     samples = {25: 1.0, 18: 0.5}
     grads = []

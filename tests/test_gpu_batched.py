@@ -152,7 +152,7 @@ def test_fused_batch_odd_steps_and_odd_graph(eq, rng):
     from wideanglepropagation_amd.batched import BatchedFDTD
     print(f"case fused batch odd steps {eq}", flush=True)
     cases = _cases(rng, eq, "f32", n=4)
-    assert all(p.resolved_schedule() in ("kg_fused", "schr_fused") for p, _, _ in cases)
+    assert all(p.resolved_schedule() in ("kg_fused", "schr_1pass") for p, _, _ in cases)
     single = _make(cases)
     for s in single:
         s.step(9)
@@ -214,8 +214,10 @@ def test_tune_batch_picks_a_chunk_per_class():
     for c in choice:
         assert all(choice[d][0] == choice[c][0] for d in choice if d[:3] == c[:3])
     assert ms == min(m for _, m in table) and ms > 0
-    # 2 uniform chunks, one alternative waves per group (2), one alternative chunk per class
-    assert len(table) == 2 + 2 + len(choice)
+    # 2 uniform chunks, one alternative waves per group whose shape is tunable (the
+    # one-pass Schrodinger group's is fixed), one alternative chunk per class
+    tunable = {c[:3] for c in choice if c[2] != "schr_1pass"}
+    assert len(table) == 2 + len(tunable) + len(choice)
 
 
 def test_batched_study_quick_runs_finite():

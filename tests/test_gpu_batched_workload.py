@@ -51,7 +51,7 @@ def test_batched_size_classes_against_oracle():
         cases.append((p, V, prof, psi0, pi0))
     sims = [FDTD3D(p, V=V, psi0=psi0, pi0=pi0, source_profile=prof)
             for p, V, prof, psi0, pi0 in cases]
-    assert {s.params.resolved_schedule() for s in sims} == {"schr_fused", "kg_fused"}
+    assert {s.params.resolved_schedule() for s in sims} == {"schr_1pass", "kg_fused"}
     nsteps = 10
     try:
         b = BatchedFDTD(sims)

@@ -175,8 +175,9 @@ def test_kg_fused_is_the_default_in_every_precision():
 
 
 def test_schr_fused_is_the_default_in_every_precision():
+    # float32: the four Horner levels in one pass; float64 and f32_kahan: two passes
     p = _params((24, 16, 16), "schrodinger", precision="f32")
-    assert p.resolved_schedule() == "schr_fused"
+    assert p.resolved_schedule() == "schr_1pass"
     assert p.with_(precision="f64").resolved_schedule() == "schr_fused"
     assert p.with_(precision="f32_kahan").resolved_schedule() == "schr_fused"
     # the 2D rule (zero stage inputs on the end rows) runs one pass per stage
@@ -194,8 +195,7 @@ def test_schr_fused_kahan_compensates(rng):
     V = V.astype(np.float32).astype(np.float64)
     out = {}
     for prec in ("f64", "f32", "f32_kahan"):
-        q = p.with_(precision=prec)
-        assert q.resolved_schedule() == "schr_fused"
+        q = p.with_(precision=prec, schedule="schr_fused")
         sim = FDTD3D(q, V=V)
         sim.set_state(st0)
         sim.step(1000)

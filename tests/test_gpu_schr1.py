@@ -1,0 +1,168 @@
+"""The one-pass Schrodinger schedule (schr1_pass: the four Horner levels of an RK4 step in
+one walk over the planes) against the FP64 CPU oracle and against the two-pass schedule.
+
+Tolerances: relative L2 2e-6 against the oracle after up to 20 steps (float32 arithmetic,
+as test_gpu_parity); against the two-pass float32 schedule, which evaluates the same
+levels with the same arithmetic, 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle import fdtd3d
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.engine import FDTD3D
+    return FDTD3D
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _params(shape, **kw):
+    from wideanglepropagation_amd.physics import schrodinger_params
+    from wideanglepropagation_amd.workloads import centred_extent
+    ext = centred_extent(shape, kw.pop("spacing", 0.05))
+    return schrodinger_params(0.05, ext, shape, pml_thick=kw.pop("pml_thick", 0.15),
+                              k_source=kw.pop("k_source", 4), precision="f32", **kw)
+
+
+def _case(p, rng, vscale=2e3):
+    V = (vscale * rng.standard_normal(p.shape)).astype(np.float32)
+    psi = (rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)) * 0.5
+    return V, psi
+
+
+def _gpu_run(p, V, psi, nsteps, profile=None):
+    FDTD3D = _gpu()
+    sim = FDTD3D(p, V=V, source_profile=profile)
+    sim.set_state(psi)
+    sim.step(nsteps)
+    out = sim.get_state(np.complex128)
+    sim.close()
+    return out
+
+
+def _oracle(p, V, psi, nsteps, profile=None):
+    return fdtd3d.run(psi, V.astype(np.float64), nsteps, p.oracle_dict(src_profile=profile),
+                      f32_time=(p.time_mode == "f32"))
+
+
+# (nz, ny, nx) and z chunk: partial x tiles (70, 125, 130 columns), grids narrower than the
+# tile's 8 interior rows (3, 5), one-plane and short chunks (chunk edges inside the source
+# band and next to the frozen planes), a single chunk (0: the cost model's choice)
+@pytest.mark.parametrize("shape,tz", [((9, 5, 70), 1), ((9, 5, 70), 3), ((11, 13, 130), 100),
+                                      ((7, 3, 2), 0), ((12, 40, 125), 5), ((10, 3, 61), 2),
+                                      ((40, 26, 64), 0), ((33, 17, 64), 7)])
+def test_schr1_matches_oracle_ragged(shape, tz, rng):
+    print(f"case schr_1pass {shape} tz={tz}", flush=True)
+    p = _params(shape, tile_z=tz, pml_thick=0.1, k_source=2, schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 8)
+    want = _oracle(p, V, psi, 8)
+    assert rel_l2(got, want) < 2e-6
+
+
+def test_schr1_matches_oracle_20_steps(rng):
+    print("case schr_1pass 24x16x16 20 steps", flush=True)
+    p = _params((24, 16, 16), schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    assert rel_l2(_gpu_run(p, V, psi, 20), _oracle(p, V, psi, 20)) < 2e-6
+
+
+def test_schr1_source_profile_and_reference_clock(rng):
+    print("case schr_1pass source profile + f32 clock", flush=True)
+    p = _params((20, 12, 16), reference_quantize=True, k_source=5, schedule="schr_1pass")
+    assert p.time_mode == "f32"
+    prof = np.exp(-np.linspace(-1, 1, 16)[None, :] ** 2 - np.linspace(-1, 1, 12)[:, None] ** 2)
+    prof = (prof * np.exp(0.3j)).astype(np.complex128)
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 12, profile=prof)
+    assert rel_l2(got, _oracle(p, V, psi, 12, profile=prof)) < 2e-6
+
+
+@pytest.mark.parametrize("shape,tz", [((40, 48, 96), 0), ((64, 40, 128), 9)])
+def test_schr1_equals_two_pass(shape, tz, rng):
+    # the same Horner levels with the same arithmetic as schr2_pass
+    print(f"case schr_1pass vs schr_fused {shape} tz={tz}", flush=True)
+    p1 = _params(shape, tile_z=tz, schedule="schr_1pass")
+    p2 = p1.with_(schedule="schr_fused", tile_z=0)
+    V, psi = _case(p1, rng)
+    a = _gpu_run(p1, V, psi, 10)
+    b = _gpu_run(p2, V, psi, 10)
+    err = rel_l2(a, b)
+    print(f"rel L2 one pass vs two passes: {err:.3e}", flush=True)
+    assert err < 1e-6
+
+
+def test_schr1_gaussian_64_200_steps():
+    # the 64^3 wavepacket over 200 steps: as close to the FP64 run as the two-pass schedule
+    from wideanglepropagation_amd.workloads import schrodinger_gaussian_64
+    print("case 64^3 wavepacket schr_1pass vs schr_fused vs f64", flush=True)
+    FDTD3D = _gpu()
+    p64, V, psi0, nsteps = schrodinger_gaussian_64("f64")
+    out = {}
+    for name, p in (("f64", p64), ("two", p64.with_(precision="f32", schedule="schr_fused")),
+                    ("one", p64.with_(precision="f32", schedule="schr_1pass"))):
+        sim = FDTD3D(p, V=V.astype(np.float64 if name == "f64" else np.float32))
+        sim.set_state(psi0)
+        sim.step(nsteps)
+        out[name] = sim.get_state(np.complex128)
+        sim.close()
+    e1, e2 = rel_l2(out["one"], out["f64"]), rel_l2(out["two"], out["f64"])
+    print(f"rel L2 vs f64: one pass {e1:.3e}, two passes {e2:.3e}", flush=True)
+    assert e1 < 1e-5 and e1 < 1.5 * e2 + 1e-7
+
+
+def test_schr1_bitwise_deterministic(rng):
+    FDTD3D = _gpu()
+    p = _params((40, 48, 96), schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    outs = []
+    for _ in range(2):
+        sim = FDTD3D(p, V=V)
+        sim.set_state(psi)
+        sim.step(7)
+        outs.append(sim.get_state(np.complex64))
+        sim.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_schr1_rejects_other_precisions_and_end_rule():
+    FDTD3D = _gpu()
+    from wideanglepropagation_amd._lib import WapError
+    p = _params((12, 8, 8), schedule="schr_1pass")
+    for bad in (p.with_(precision="f64"), p.with_(precision="f32_kahan"),
+                p.with_(end_planes="zero")):
+        with pytest.raises(WapError, match="one-pass Schrodinger schedule"):
+            FDTD3D(bad)
+
+
+def test_schr1_adjoint_equals_two_pass_adjoint(rng):
+    # records of the one-pass schedule: the same float32 gradient as with the two-pass
+    # records, to float32 rounding
+    _gpu()
+    from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
+    print("case schr_1pass adjoint", flush=True)
+    shape = (22, 10, 12)
+    p1 = _params(shape, schedule="schr_1pass")
+    V0, psi0 = _case(p1, rng, vscale=300.0)
+    grads, Es = [], []
+    for p in (p1, p1.with_(schedule="schr_fused")):
+        V = torch.tensor(V0, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = exit_wave(V, p, 25, p.nz - 5, samples={25: 1.0, 18: 0.5}, checkpoint_every=6,
+                      psi0=psi0)
+        torch.sum(far_field_intensity(E) ** 2).backward()
+        grads.append(V.grad.detach().double().cpu().numpy())
+        Es.append(E.detach().cpu().numpy())
+    assert np.abs(grads[1]).max() > 0
+    assert rel_l2(Es[0], Es[1]) < 1e-6
+    rel = rel_l2(grads[0], grads[1])
+    print(f"one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
+    assert rel < 1e-4

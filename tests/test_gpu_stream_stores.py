@@ -43,7 +43,7 @@ def test_fused_forward_with_streaming_stores_matches_f64(eq, rng):
     from wideanglepropagation_amd.engine import FDTD3D
     print(f"case stream-store forward {eq} {SHAPE}", flush=True)
     p32, p64 = _params(eq, "f32"), _params(eq, "f64")
-    assert p32.resolved_schedule() in ("kg_fused", "schr_fused")
+    assert p32.resolved_schedule() in ("kg_fused", "schr_1pass")
     V, psi0, pi0 = _inputs(rng, p32)
     out = []
     for p in (p32, p64):

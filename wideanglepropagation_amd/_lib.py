@@ -16,7 +16,7 @@ WAP_OK = 0
 EQ = {"schrodinger": 0, "klein_gordon": 1}
 PREC = {"f64": 0, "f32": 1, "f32_kahan": 2}
 TIME = {"exact": 0, "f32": 1}
-SCHED = {"auto": 0, "stages": 1, "kg_2pass": 2, "kg_fused": 3, "schr_fused": 4}
+SCHED = {"auto": 0, "stages": 1, "kg_2pass": 2, "kg_fused": 3, "schr_fused": 4, "schr_1pass": 5}
 DT_F32, DT_F64, DT_C64, DT_C128 = 0, 1, 2, 3
 WAP_STREAM_ORDERED = 2   # on_device value: device memory, copied on the handle's stream
 

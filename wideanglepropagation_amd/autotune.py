@@ -81,6 +81,8 @@ def tune(params, tile_ys=TILE_Y, tile_zs=TILE_Z, steps=5, warmup=2, use_cache=Tr
     key = _key(params)
     if params.resolved_schedule() in ("kg_fused", "schr_fused") and tile_ys == TILE_Y:
         tile_ys = (4, 8, 16)   # the fused passes need >= 3 rows (one output row + halo)
+    elif params.resolved_schedule() == "schr_1pass":
+        tile_ys = (16,)        # one fixed workgroup shape: only the z chunk is tuned
     cache = load_cache() if use_cache else {}
     if use_cache and key in cache:
         e = cache[key]
@@ -172,7 +174,9 @@ def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, ste
         ms = run(choice, waves)
         if ms < best_ms:
             best, best_ms = choice, ms
-    for g in groups:     # the waves of each launch group
+    for g in groups:     # the waves of each launch group (the one-pass Schrodinger shape is fixed)
+        if g[2] == "schr_1pass":
+            continue
         for ty in [t for t in tile_ys if t != waves[g]]:
             trial = dict(waves)
             trial[g] = ty

@@ -34,8 +34,9 @@ class BatchedFDTD:
         for key, members in groups.items():
             arr = (ctypes.c_void_p * len(members))(*[m.handle.value for m in members])
             b = ctypes.c_void_p()
-            # fused groups keep their handles' workgroup shape (64 x 4, 8 or 16 waves)
-            ty = 0 if key[2] in ("kg_fused", "schr_fused") else int(tile_y)
+            # fused groups keep their handles' workgroup shape (64 x 4, 8 or 16 waves; the
+            # one-pass Schrodinger shape is fixed)
+            ty = 0 if key[2] in ("kg_fused", "schr_fused", "schr_1pass") else int(tile_y)
             check(self._L.wap_batch_create(arr, len(members), ty, ctypes.byref(b)),
                   "wap_batch_create")
             self.groups.append((key, members, b))

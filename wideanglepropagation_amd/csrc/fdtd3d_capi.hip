@@ -386,11 +386,15 @@ bool schr_fused_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
          fused_plane_fits(d);
 }
+// the one-pass Schrodinger schedule: float32 without compensation
+bool schr_1pass_ok(const wap_desc &d) { return schr_fused_ok(d) && d.precision == WAP_PREC_F32; }
 
 int sched_of(const wap_handle *h) {
   if (!is_kg(h)) {
     if (h->d.schedule == WAP_SCHED_AUTO)
-      return schr_fused_ok(h->d) ? WAP_SCHED_SCHR_FUSED : WAP_SCHED_STAGES;
+      return schr_1pass_ok(h->d)   ? WAP_SCHED_SCHR_1PASS
+             : schr_fused_ok(h->d) ? WAP_SCHED_SCHR_FUSED
+                                   : WAP_SCHED_STAGES;
     return h->d.schedule;
   }
   if (h->d.schedule == WAP_SCHED_AUTO)
@@ -419,28 +423,30 @@ int kg1_nty(const wap_handle *h, int waves) {
 int kg1_nty(const wap_handle *h) { return kg1_nty(h, kg1_waves(h)); }
 
 // The fused passes hold per_cu workgroups per CU on 256 CUs.  Pick the z chunk
-// that minimises (full rounds of workgroups) x (planes per chunk + 2 halo planes):
+// that minimises (full rounds of workgroups) x (planes per chunk + halo planes):
 // few, long chunks amortise the halo planes, but a last round that is mostly empty
 // costs a whole round.
-int kg1_tile_z(const wap_handle *h, int waves, int per_cu) {
+int chunk_tile_z(const wap_handle *h, long cols, int per_cu, int halo) {
   if (h->d.tile_z > 0) {
     const int tz = h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
     return tz < wap::kKG1MaxTileZ ? tz : wap::kKG1MaxTileZ;
   }
-  const long cols = (long)kg1_ntx(h) * kg1_nty(h, waves);
   const long slots = 256L * per_cu;
   const int tmax = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
   int best = tmax;
   double best_cost = 1e300;
   for (int tz = 8 < tmax ? 8 : tmax; tz <= tmax; ++tz) {
     const long chunks = (h->d.nz + tz - 1) / tz;
-    const double cost = (double)((cols * chunks + slots - 1) / slots) * (tz + 2);
+    const double cost = (double)((cols * chunks + slots - 1) / slots) * (tz + halo);
     if (cost < best_cost) {  // ties keep the shorter chunk (more workgroups in flight)
       best_cost = cost;
       best = tz;
     }
   }
   return best;
+}
+int kg1_tile_z(const wap_handle *h, int waves, int per_cu) {
+  return chunk_tile_z(h, (long)kg1_ntx(h) * kg1_nty(h, waves), per_cu, 2);
 }
 int kg1_tile_z(const wap_handle *h) { return kg1_tile_z(h, kg1_waves(h), 16 / kg1_waves(h)); }
 
@@ -596,6 +602,74 @@ int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
   return WAP_OK;
 }
 
+// One-pass Schrodinger step (schr1_pass): kSchr1Rows row waves (8 interior rows of 64
+// columns) and a side wave per workgroup, kSchr1PerCu workgroups per CU.  A chunk of tz
+// planes evaluates 4 tz + 12 levels (the z halo of levels 1..3), which the chunk choice
+// counts as 3 extra planes.
+int schr1_nty(const wap_handle *h) {
+  constexpr int out_rows = wap::kSchr1Rows - 6;
+  return (h->d.ny + out_rows - 1) / out_rows;
+}
+int schr1_tile_z(const wap_handle *h) {
+  return chunk_tile_z(h, (long)kg1_ntx(h) * schr1_nty(h), wap::kSchr1PerCu, 3);
+}
+int64_t schr1_tiles(const wap_handle *h) {
+  const int tz = schr1_tile_z(h);
+  return (int64_t)kg1_ntx(h) * schr1_nty(h) * ((h->d.nz + tz - 1) / tz);
+}
+
+template <typename R>
+wap::Schr1Args<R> schr1_args(const wap_handle *h) {
+  using C = wap::cplx<R>;
+  wap::Schr1Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  const int tz = schr1_tile_z(h);
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = kg1_ntx(h); a.nty = schr1_nty(h); a.ntz = (h->d.nz + tz - 1) / tz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  const double dt = h->d.dt;
+  a.f1 = (R)(dt / 4.0);
+  a.f2 = (R)(dt / 3.0);
+  a.f3 = (R)(dt / 2.0);
+  a.f4 = (R)dt;
+  // the state ping-pongs between psi and yb: input, output and V per step
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
+  return a;
+}
+
+// Schrodinger step in one fused pass: psi -> psi' (yb), then the two buffers swap.
+// record: non-temporal result stores (an adjoint record step, whose output waits in its slot)
+template <typename R>
+int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
+                    double *pass_ms, bool record = false) {
+  using C = wap::cplx<R>;
+  const dim3 block(wap::kWaveX, wap::kSchr1Rows + 1);
+  const dim3 grid((unsigned)schr1_tiles(h));
+  wap::Schr1Args<R> a = schr1_args<R>(h);
+  if (record) a.stream_out = 1;
+  for (int64_t n = 0; n < nsteps; ++n) {
+    fill_source(h, step_time(h, step0, n, t0), a);
+    a.in = (const C *)h->psi;
+    a.out = (C *)h->yb_psi;
+    if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
+    hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
+    std::swap(h->psi, h->yb_psi);
+    if (ev) {
+      WAP_HIP(hipEventRecord(ev[1], h->stream));
+      WAP_HIP(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      WAP_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      pass_ms[0] += ms;
+    }
+  }
+  WAP_HIP(hipGetLastError());
+  return WAP_OK;
+}
+
 template <typename R>
 wap::KG2Args<R> kg2_args(const wap_handle *h, int ty = 0) {
   using C = wap::cplx<R>;
@@ -665,6 +739,7 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
+  if (sched_of(h) == WAP_SCHED_SCHR_1PASS) return run_steps_schr1<float>(h, nsteps, step0, t0, ev, pass_ms);
   if (sched_of(h) == WAP_SCHED_SCHR_FUSED) {
     switch (h->d.precision) {
       case WAP_PREC_F64: return run_steps_schr2<double, false>(h, nsteps, step0, t0, ev, pass_ms);
@@ -748,8 +823,11 @@ int validate(const wap_desc *d) {
   if (d->tile_y < 0 || d->tile_y > 16 || (d->tile_y & (d->tile_y - 1)))
     return fail(WAP_ERR_ARG, "tile_y must be 0 (auto) or 1, 2, 4, 8, 16");
   if (d->tile_z < 0) return fail(WAP_ERR_ARG, "tile_z must be >= 0");
-  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_SCHR_FUSED)
+  if (d->schedule < WAP_SCHED_AUTO || d->schedule > WAP_SCHED_SCHR_1PASS)
     return fail(WAP_ERR_ARG, "unknown schedule");
+  if (d->schedule == WAP_SCHED_SCHR_1PASS && !schr_1pass_ok(*d))
+    return fail(WAP_ERR_ARG, "the one-pass Schrodinger schedule needs the Schrodinger equation "
+                             "in f32, frozen end planes and fields under 2 GiB");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
     return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
                              "frozen end planes and fields under 2 GiB");
@@ -1123,6 +1201,12 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
+  if (sched_of(h) == WAP_SCHED_SCHR_1PASS) {
+    if (tile_y) *tile_y = wap::kSchr1Rows + 1;
+    if (tile_z) *tile_z = schr1_tile_z(h);
+    if (groups) *groups = schr1_tiles(h);
+    return WAP_OK;
+  }
   if (sched_of(h) == WAP_SCHED_KG_FUSED || sched_of(h) == WAP_SCHED_SCHR_FUSED) {
     const int tz = kg1_tile_z(h);
     if (tile_y) *tile_y = kg1_waves(h);
@@ -1192,7 +1276,8 @@ void timeless_source(const wap_handle *h, A &a) {
 // (Re)write the per-instance arguments into the batch's device arrays (the
 // arrays themselves never move: a captured graph keeps pointing at them).
 bool batch_fused(const wap_batch *b) {
-  return b->sched == WAP_SCHED_KG_FUSED || b->sched == WAP_SCHED_SCHR_FUSED;
+  return b->sched == WAP_SCHED_KG_FUSED || b->sched == WAP_SCHED_SCHR_FUSED ||
+         b->sched == WAP_SCHED_SCHR_1PASS;
 }
 
 // Fused batches ping-pong every instance's state: step t of a launch sequence uses the
@@ -1216,6 +1301,17 @@ int batch_upload_fused(wap_batch *b) {
         a[i].in_pi = (const C *)(par ? h->ya_pi : h->pi);
         a[i].out_psi = (C *)(par ? h->psi : h->ya_psi);
         a[i].out_pi = (C *)(par ? h->pi : h->ya_pi);
+        a[i].stream_out = stream;
+      }
+      WAP_HIP(hipMemcpy(b->d_args[par], a.data(), n * sizeof(a[0]), hipMemcpyHostToDevice));
+    } else if (b->sched == WAP_SCHED_SCHR_1PASS) {
+      std::vector<wap::Schr1Args<R>> a(n);
+      for (int i = 0; i < n; ++i) {
+        const wap_handle *h = b->hs[i];
+        a[i] = schr1_args<R>(h);
+        timeless_source(h, a[i]);
+        a[i].in = (const C *)(par ? h->yb_psi : h->psi);
+        a[i].out = (C *)(par ? h->psi : h->yb_psi);
         a[i].stream_out = stream;
       }
       WAP_HIP(hipMemcpy(b->d_args[par], a.data(), n * sizeof(a[0]), hipMemcpyHostToDevice));
@@ -1293,7 +1389,9 @@ int batch_build(wap_batch *b) {
   for (int i = 0; i < n; ++i) {
     const wap_handle *h = b->hs[i];
     const int nty = (h->d.ny + b->tile_y - 1) / b->tile_y;
-    const int64_t t = batch_fused(b) ? fused_tiles(h, b->waves) : (int64_t)h->ntx * nty * h->ntz;
+    const int64_t t = b->sched == WAP_SCHED_SCHR_1PASS ? schr1_tiles(h)
+                      : batch_fused(b)                   ? fused_tiles(h, b->waves)
+                                                         : (int64_t)h->ntx * nty * h->ntz;
     if (begin[i] + t > INT32_MAX) return fail(WAP_ERR_ARG, "too many tiles in one batch");
     begin[i + 1] = begin[i] + (int)t;
   }
@@ -1308,6 +1406,9 @@ int batch_build(wap_batch *b) {
   } else if (b->sched == WAP_SCHED_SCHR_FUSED) {
     one = sizeof(wap::Schr2Args<R>);
     nargs = 4;
+  } else if (b->sched == WAP_SCHED_SCHR_1PASS) {
+    one = sizeof(wap::Schr1Args<R>);
+    nargs = 2;
   }
   for (int s = 0; s < nargs; ++s) WAP_HIP(hipMalloc(&b->d_args[s], n * one));
   return batch_upload<R>(b);
@@ -1372,6 +1473,15 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
       case 4: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 3, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
       case 8: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 7, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
       default: hipLaunchKernelGGL((wap::kg1_pass_batched<R, 15, false>), grid, block, 0, b->stream, a, b->d_tile_begin, n, ph); break;
+    }
+    return;
+  }
+  if (b->sched == WAP_SCHED_SCHR_1PASS) {
+    if constexpr (sizeof(R) == 4) {
+      const wap::Schr1Args<R> *a = (const wap::Schr1Args<R> *)b->d_args[par];
+      hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows>), grid,
+                         dim3(wap::kWaveX, wap::kSchr1Rows + 1), 0, b->stream, a,
+                         b->d_tile_begin, n, ph);
     }
     return;
   }
@@ -1468,7 +1578,8 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
     if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set on a batched handle");
   }
   const int s0 = sched_of(h0);
-  const bool fused = s0 == WAP_SCHED_KG_FUSED || s0 == WAP_SCHED_SCHR_FUSED;
+  const bool fused = s0 == WAP_SCHED_KG_FUSED || s0 == WAP_SCHED_SCHR_FUSED ||
+                     s0 == WAP_SCHED_SCHR_1PASS;
   if (fused ? (tile_y != 0 && tile_y < 4) : tile_y > 8)
     return fail(WAP_ERR_ARG, fused ? "fused batches take tile_y 0 (auto), 4, 8 or 16"
                                    : "tile_y must be 0 (auto) or 1, 2, 4, 8");
@@ -1479,6 +1590,7 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
   b->kg2 = kg_two_pass(h0) && !fused;
   b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
   if (is_f64(h0) && is_kg(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
+  if (s0 == WAP_SCHED_SCHR_1PASS) b->waves = wap::kSchr1Rows + 1;  // one fixed shape
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;
@@ -2002,8 +2114,10 @@ int adjoint_record_fused(wap_adjoint *ad, int64_t step, double t0, int slot) {
   void **sp = &ad->st[(size_t)slot * ad->ncomp];
   int rc;
   if (ad->schr) {
-    // the 2-pass step leaves psi_n in the buffer its output swapped with
-    rc = run_steps_schr2<float, false>(h, 1, step, t0, nullptr, nullptr);
+    // the fused steps leave psi_n in the buffer their output swapped with (yb)
+    rc = sched_of(h) == WAP_SCHED_SCHR_1PASS
+             ? run_steps_schr1<float>(h, 1, step, t0, nullptr, nullptr, true)
+             : run_steps_schr2<float, false>(h, 1, step, t0, nullptr, nullptr);
     if (rc) return rc;
     std::swap(h->yb_psi, sp[0]);
   } else if (sched_of(h) == WAP_SCHED_KG_FUSED) {
@@ -2167,7 +2281,8 @@ int adjoint_new(wap_handle *h, int slots, wap_adjoint **out) {
   a->slots = slots;
   a->ncomp = is_kg(h) ? 2 : 1;
   // the fused Schrodinger adjoint is float32 (float64 handles reverse stage by stage)
-  a->schr = !is_kg(h) && sched_of(h) == WAP_SCHED_SCHR_FUSED && h->d.precision == WAP_PREC_F32;
+  a->schr = !is_kg(h) && h->d.precision == WAP_PREC_F32 &&
+            (sched_of(h) == WAP_SCHED_SCHR_FUSED || sched_of(h) == WAP_SCHED_SCHR_1PASS);
   a->fused = (is_kg(h) && fused_plane_fits(h->d)) || a->schr;
   if (a->fused) {
     a->st.assign((size_t)slots * a->ncomp, nullptr);

@@ -1098,8 +1098,8 @@ struct Schr2Args {
 };
 
 // H y from the lateral neighbour sums sx, sy and the z neighbours zp, zm
-template <typename R>
-__device__ __forceinline__ cplx<R> schr_h(const Schr2Args<R> &p, R v, cplx<R> wp, cplx<R> wm,
+template <typename R, typename A>
+__device__ __forceinline__ cplx<R> schr_h(const A &p, R v, cplx<R> wp, cplx<R> wm,
                                           cplx<R> y, cplx<R> sx, cplx<R> sy, cplx<R> zp,
                                           cplx<R> zm) {
   const cplx<R> dp = csub<R>(zp, y), dm = csub<R>(y, zm);
@@ -1332,6 +1332,313 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int inst = find_instance(tile_begin, ninst, t);
   const Schr2Args<R> a = args[inst];
   schr2_body<R, ROWS, PB, KAHAN>(a, t - tile_begin[inst], ph[inst]);
+}
+
+// ----------------------------------------- Schrodinger RK4 step in 1 fused pass
+//
+// The four Horner levels of schr2_pass (u4 = b + h/4 M b, u3 = b + h/3 M u4,
+// u2 = b + h/2 M u3, psi' = b + h M u2, b = inject(psi_n)) in one walk over the planes:
+// step m evaluates level 1 at plane m, level 2 at m-1, level 3 at m-2 and level 4 (psi')
+// at m-3.  A level takes its lateral neighbours from the LDS plane the previous level
+// wrote one step earlier and its z neighbours from this lane's registers (the plane above
+// from the level just evaluated in the same step).  HBM bytes per voxel-step (complex64,
+// float32 V): read psi_n and V (12), write psi_{n+1} (8) = 20 B, against 48 B for the two
+// passes of schr2_pass, whose per-level arithmetic (schr_h, horner) this pass repeats.
+//
+// The lateral stencil has 5 points, so level l is needed within Manhattan distance 4 - l
+// of the tile's interior and b within distance 4.  A workgroup is ROWS row waves and one
+// side wave:
+//   row wave r: columns 0..63 of grid row y0 - 3 + r.  Rows 3..ROWS-4 are the interior
+//     and write psi'; a row at distance dy from it runs levels 1..4-dy.  Rows 0 and
+//     ROWS-1 also load b of the row beyond (distance 4).
+//   side wave: on each side, the columns at distance dx = 1, 2, 3 in the rows within
+//     distance 3 - dx (ROWS-2, ROWS-4 and ROWS-6 lanes), and the column dx = 1 in rows 0
+//     and ROWS-1 (distance 4: b only), 3 ROWS - 10 = 32 lanes at ROWS = 14.  The lanes at
+//     distance 3 also load b of the column beyond.  The wave runs levels 1..3 on all its
+//     lanes; what a lane computes beyond its distance is never read by a needed value.
+// Level l is needed on the planes of the chunk (k0..k1-1) grown by 4 - l on each side and
+// clamped to the grid, so a chunk walks m = k0-3 .. k1+2 and each level runs on the steps
+// whose plane lies in its range.  Planes outside the grid are loaded from the nearest end
+// plane; the LDS planes and register rings start at zero, so every value that a frozen
+// plane's zero factor multiplies is finite.
+template <typename R>
+struct Schr1Args {
+  using C = cplx<R>;
+  const C *in;  // psi_n
+  C *out;       // psi_{n+1} (another buffer: neighbouring tiles still read psi_n)
+  const R *V;
+  const C *zc;  // zc[2k], zc[2k+1]: weights of y_{k+1}, y_{k-1} on plane k
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  R hx, hy;
+  R f1, f2, f3, f4;  // level factors h/4, h/3, h/2, h
+  int src_k0, src_n;
+  const SrcCoef<R> *src_tab;
+  C src_ph;
+  int stream_out;  // non-temporal result stores
+};
+
+constexpr int kSchr1Rows = 14;           // row waves (8 interior rows) plus one side wave
+constexpr int kSchr1Pitch = kWaveX + 9;  // LDS row pitch: columns -4 .. 67 and one unused slot
+constexpr int kSchr1Zc = kKG1MaxTileZ + 16;  // z weights of planes k0-7 .. k1+8
+
+template <typename R, int ROWS>
+struct Schr1Lds {
+  static constexpr int nps = kSchr1Pitch * (ROWS + 2);  // rows -1 .. ROWS
+  cplx<R> lb[2][nps];  // b of the plane of level 1
+  cplx<R> l1[2][nps];  // level outputs, each read by the next level one step later
+  cplx<R> l2[2][nps];
+  cplx<R> l3[2][nps];
+  cplx<R> lzc[2 * kSchr1Zc];
+  SrcCoef<R> lsrc[kMaxBand];
+};
+template <typename R, int ROWS>
+__device__ __forceinline__ Schr1Lds<R, ROWS> &schr1_lds() {
+  __shared__ Schr1Lds<R, ROWS> l;
+  return l;
+}
+
+template <typename R, int ROWS, int CPOL>
+__device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile, const cplx<R> ph) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 7 && 3 * ROWS - 10 <= kWaveX / 2, "rows");
+  Schr1Lds<R, ROWS> &L = schr1_lds<R, ROWS>();
+  constexpr int pw = kSchr1Pitch;
+  constexpr int nps = Schr1Lds<R, ROWS>::nps;
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
+  const bool side = wv == ROWS;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  // this lane's row r (0..ROWS-1) and column c (-4..67, relative to the tile); dxo = +-1 when
+  // it also loads b of the column beyond
+  int r, c, dxo = 0;
+  if (!side) {
+    r = wv;
+    c = lx;
+  } else {
+    constexpr int n1 = ROWS - 2, n2 = ROWS - 4, n3 = ROWS - 6;
+    const int t = lx & 31;
+    const bool right = lx >= 32;
+    int dx;
+    if (t < n1) {
+      dx = 1;
+      r = 1 + t;
+    } else if (t < n1 + n2) {
+      dx = 2;
+      r = 2 + (t - n1);
+    } else if (t < n1 + n2 + n3) {
+      dx = 3;
+      r = 3 + (t - n1 - n2);
+    } else {
+      dx = 1;
+      r = (t == n1 + n2 + n3) ? 0 : ROWS - 1;
+    }
+    const int dy = max(max(3 - r, r - (ROWS - 4)), 0);
+    c = right ? kWaveX - 1 + dx : -dx;
+    if (dx + dy == 3) dxo = right ? 1 : -1;
+  }
+  const int gi = tx * kWaveX + c;
+  const int gj = tyi * (ROWS - 6) - 3 + r;
+  const int i = ((gi % nx) + nx) % nx;
+  const int j = ((gj % ny) + ny) % ny;
+  const int ij = j * nx + i;
+  const int ss = (r + 1) * pw + (c + 4);  // LDS slot
+  int oo = -1, hs = ss;                   // the outer b: grid index and LDS slot
+  if (!side) {
+    if (wv == 0) {
+      oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+      hs = ss - pw;
+    } else if (wv == ROWS - 1) {
+      oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+      hs = ss + pw;
+    }
+  } else if (dxo != 0) {
+    oo = j * nx + (((i + dxo) % nx) + nx) % nx;
+    hs = ss + dxo;
+  }
+  const bool has_o = oo >= 0;
+  const bool outer_wave = side || wv == 0 || wv == ROWS - 1;  // uniform
+  // levels this wave runs (uniform): a row at distance dy runs 1..4-dy, the side wave 1..3
+  const int dyw = side ? 1 : max(max(3 - wv, wv - (ROWS - 4)), 0);
+  const int nlev = 4 - dyw;
+  const bool writer_wave = dyw == 0;
+  const bool writer = writer_wave && gi < nx && gj < ny;
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t ow = writer ? oc : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+  const int m_start = k0 - 3, m_end = k1 + 2;
+  const int zbase = k0 - 7;  // plane of lzc[0..1]
+  // the planes each level is needed on
+  const int lo1 = max(k0 - 3, 0), hi1 = min(k1 + 2, nz - 1);
+  const int lo2 = max(k0 - 2, 0), hi2 = min(k1 + 1, nz - 1);
+  const int lo3 = max(k0 - 1, 0), hi3 = min(k1, nz - 1);
+  {
+    const int tid = wv * kWaveX + lx;
+    constexpr int nthr = kWaveX * (ROWS + 1);
+    for (int t = tid; t < nps; t += nthr) {
+      L.lb[0][t] = zero;
+      L.lb[1][t] = zero;
+      L.l1[0][t] = zero;
+      L.l1[1][t] = zero;
+      L.l2[0][t] = zero;
+      L.l2[1][t] = zero;
+      L.l3[0][t] = zero;
+      L.l3[1][t] = zero;
+    }
+    for (int t = tid; t < kSchr1Zc; t += nthr) {
+      const int k = min(max(zbase + t, 0), nz - 1);
+      L.lzc[2 * t] = p.zc[2 * k];
+      L.lzc[2 * t + 1] = p.zc[2 * k + 1];
+    }
+    if (tid < p.src_n) L.lsrc[tid] = p.src_tab[tid];
+  }
+
+  struct Pl {  // one plane of this lane: b (raw until injected), its outer b, V
+    C b, o;
+    R v;
+  };
+  // every lane issues the same vector-memory operations per plane (own psi and V, then
+  // the outer psi on the waves that have outer lanes or a store that writes nothing on
+  // the others), so the in-order counter waits exactly for the plane a step needs
+  auto load_plane = [&](int m, Pl &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    w.b = buf_ld<C>(rin, oc, bc);
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    if (outer_wave) {
+      w.o = buf_ld<C>(rin, ooc, bc);
+    } else {
+      vmem_pad(rout);
+      w.o = zero;
+    }
+  };
+  auto inject_plane = [&](int m, Pl &w) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = L.lsrc[bnd].keep;
+    sp.c = cmul<R>(L.lsrc[bnd].c, ph);
+    sp.cq = sp.c;
+    w.b = inj<R>(w.b, sp, sp.c, prof, ij);
+    if (has_o) w.o = inj<R>(w.o, sp, sp.c, prof, oo);
+  };
+  // one level at plane k: y its input there, zp / zm at k+1 / k-1, Lin the LDS plane of
+  // the input (lateral neighbours); the factor is 0 on the frozen planes and outside the grid
+  auto level = [&](const C *Lin, R fl, int k, C y, C zp, C zm, C base, R v) {
+    const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
+    const int t = k - zbase;
+    const C hyv = schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
+    return horner<R>(base, (k > 0 && k < nz - 1) ? fl : R(0), hyv);
+  };
+
+  // plane m_start + q lives in ring slot q mod 6; levels 1..3 keep their last two planes
+  // (plane k in slot (k - m_start) & 1)
+  Pl pq[6];
+  C l1q[2], l2q[2], l3q[2];
+  for (int q = 0; q < 6; ++q) {
+    pq[q].b = pq[q].o = zero;
+    pq[q].v = R(0);
+  }
+  l1q[0] = l1q[1] = l2q[0] = l2q[1] = l3q[0] = l3q[1] = zero;
+  load_plane(m_start - 1, pq[5]);
+  load_plane(m_start, pq[0]);
+  load_plane(m_start + 1, pq[1]);
+  __syncthreads();  // zeroed LDS planes, lzc, lsrc
+  inject_plane(m_start - 1, pq[5]);
+  inject_plane(m_start, pq[0]);
+  L.lb[0][ss] = pq[0].b;
+  if (has_o) L.lb[0][hs] = pq[0].o;
+  __syncthreads();
+
+  // step m = m_start + 6 q + U: buffer U & 1 of each LDS pair holds b(m), L1(m-1), L2(m-2),
+  // L3(m-3); the other buffer receives b(m+1), L1(m), L2(m-1), L3(m-2)
+  auto step = [&](auto uc, const int m) {
+    constexpr int U = decltype(uc)::value;
+    constexpr int rd = U & 1, wr = rd ^ 1;
+    Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
+    load_plane(m + 2, pp2);
+    inject_plane(m + 1, pp1);
+    C n1 = zero, n2 = zero, n3 = zero;
+    if (m >= lo1 && m <= hi1) n1 = level(L.lb[rd], p.f1, m, p0.b, pp1.b, pm1.b, p0.b, p0.v);
+    if (nlev >= 2 && m - 1 >= lo2 && m - 1 <= hi2)
+      n2 = level(L.l1[rd], p.f2, m - 1, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
+    if (nlev >= 3 && m - 2 >= lo3 && m - 2 <= hi3)
+      n3 = level(L.l2[rd], p.f3, m - 2, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
+    const int k4 = m - 3;
+    const bool st = writer_wave && k4 >= k0 && k4 < k1;  // uniform
+    C o = zero;
+    if (st) o = level(L.l3[rd], p.f4, k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
+    L.lb[wr][ss] = pp1.b;
+    if (has_o) L.lb[wr][hs] = pp1.o;
+    L.l1[wr][ss] = n1;
+    if (nlev >= 2) L.l2[wr][ss] = n2;
+    if (nlev >= 3) L.l3[wr][ss] = n3;
+    l1q[U & 1] = n1;
+    l2q[(U + 1) & 1] = n2;
+    l3q[U & 1] = n3;
+    __syncthreads();
+  };
+  for (int m0 = m_start; m0 <= m_end; m0 += 6) {
+    step(std::integral_constant<int, 0>{}, m0);
+    if (m0 + 1 > m_end) break;
+    step(std::integral_constant<int, 1>{}, m0 + 1);
+    if (m0 + 2 > m_end) break;
+    step(std::integral_constant<int, 2>{}, m0 + 2);
+    if (m0 + 3 > m_end) break;
+    step(std::integral_constant<int, 3>{}, m0 + 3);
+    if (m0 + 4 > m_end) break;
+    step(std::integral_constant<int, 4>{}, m0 + 4);
+    if (m0 + 5 > m_end) break;
+    step(std::integral_constant<int, 5>{}, m0 + 5);
+  }
+}
+
+// two 15-wave workgroups per CU (79 KB of LDS each): the body is held to 64 VGPRs, which
+// spills the source profile's address (reloaded on band planes only) and two LDS addresses
+// of the level-4 path; at 74 VGPRs, one workgroup per CU, the pass is 7-12 % slower on grids
+// of 33-67 M voxels (5 % faster at 128 x 128 x 256, where few workgroups are in flight).
+// Reading each level's centre back from LDS instead of keeping it in registers (6 VGPRs
+// fewer) measured 1-2 % slower.
+constexpr int kSchr1WavesPerEU = 8;
+constexpr int kSchr1PerCu = kSchr1WavesPerEU >= 8 ? 2 : 1;
+
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass(
+    const Schr1Args<R> p) {
+  const int tile = zgroup_tile<kSchr1PerCu * kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz),
+                                                      p.ntx * p.nty, p.ntz);
+  if (p.stream_out)
+    schr1_body<R, ROWS, kStream>(p, tile, p.src_ph);
+  else
+    schr1_body<R, ROWS, kCached>(p, tile, p.src_ph);
+}
+
+template <typename R, int ROWS>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass_batched(
+    const Schr1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
+    const cplx<R> *__restrict__ ph) {
+  const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);
+  const int inst = find_instance(tile_begin, ninst, t);
+  const Schr1Args<R> a = args[inst];
+  if (a.stream_out)
+    schr1_body<R, ROWS, kStream>(a, t - tile_begin[inst], ph[inst]);
+  else
+    schr1_body<R, ROWS, kCached>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ------------------------------------------------------------------ adjoint

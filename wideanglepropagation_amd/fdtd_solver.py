@@ -32,7 +32,7 @@ What is new (only through keyword arguments)
   when that exceeds 1e-9; returned as "exit_wave", and "exit_incident"),
   backend ("hip" | "cli": run the
   ``fdtd_3d_bin`` command through the file contract), return_timing,
-  schedule ("auto" | "stages" | "schr_fused" | "kg_2pass" | "kg_fused": how a step
+  schedule ("auto" | "stages" | "schr_fused" | "schr_1pass" | "kg_2pass" | "kg_fused": how a step
   is split into passes; "auto" fuses where the precision allows), tile_y, tile_z.
 
 Deviations (documented): AU mode computes P_INF from the wavelength instead

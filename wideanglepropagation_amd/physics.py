@@ -30,7 +30,7 @@ PML_THETA_DEFAULT = 0.785
 PML_POWER_DEFAULT = 3.0
 
 EQUATIONS = ("schrodinger", "klein_gordon")
-SCHEDULES = ("auto", "stages", "kg_2pass", "kg_fused", "schr_fused")
+SCHEDULES = ("auto", "stages", "kg_2pass", "kg_fused", "schr_fused", "schr_1pass")
 PRECISIONS = ("f64", "f32", "f32_kahan")
 
 
@@ -99,7 +99,7 @@ class FDTDParams:
     src_alpha: float = SRC_ALPHA
     tile_y: int = 0
     tile_z: int = 0
-    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused" | "schr_fused"
+    schedule: str = "auto"            # "auto" | "stages" | "kg_2pass" | "kg_fused" | "schr_fused" | "schr_1pass"
     end_planes: str = "frozen"        # "frozen" (3D reference) | "zero" (2D reference)
 
     def __post_init__(self):
@@ -140,7 +140,9 @@ class FDTDParams:
         fits = self.nx * self.ny * self.nz * (16 if self.precision == "f64" else 8) < 2 ** 31
         if self.equation != "klein_gordon":
             if self.schedule == "auto":
-                return "schr_fused" if self.end_planes == "frozen" and fits else "stages"
+                if self.end_planes != "frozen" or not fits:
+                    return "stages"
+                return "schr_1pass" if self.precision == "f32" else "schr_fused"
             return self.schedule
         if self.schedule == "auto":
             return "kg_fused" if fits else "kg_2pass"
