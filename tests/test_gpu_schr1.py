@@ -3,7 +3,7 @@ one walk over the planes) against the FP64 CPU oracle and against the two-pass s
 
 Tolerances: relative L2 2e-6 against the oracle after up to 20 steps (float32 arithmetic,
 as test_gpu_parity); against the two-pass float32 schedule, which evaluates the same
-levels with the same arithmetic, 1e-6.
+levels (outside the PML with the z term as a second difference), 1e-6.
 """
 import numpy as np
 import pytest
@@ -67,6 +67,23 @@ def test_schr1_matches_oracle_ragged(shape, tz, rng):
     got = _gpu_run(p, V, psi, 8)
     want = _oracle(p, V, psi, 8)
     assert rel_l2(got, want) < 2e-6
+
+
+# the steady-state step (all levels outside the PML, no source plane, no clamped plane)
+# against the general one: a source band in the middle of the steady-state range (it splits
+# the range), a thin and a thick PML, a PML on the end planes only, chunk edges inside the
+# range
+@pytest.mark.parametrize("shape,tz,k_source,pml", [((60, 20, 64), 0, 30, 0.15),
+                                                   ((60, 20, 64), 16, 29, 0.15),
+                                                   ((70, 12, 64), 0, 8, 0.6),
+                                                   ((50, 9, 40), 13, 3, 0.025)])
+def test_schr1_steady_state_range(shape, tz, k_source, pml, rng):
+    print(f"case schr_1pass steady state {shape} tz={tz} k_source={k_source} pml={pml}",
+          flush=True)
+    p = _params(shape, tile_z=tz, pml_thick=pml, k_source=k_source, schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 10)
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < 2e-6
 
 
 def test_schr1_matches_oracle_20_steps(rng):

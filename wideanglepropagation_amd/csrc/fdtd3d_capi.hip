@@ -69,6 +69,10 @@ struct wap_handle {
   int ntx = 1, nty = 1, ntz = 1;
   double hx = 0, hy = 0, m2 = 0;
   int gz_lo = 1, gz_hi = 0;  // the longest run of planes with gamma = 0 (empty if lo > hi)
+  // Schrodinger: the longest run of planes whose two z weights are one real number hz (in
+  // the stepper's precision), i.e. outside the PML (empty if lo > hi)
+  int pz_lo = 1, pz_hi = 0;
+  double pz_w = 0;
   int nsrc = 0;
   int src_k0 = 0;
   double src_keep[wap::kMaxBand] = {};
@@ -173,6 +177,23 @@ int build_tables(wap_handle *h) {
   for (int k = 1; k + 1 < nz; ++k) {
     zc[2 * k] = scale * a[k] * 0.5 * (a[k] + a[k + 1]) / (dz * dz);
     zc[2 * k + 1] = scale * a[k] * 0.5 * (a[k - 1] + a[k]) / (dz * dz);
+  }
+  {
+    // rounded as the stepper sees them: equal real weights compare equal after rounding
+    auto rd = [&](double x) { return is_f64(h) ? x : (double)(float)x; };
+    h->pz_lo = 1;
+    h->pz_hi = 0;
+    for (int k = 1, run = 0; k + 1 < nz; ++k) {
+      const bool real_eq = zc[2 * k].imag() == 0.0 && zc[2 * k + 1].imag() == 0.0 &&
+                           rd(zc[2 * k].real()) == rd(zc[2 * k + 1].real()) &&
+                           (run == 0 || rd(zc[2 * k].real()) == rd(zc[2 * (k - 1)].real()));
+      run = real_eq ? run + 1 : 0;
+      if (run > h->pz_hi - h->pz_lo + 1) {
+        h->pz_lo = k - run + 1;
+        h->pz_hi = k;
+        h->pz_w = zc[2 * k].real();
+      }
+    }
   }
   std::vector<double> gam(nz, 0.0);
   if (is_kg(h))
@@ -636,6 +657,9 @@ wap::Schr1Args<R> schr1_args(const wap_handle *h) {
   a.f2 = (R)(dt / 3.0);
   a.f3 = (R)(dt / 2.0);
   a.f4 = (R)dt;
+  a.pz_lo = h->pz_lo;
+  a.pz_hi = h->pz_hi;
+  a.hz = (R)h->pz_w;
   // the state ping-pongs between psi and yb: input, output and V per step
   a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
   return a;

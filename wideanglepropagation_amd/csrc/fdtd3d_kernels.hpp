@@ -1109,6 +1109,16 @@ __device__ __forceinline__ cplx<R> schr_h(const A &p, R v, cplx<R> wp, cplx<R> w
                 p.hx * mx + p.hy * my + (wp.x * dp.y + wp.y * dp.x) - (wm.x * dm.y + wm.y * dm.x) + v * y.y);
 }
 
+// H y on a plane whose two z weights are one real number hz (outside the PML): the z term
+// as a second difference, like x and y
+template <typename R, typename A>
+__device__ __forceinline__ cplx<R> schr_h_real(const A &p, R v, R hz, cplx<R> y, cplx<R> sx,
+                                               cplx<R> sy, cplx<R> zp, cplx<R> zm) {
+  const R lx = sx.x - R(2) * y.x, ly = sy.x - R(2) * y.x, lz = (zp.x + zm.x) - R(2) * y.x;
+  const R mx = sx.y - R(2) * y.y, my = sy.y - R(2) * y.y, mz = (zp.y + zm.y) - R(2) * y.y;
+  return mkc<R>(p.hx * lx + p.hy * ly + hz * lz + v * y.x, p.hx * mx + p.hy * my + hz * mz + v * y.y);
+}
+
 // b + f M y = b + f (-i H y)
 template <typename R>
 __device__ __forceinline__ cplx<R> horner(cplx<R> b, R f, cplx<R> hy) {
@@ -1374,6 +1384,9 @@ struct Schr1Args {
   int ntx, nty, ntz;
   R hx, hy;
   R f1, f2, f3, f4;  // level factors h/4, h/3, h/2, h
+  // planes pz_lo..pz_hi (outside the PML) have both z weights equal to the real hz there
+  int pz_lo, pz_hi;
+  R hz;
   int src_k0, src_n;
   const SrcCoef<R> *src_tab;
   C src_ph;
@@ -1539,8 +1552,13 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   // the input (lateral neighbours); the factor is 0 on the frozen planes and outside the grid
   auto level = [&](const C *Lin, R fl, int k, C y, C zp, C zm, C base, R v) {
     const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
-    const int t = k - zbase;
-    const C hyv = schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
+    C hyv;
+    if (k >= p.pz_lo && k <= p.pz_hi) {  // uniform: outside the PML
+      hyv = schr_h_real<R>(p, v, p.hz, y, sx, sy, zp, zm);
+    } else {
+      const int t = k - zbase;
+      hyv = schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
+    }
     return horner<R>(base, (k > 0 && k < nz - 1) ? fl : R(0), hyv);
   };
 
@@ -1565,9 +1583,24 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
 
   // step m = m_start + 6 q + U: buffer U & 1 of each LDS pair holds b(m), L1(m-1), L2(m-2),
   // L3(m-3); the other buffer receives b(m+1), L1(m), L2(m-1), L3(m-2)
+  // the end of a step: b(m+1) and this step's level outputs into the other LDS buffers and
+  // the register rings
+  auto finish = [&](auto uc, Pl &pp1, C n1, C n2, C n3) {
+    constexpr int U = decltype(uc)::value;
+    constexpr int wr = (U & 1) ^ 1;
+    L.lb[wr][ss] = pp1.b;
+    if (has_o) L.lb[wr][hs] = pp1.o;
+    L.l1[wr][ss] = n1;
+    if (nlev >= 2) L.l2[wr][ss] = n2;
+    if (nlev >= 3) L.l3[wr][ss] = n3;
+    l1q[U & 1] = n1;
+    l2q[(U + 1) & 1] = n2;
+    l3q[U & 1] = n3;
+    __syncthreads();
+  };
   auto step = [&](auto uc, const int m) {
     constexpr int U = decltype(uc)::value;
-    constexpr int rd = U & 1, wr = rd ^ 1;
+    constexpr int rd = U & 1;
     Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
     Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
     load_plane(m + 2, pp2);
@@ -1583,38 +1616,88 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     C o = zero;
     if (st) o = level(L.l3[rd], p.f4, k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
     buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
-    L.lb[wr][ss] = pp1.b;
-    if (has_o) L.lb[wr][hs] = pp1.o;
-    L.l1[wr][ss] = n1;
-    if (nlev >= 2) L.l2[wr][ss] = n2;
-    if (nlev >= 3) L.l3[wr][ss] = n3;
-    l1q[U & 1] = n1;
-    l2q[(U + 1) & 1] = n2;
-    l3q[U & 1] = n3;
-    __syncthreads();
+    finish(uc, pp1, n1, n2, n3);
   };
-  for (int m0 = m_start; m0 <= m_end; m0 += 6) {
-    step(std::integral_constant<int, 0>{}, m0);
-    if (m0 + 1 > m_end) break;
-    step(std::integral_constant<int, 1>{}, m0 + 1);
-    if (m0 + 2 > m_end) break;
-    step(std::integral_constant<int, 2>{}, m0 + 2);
-    if (m0 + 3 > m_end) break;
-    step(std::integral_constant<int, 3>{}, m0 + 3);
-    if (m0 + 4 > m_end) break;
-    step(std::integral_constant<int, 4>{}, m0 + 4);
-    if (m0 + 5 > m_end) break;
-    step(std::integral_constant<int, 5>{}, m0 + 5);
+  // A steady-state step: all four levels on planes m-3..m, which lie outside the PML (real,
+  // equal z weights; no frozen plane), no source plane to inject at m+1, plane m+2 inside
+  // the grid.  The same arithmetic as step's real-weight path without its per-step range
+  // tests, clamps and table lookups (the scalar unit issues as many instructions per step as
+  // the vector units otherwise).
+  auto level_fast = [&](const C *Lin, R fl, C y, C zp, C zm, C base, R v) {
+    const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
+    return horner<R>(base, fl, schr_h_real<R>(p, v, p.hz, y, sx, sy, zp, zm));
+  };
+  auto fast_step = [&](auto uc, const int m) {
+    constexpr int U = decltype(uc)::value;
+    constexpr int rd = U & 1;
+    Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
+    {
+      const uint32_t b = (uint32_t)(m + 2) * plane;
+      const uint32_t bc = b * (uint32_t)sizeof(C);
+      pp2.b = buf_ld<C>(rin, oc, bc);
+      pp2.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+      if (outer_wave) {
+        pp2.o = buf_ld<C>(rin, ooc, bc);
+      } else {
+        vmem_pad(rout);
+        pp2.o = zero;
+      }
+    }
+    const C n1 = level_fast(L.lb[rd], p.f1, p0.b, pp1.b, pm1.b, p0.b, p0.v);
+    C n2 = zero, n3 = zero, o = zero;
+    if (nlev >= 2) n2 = level_fast(L.l1[rd], p.f2, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
+    if (nlev >= 3) n3 = level_fast(L.l2[rd], p.f3, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
+    if (writer_wave) o = level_fast(L.l3[rd], p.f4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), o);
+    finish(uc, pp1, n1, n2, n3);
+  };
+  // the steady-state steps (uniform): m in [mf_lo, mf_hi], with m + 1 outside the source
+  // band (the range keeps the longer side of the band when the band cuts it)
+  int mf_lo = max(k0 + 3, p.pz_lo + 3);
+  int mf_hi = min(min(k1 + 2, p.pz_hi), nz - 3);
+  if (p.src_n > 0) {
+    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2;  // steps injecting m + 1
+    if (sb_lo <= mf_hi && sb_hi >= mf_lo) {
+      if (sb_lo - mf_lo >= mf_hi - sb_hi)
+        mf_hi = sb_lo - 1;
+      else
+        mf_lo = sb_hi + 1;
+    }
   }
+  auto general6 = [&](int m0) {
+    step(std::integral_constant<int, 0>{}, m0);
+    if (m0 + 1 > m_end) return;
+    step(std::integral_constant<int, 1>{}, m0 + 1);
+    if (m0 + 2 > m_end) return;
+    step(std::integral_constant<int, 2>{}, m0 + 2);
+    if (m0 + 3 > m_end) return;
+    step(std::integral_constant<int, 3>{}, m0 + 3);
+    if (m0 + 4 > m_end) return;
+    step(std::integral_constant<int, 4>{}, m0 + 4);
+    if (m0 + 5 > m_end) return;
+    step(std::integral_constant<int, 5>{}, m0 + 5);
+  };
+  // blocks of six steps from m_start (the ring slot of plane m is (m - m_start) mod 6)
+  int m0 = m_start;
+  for (; m0 <= m_end && m0 < mf_lo; m0 += 6) general6(m0);
+  for (; m0 + 5 <= mf_hi; m0 += 6) {
+    fast_step(std::integral_constant<int, 0>{}, m0);
+    fast_step(std::integral_constant<int, 1>{}, m0 + 1);
+    fast_step(std::integral_constant<int, 2>{}, m0 + 2);
+    fast_step(std::integral_constant<int, 3>{}, m0 + 3);
+    fast_step(std::integral_constant<int, 4>{}, m0 + 4);
+    fast_step(std::integral_constant<int, 5>{}, m0 + 5);
+  }
+  for (; m0 <= m_end; m0 += 6) general6(m0);
 }
 
-// two 15-wave workgroups per CU (79 KB of LDS each): the body is held to 64 VGPRs, which
-// spills the source profile's address (reloaded on band planes only) and two LDS addresses
-// of the level-4 path; at 74 VGPRs, one workgroup per CU, the pass is 7-12 % slower on grids
-// of 33-67 M voxels (5 % faster at 128 x 128 x 256, where few workgroups are in flight).
-// Reading each level's centre back from LDS instead of keeping it in registers (6 VGPRs
-// fewer) measured 1-2 % slower.
-constexpr int kSchr1WavesPerEU = 8;
+// one 15-wave workgroup per CU (77 VGPRs, 79 KB of LDS).  Before the steady-state step the
+// body fit 64 VGPRs with a few spills and two workgroups per CU ran 7-12 % faster than one;
+// with it, held to 64 VGPRs, the body spills ~80 registers and runs 0.66 against 0.42 ms
+// at 512 x 512 x 256 (one workgroup per CU, steady-state step: 161 Gvoxel-steps/s at
+// 256 x 256 x 512, against 132 for two workgroups per CU without it).
+constexpr int kSchr1WavesPerEU = 4;
 constexpr int kSchr1PerCu = kSchr1WavesPerEU >= 8 ? 2 : 1;
 
 template <typename R, int ROWS>
