@@ -2337,7 +2337,7 @@ struct KG1AdjArgs {
 // store instructions made every wait at the loop latch drain them, 6 % of the kernel's
 // time at 512 x 512 x 256).
 template <typename R, int ROWS, bool NT>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void kg1_adj_pass(
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) void kg1_adj_pass(
     const KG1AdjArgs<R> p) {
   using C = cplx<R>;
   static_assert(ROWS >= 3 && 2 * ROWS <= kWaveX, "rows");
