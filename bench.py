@@ -273,7 +273,8 @@ def main():
                    "sqrt_t_plan": {"T": T, "S": S, "recomputed_steps": T - S,
                                    "backward_over_forward": round(
                                        ((T - S) * rec_s + T * rev_s) / (T * step_s), 3)},
-                   "adjoint_kernel": ("schr_adj_pass (2 passes)" if sched == "schr_fused" else
+                   "adjoint_kernel": ("schr_adj1_pass (1 pass)" if sched == "schr_1pass" else
+                                      "schr_adj_pass (2 passes)" if sched == "schr_fused" else
                                       "kg1_adj_pass" if p.equation == "klein_gordon"
                                       and vox * 8 < 2 ** 31 else "adj_pass (4 stage passes)"),
                    "adjoint_Gvoxels": round(vox / rev_s / 1e9, 2)}

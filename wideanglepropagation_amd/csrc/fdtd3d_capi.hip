@@ -2243,11 +2243,49 @@ int adjoint_count_step(wap_adjoint *a) {
   return ++a->since_flush >= kGradFlush ? adjoint_flush(a) : WAP_OK;
 }
 
+// Reverse of one recorded one-pass Schrodinger step: schr_adj1_pass, the tiles of
+// schr1_pass (one workgroup per CU: 135 KB of LDS)
+int adjoint_step_schr1(wap_adjoint *ad, int slot) {
+  using R = float;
+  using C = wap::cplx<R>;
+  wap_handle *h = ad->h;
+  wap::SchrAdj1Args<R> a;
+  std::memset((void *)&a, 0, sizeof(a));
+  const int tz = schr1_tile_z(h);
+  a.rec = (const C *)ad->st[slot];
+  a.lam = (const C *)ad->lam2[ad->cur][0];
+  a.lam_out = (C *)ad->lam2[ad->cur ^ 1][0];
+  a.vbar = (R *)ad->vacc;
+  a.V = (const R *)h->V;
+  a.zc = (const C *)h->zc;
+  a.prof = (const C *)h->prof;
+  a.nx = h->d.nx; a.ny = h->d.ny; a.nz = h->d.nz;
+  a.tile_z = tz;
+  a.ntx = kg1_ntx(h); a.nty = schr1_nty(h); a.ntz = (h->d.nz + tz - 1) / tz;
+  a.hx = (R)h->hx; a.hy = (R)h->hy;
+  a.h = (R)h->d.dt;
+  a.pz_lo = h->pz_lo;
+  a.pz_hi = h->pz_hi;
+  a.hz = (R)h->pz_w;
+  a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
+  fill_source(h, step_time(h, ad->st_step[slot], 0, ad->st_t0[slot]), a);
+  const dim3 block(wap::kWaveX, wap::kSchr1Rows + 1);
+  const dim3 grid((unsigned)schr1_tiles(h));
+  if (a.stream_out)
+    hipLaunchKernelGGL((wap::schr_adj1_pass<R, wap::kSchr1Rows, true>), grid, block, 0, h->stream, a);
+  else
+    hipLaunchKernelGGL((wap::schr_adj1_pass<R, wap::kSchr1Rows, false>), grid, block, 0, h->stream, a);
+  WAP_HIP(hipGetLastError());
+  ad->cur ^= 1;
+  return WAP_OK;
+}
+
 // Reverse of one recorded Schrodinger 2-pass step: schr_adj_pass I then II.
 int adjoint_step_schr(wap_adjoint *ad, int slot) {
   using R = float;
   using C = wap::cplx<R>;
   wap_handle *h = ad->h;
+  if (sched_of(h) == WAP_SCHED_SCHR_1PASS) return adjoint_step_schr1(ad, slot);
   const int waves = kg1_waves(h);
   const int tz = kg1_tile_z(h, waves, 16 / waves);
   wap::SchrAdjArgs<R> a;

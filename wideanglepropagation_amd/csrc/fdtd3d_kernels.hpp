@@ -1413,23 +1413,22 @@ __device__ __forceinline__ Schr1Lds<R, ROWS> &schr1_lds() {
   return l;
 }
 
-template <typename R, int ROWS, int CPOL>
-__device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile, const cplx<R> ph) {
-  using C = cplx<R>;
-  static_assert(ROWS >= 7 && 3 * ROWS - 10 <= kWaveX / 2, "rows");
-  Schr1Lds<R, ROWS> &L = schr1_lds<R, ROWS>();
+// The position of this lane in a one-pass Schrodinger workgroup (schr1_body's layout):
+// row r (0..ROWS-1) and column c (-4..67, relative to the tile); ij its grid column, ss its
+// LDS slot (pitch kSchr1Pitch); oo / hs the grid column and LDS slot of the outer value it
+// also loads (rows 0 and ROWS-1: the row beyond; side lanes at distance 3: the column
+// beyond; -1 / ss if none); dyw the wave's row distance from the interior (uniform; 1 for
+// the side wave); writer: an interior-row lane inside the grid.
+struct Schr1Lane {
+  int ij, ss, oo, hs, dyw;
+  bool has_o, outer_wave, writer;
+};
+template <int ROWS>
+__device__ __forceinline__ Schr1Lane schr1_lane(int tx, int tyi, int nx, int ny) {
   constexpr int pw = kSchr1Pitch;
-  constexpr int nps = Schr1Lds<R, ROWS>::nps;
-  const int tx = tile % p.ntx;
-  const int rest = tile / p.ntx;
-  const int tyi = rest % p.nty;
-  const int tz = rest / p.nty;
   const int lx = threadIdx.x;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
   const bool side = wv == ROWS;
-  const int nx = p.nx, ny = p.ny, nz = p.nz;
-  // this lane's row r (0..ROWS-1) and column c (-4..67, relative to the tile); dxo = +-1 when
-  // it also loads b of the column beyond
   int r, c, dxo = 0;
   if (!side) {
     r = wv;
@@ -1460,28 +1459,51 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   const int gj = tyi * (ROWS - 6) - 3 + r;
   const int i = ((gi % nx) + nx) % nx;
   const int j = ((gj % ny) + ny) % ny;
-  const int ij = j * nx + i;
-  const int ss = (r + 1) * pw + (c + 4);  // LDS slot
-  int oo = -1, hs = ss;                   // the outer b: grid index and LDS slot
+  Schr1Lane g;
+  g.ij = j * nx + i;
+  g.ss = (r + 1) * pw + (c + 4);
+  g.oo = -1;
+  g.hs = g.ss;
   if (!side) {
     if (wv == 0) {
-      oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
-      hs = ss - pw;
+      g.oo = ((j == 0) ? ny - 1 : j - 1) * nx + i;
+      g.hs = g.ss - pw;
     } else if (wv == ROWS - 1) {
-      oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
-      hs = ss + pw;
+      g.oo = ((j + 1 == ny) ? 0 : j + 1) * nx + i;
+      g.hs = g.ss + pw;
     }
   } else if (dxo != 0) {
-    oo = j * nx + (((i + dxo) % nx) + nx) % nx;
-    hs = ss + dxo;
+    g.oo = j * nx + (((i + dxo) % nx) + nx) % nx;
+    g.hs = g.ss + dxo;
   }
-  const bool has_o = oo >= 0;
-  const bool outer_wave = side || wv == 0 || wv == ROWS - 1;  // uniform
+  g.has_o = g.oo >= 0;
+  g.outer_wave = side || wv == 0 || wv == ROWS - 1;
+  g.dyw = side ? 1 : max(max(3 - wv, wv - (ROWS - 4)), 0);
+  g.writer = g.dyw == 0 && gi < nx && gj < ny;
+  return g;
+}
+
+template <typename R, int ROWS, int CPOL>
+__device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile, const cplx<R> ph) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 7 && 3 * ROWS - 10 <= kWaveX / 2, "rows");
+  Schr1Lds<R, ROWS> &L = schr1_lds<R, ROWS>();
+  constexpr int pw = kSchr1Pitch;
+  constexpr int nps = Schr1Lds<R, ROWS>::nps;
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int lx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
+  const bool side = wv == ROWS;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const Schr1Lane g = schr1_lane<ROWS>(tx, tyi, nx, ny);
+  const int ij = g.ij, ss = g.ss, oo = g.oo, hs = g.hs;
+  const bool has_o = g.has_o, outer_wave = g.outer_wave, writer = g.writer;
   // levels this wave runs (uniform): a row at distance dy runs 1..4-dy, the side wave 1..3
-  const int dyw = side ? 1 : max(max(3 - wv, wv - (ROWS - 4)), 0);
-  const int nlev = 4 - dyw;
-  const bool writer_wave = dyw == 0;
-  const bool writer = writer_wave && gi < nx && gj < ny;
+  const int nlev = 4 - g.dyw;
+  const bool writer_wave = g.dyw == 0;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
   const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
@@ -2242,6 +2264,388 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   if (m <= m_end) plane_step(m, r0, r1, r2, r3, q0, q1);
   if (m + 1 <= m_end) plane_step(m + 1, r1, r2, r3, r0, q1, q0);
   if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, q0, q1);
+}
+
+// ------------------------------------ Schrodinger adjoint step in 1 fused pass
+//
+// The reverse of one schr1_pass step (u4 = b + h/4 M b, u3 = b + h/3 M u4,
+// u2 = b + h/2 M u3, psi' = b + h M u2, b = K psi_n + source) with the formulas of
+// schr_adj_pass (v2 = h M^H lam^, v3 = h/2 M^H v2^, v4 = h/3 M^H v3^,
+// lam_n = K (lam + v2 + v3 + v4 + h/4 M^H v4^), dL/dV += h Im(conj(lam) u2) +
+// h/2 Im(conj(v2) u3) + h/3 Im(conj(v3) u4) + h/4 Im(conj(v4) b) on interior voxels) in
+// one walk over the planes, with schr1_pass's layout: step m evaluates
+//   the adjoint chain  v2 at m, v3 at m-1, v4 at m-2, h/4 M^H v4^ (and lam_n) at m-3,
+//   the forward chain  u4 at m, u3 at m-1, u2 at m-2 (recomputed from the record psi_n),
+// the dL/dV terms of v2, v3 at m-1 and of lam, v4 at m-2 (vbar of m-2 read-modify-written),
+// and keeps lam + v2 + v3 + v4 of planes m..m-3 in a shift register.  The adjoint chain
+// needs lam within Manhattan distance 4 of the tile's interior and the forward chain b
+// within distance 3, so schr1_pass's tile and outer loads (of lam here) serve both: a row
+// at distance dy runs adjoint levels 1..4-dy and forward levels 1..3-dy, the side wave
+// adjoint levels 1..3 and forward levels 1..2.  HBM bytes per voxel of a reverse step:
+// read psi_n, lam (16), V (4), vbar (4); write lam_n (8), vbar (4) = 36 B, against 112 B
+// for schr_adj_pass's two passes.
+template <typename R>
+struct SchrAdj1Args {
+  using C = cplx<R>;
+  const C *rec;  // the step's input state psi_n
+  const C *lam;  // dL/dpsi_{n+1}
+  C *lam_out;    // dL/dpsi_n
+  R *vbar;       // dL/dV accumulator (the handle's real type)
+  const R *V;
+  const C *zc;
+  const C *prof;
+  int nx, ny, nz;
+  int tile_z;
+  int ntx, nty, ntz;
+  R hx, hy, h;
+  // planes pz_lo..pz_hi (outside the PML) have both z weights equal to the real hz there
+  int pz_lo, pz_hi;
+  R hz;
+  int src_k0, src_n;
+  const SrcCoef<R> *src_tab;
+  C src_ph;
+  int stream_out;  // non-temporal result stores
+};
+
+constexpr int kSchrA1Zc = kKG1MaxTileZ + 20;  // z weights of planes k0-8 .. k1+11
+
+template <typename R, int ROWS>
+struct SchrAdj1Lds {
+  static constexpr int nps = kSchr1Pitch * (ROWS + 2);
+  cplx<R> lb[2][nps];   // b (forward level 1)
+  cplx<R> lu4[2][nps];  // u4 (forward level 2)
+  cplx<R> lu3[2][nps];  // u3 (forward level 3)
+  cplx<R> ll[2][nps];   // lam^ (adjoint level 1)
+  cplx<R> lv2[2][nps];  // v2^ (adjoint level 2)
+  cplx<R> lv3[2][nps];  // v3^ (adjoint level 3)
+  cplx<R> lv4[2][nps];  // v4^ (adjoint level 4)
+  cplx<R> lzc[2 * kSchrA1Zc];
+  SrcCoef<R> lsrc[kMaxBand];
+};
+
+template <typename R, int ROWS, bool NT>
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(4))) void schr_adj1_pass(
+    const SchrAdj1Args<R> p) {
+  using C = cplx<R>;
+  static_assert(ROWS >= 7 && 3 * ROWS - 10 <= kWaveX / 2, "rows");
+  __shared__ SchrAdj1Lds<R, ROWS> L;
+  constexpr int pw = kSchr1Pitch;
+  constexpr int nps = SchrAdj1Lds<R, ROWS>::nps;
+  constexpr int CPOL = NT ? kStream : kCached;
+  const int tile = zgroup_tile<kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz), p.ntx * p.nty,
+                                        p.ntz);
+  const int tx = tile % p.ntx;
+  const int rest = tile / p.ntx;
+  const int tyi = rest % p.nty;
+  const int tz = rest / p.nty;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const Schr1Lane g = schr1_lane<ROWS>(tx, tyi, nx, ny);
+  const int ij = g.ij, ss = g.ss, oo = g.oo, hs = g.hs;
+  const bool has_o = g.has_o;
+  const int nA = 4 - g.dyw, nF = 3 - g.dyw;  // levels this wave runs (uniform)
+  const bool writer_wave = g.dyw == 0;
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc(p.rec), rlam = buf_rsrc(p.lam), rv = buf_rsrc(p.V);
+  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc(p.lam_out), rvb = buf_rsrc(p.vbar);
+  const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
+  const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
+  const uint32_t ow = g.writer ? oc : kBufSkip;
+  const uint32_t owr = g.writer ? ov : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C ph = p.src_ph;
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+  const int m_start = k0 - 3, m_end = k1 + 2;
+  const int zbase = k0 - 8;  // plane of lzc[0..1]
+  const R h = p.h, h2 = p.h * R(0.5), h3 = p.h / R(3), h4 = p.h * R(0.25);
+  // the planes each level is needed on (the chunk grown by the z halo it feeds)
+  const int la1 = max(k0 - 3, 0), ha1 = min(k1 + 2, nz - 1);  // adjoint levels
+  const int la2 = max(k0 - 2, 0), ha2 = min(k1 + 1, nz - 1);
+  const int la3 = max(k0 - 1, 0), ha3 = min(k1, nz - 1);
+  const int lf1 = max(k0 - 2, 0), hf1 = min(k1 + 1, nz - 1);  // forward levels
+  const int lf2 = max(k0 - 1, 0), hf2 = min(k1, nz - 1);
+  {
+    const int tid = (int)(threadIdx.y * kWaveX + threadIdx.x);
+    constexpr int nthr = kWaveX * (ROWS + 1);
+    for (int t = tid; t < nps; t += nthr) {
+      L.lb[0][t] = L.lb[1][t] = zero;
+      L.lu4[0][t] = L.lu4[1][t] = zero;
+      L.lu3[0][t] = L.lu3[1][t] = zero;
+      L.ll[0][t] = L.ll[1][t] = zero;
+      L.lv2[0][t] = L.lv2[1][t] = zero;
+      L.lv3[0][t] = L.lv3[1][t] = zero;
+      L.lv4[0][t] = L.lv4[1][t] = zero;
+    }
+    // z weights (0 outside the grid: the adjoint z stencil reads the neighbours' weights)
+    for (int t = tid; t < kSchrA1Zc; t += nthr) {
+      const int k = zbase + t;
+      const bool in = k >= 0 && k < nz;
+      L.lzc[2 * t] = in ? p.zc[2 * k] : zero;
+      L.lzc[2 * t + 1] = in ? p.zc[2 * k + 1] : zero;
+    }
+    if (tid < p.src_n) L.lsrc[tid] = p.src_tab[tid];
+  }
+  auto interior = [&](int k) { return k > 0 && k < nz - 1; };
+  auto zw = [&](int k, int which) { return L.lzc[2 * (k - zbase) + which]; };  // 0: cp, 1: cm
+  auto keep_of = [&](int k) {
+    const int bnd = k - p.src_k0;
+    return (bnd >= 0 && bnd < p.src_n) ? L.lsrc[bnd].keep : R(1);
+  };
+  // x on interior planes, else 0, selected by value (see schr_adj_pass)
+  auto mask = [&](int k, C x) {
+    const bool in = interior(k);
+    x.x = in ? x.x : R(0);
+    x.y = in ? x.y : R(0);
+    return x;
+  };
+  auto lat = [&](const C *Lp, C &sx, C &sy) {
+    sx = cadd<R>(Lp[ss + 1], Lp[ss - 1]);
+    sy = cadd<R>(Lp[ss + pw], Lp[ss - pw]);
+  };
+  // forward level: b + f M y at plane k (f = 0 off the interior)
+  auto fwd = [&](const C *Lp, R f, int k, C base, R v, C y, C zp, C zm) {
+    C sx, sy;
+    lat(Lp, sx, sy);
+    const C hy = schr_hf<R>(p.hx, p.hy, v, zw(k, 0), zw(k, 1), y, sx, sy, zp, zm);
+    return horner<R>(base, interior(k) ? f : R(0), hy);
+  };
+  // adjoint level: c M^H x at plane k, x masked to the interior (the LDS planes hold it
+  // masked; the z neighbours are masked here)
+  auto adj = [&](const C *Lp, R cc, int k, R v, C x, C xp, C xm) {
+    C sx, sy;
+    lat(Lp, sx, sy);
+    return schr_mh<R>(p.hx, p.hy, v, cc, zw(k - 1, 0), cadd<R>(zw(k, 0), zw(k, 1)), zw(k + 1, 1),
+                      mask(k, x), sx, sy, mask(k + 1, xp), mask(k - 1, xm));
+  };
+
+  struct Pl {  // one plane of this lane: b (raw until injected), lam own and outer, V
+    C b, l, ol;
+    R v;
+  };
+  // every wave issues the same vector-memory operations per plane (loads at kBufSkip
+  // fetch nothing), so the in-order counter waits exactly for the plane a step needs
+  auto load_plane = [&](int m, Pl &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bc = b * (uint32_t)sizeof(C);
+    w.b = buf_ld<C>(rrec, oc, bc);
+    w.l = buf_ld<C>(rlam, oc, bc);
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    w.ol = buf_ld<C>(rlam, ooc, bc);
+  };
+  // b = inject(psi_n) on a band plane (this lane's own column: the forward chain needs b
+  // within distance 3 only)
+  auto inject_plane = [&](int m, Pl &w) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = L.lsrc[bnd].keep;
+    sp.c = cmul<R>(L.lsrc[bnd].c, ph);
+    sp.cq = sp.c;
+    w.b = inj<R>(w.b, sp, sp.c, prof, ij);
+  };
+  auto ld_vbar = [&](int k) {
+    const bool in = k >= k0 && k < k1;
+    return buf_ld<R>(rvb, in ? owr : kBufSkip,
+                     (uint32_t)min(max(k, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+  };
+
+  Pl pq[6];  // plane m_start + q in slot q mod 6
+  for (int q = 0; q < 6; ++q) {
+    pq[q].b = pq[q].l = pq[q].ol = zero;
+    pq[q].v = R(0);
+  }
+  // the last two planes of the chained levels (plane k in slot (k - m_start) & 1)
+  C u4q[2], u3q[2], v2q[2], v3q[2], v4q[2];
+  for (int q = 0; q < 2; ++q) u4q[q] = u3q[q] = v2q[q] = v3q[q] = v4q[q] = zero;
+  // lam + v2 + v3 + v4 of planes m-1, m-2, m-3 (writer waves), the dL/dV terms of plane
+  // m-2 computed one step earlier, vbar of plane m-2
+  C s1 = zero, s2 = zero, s3 = zero;
+  R gprev = R(0), vbc = R(0);
+  load_plane(m_start - 1, pq[5]);
+  load_plane(m_start, pq[0]);
+  load_plane(m_start + 1, pq[1]);
+  R vbn = ld_vbar(m_start - 1);
+  __syncthreads();  // zeroed LDS planes, lzc, lsrc
+  inject_plane(m_start - 1, pq[5]);
+  inject_plane(m_start, pq[0]);
+  L.lb[0][ss] = pq[0].b;
+  L.ll[0][ss] = mask(m_start, pq[0].l);
+  if (has_o) L.ll[0][hs] = mask(m_start, pq[0].ol);
+  __syncthreads();
+
+  // step m: buffer U & 1 of each LDS pair holds b(m), lam^(m), u4(m-1), v2^(m-1), u3(m-2),
+  // v3^(m-2), v4^(m-3); the other buffer receives the planes one higher
+  auto step = [&](auto uc, const int m) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    constexpr int rd = U & 1, wr = rd ^ 1, P = U & 1, Q = P ^ 1;
+    Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
+    load_plane(m + 2, pp2);
+    // vbar of plane m-2, loaded two steps ahead (vbn: plane m-1 for the next step)
+    const R vb_here = vbc;
+    vbc = vbn;
+    vbn = ld_vbar(m);
+    inject_plane(m + 1, pp1);
+    // ---- forward chain: u4 at m, u3 at m-1, u2 at m-2
+    C nu4 = zero, nu3 = zero, nu2 = zero;
+    if (nF >= 1 && m >= lf1 && m <= hf1)
+      nu4 = fwd(L.lb[rd], h4, m, p0.b, p0.v, p0.b, pp1.b, pm1.b);
+    if (nF >= 2 && m - 1 >= lf2 && m - 1 <= hf2)
+      nu3 = fwd(L.lu4[rd], h3, m - 1, pm1.b, pm1.v, u4q[Q], nu4, u4q[P]);
+    const int k2 = m - 2;  // the plane whose dL/dV completes this step
+    const bool st2 = writer_wave && k2 >= k0 && k2 < k1;  // uniform
+    if (st2) nu2 = fwd(L.lu3[rd], h2, k2, pm2.b, pm2.v, u3q[P], nu3, u3q[Q]);
+    // ---- adjoint chain: v2 at m, v3 at m-1, v4 at m-2, lam_n at m-3
+    C nv2 = zero, nv3 = zero, nv4 = zero;
+    if (m >= la1 && m <= ha1) nv2 = adj(L.ll[rd], h, m, p0.v, p0.l, pp1.l, pm1.l);
+    if (nA >= 2 && m - 1 >= la2 && m - 1 <= ha2)
+      nv3 = adj(L.lv2[rd], h2, m - 1, pm1.v, v2q[Q], nv2, v2q[P]);
+    if (nA >= 3 && m - 2 >= la3 && m - 2 <= ha3)
+      nv4 = adj(L.lv3[rd], h3, m - 2, pm2.v, v3q[P], nv3, v3q[Q]);
+    const int k3 = m - 3;
+    const bool st3 = writer_wave && k3 >= k0 && k3 < k1;  // uniform
+    C ln = zero;
+    R gsum = R(0), gnew = R(0);
+    if (writer_wave) {
+      if (st3) {
+        const C w = adj(L.lv4[rd], h4, k3, pm3.v, v4q[Q], nv4, v4q[P]);
+        ln = cscale<R>(cadd<R>(s3, w), keep_of(k3));
+      }
+      // lam + v2 (+ v3, + v4 as they come) of planes m, m-1, m-2
+      const C s0 = cadd<R>(p0.l, nv2);
+      s1 = cadd<R>(s1, nv3);
+      s2 = cadd<R>(s2, nv4);
+      // dL/dV: the v2, v3 terms of plane m-1 (kept for the next step), the lam, v4 terms
+      // of plane m-2 added to those its previous step kept
+      if (interior(m - 1)) gnew = h2 * imcj<R>(v2q[Q], nu3) + h3 * imcj<R>(nv3, u4q[Q]);
+      if (interior(k2)) gsum = gprev + h * imcj<R>(pm2.l, nu2) + h4 * imcj<R>(nv4, pm2.b);
+      s3 = s2;
+      s2 = s1;
+      s1 = s0;
+      gprev = gnew;
+    }
+    buf_st<C, CPOL>(rlo, st3 ? ow : kBufSkip, (uint32_t)max(k3, 0) * plane * (uint32_t)sizeof(C), ln);
+    buf_st<R, CPOL>(rvb, st2 ? owr : kBufSkip, (uint32_t)max(k2, 0) * plane * (uint32_t)sizeof(R),
+                    vb_here + gsum);
+    // ---- LDS planes one higher, register rings
+    L.lb[wr][ss] = pp1.b;
+    L.ll[wr][ss] = mask(m + 1, pp1.l);
+    if (has_o) L.ll[wr][hs] = mask(m + 1, pp1.ol);
+    L.lu4[wr][ss] = nu4;
+    L.lv2[wr][ss] = mask(m, nv2);
+    if (nF >= 2) L.lu3[wr][ss] = nu3;
+    if (nA >= 2) L.lv3[wr][ss] = mask(m - 1, nv3);
+    if (nA >= 3) L.lv4[wr][ss] = mask(m - 2, nv4);
+    u4q[P] = nu4;
+    u3q[Q] = nu3;
+    v2q[P] = nv2;
+    v3q[Q] = nv3;
+    v4q[P] = nv4;
+    __syncthreads();
+  };
+  // A steady-state step: every level active, planes m-4..m+1 outside the PML (real, equal
+  // z weights: H and H^H are one real symmetric stencil; nothing to mask), no source plane
+  // among m-3..m+1, the dL/dV and lam_n planes and vbar's inside the chunk.  The same
+  // arithmetic without per-step range tests, masks, clamps or table lookups.
+  auto hreal = [&](const C *Lp, R v, C y, C zp, C zm) {
+    C sx, sy;
+    lat(Lp, sx, sy);
+    return schr_h_real<R>(p, v, p.hz, y, sx, sy, zp, zm);
+  };
+  auto step_fast = [&](auto uc, const int m) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    constexpr int rd = U & 1, wr = rd ^ 1, P = U & 1, Q = P ^ 1;
+    Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
+    {
+      const uint32_t b = (uint32_t)(m + 2) * plane;
+      const uint32_t bc = b * (uint32_t)sizeof(C);
+      pp2.b = buf_ld<C>(rrec, oc, bc);
+      pp2.l = buf_ld<C>(rlam, oc, bc);
+      pp2.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+      pp2.ol = buf_ld<C>(rlam, ooc, bc);
+    }
+    const R vb_here = vbc;
+    vbc = vbn;
+    vbn = buf_ld<R>(rvb, owr, (uint32_t)m * plane * (uint32_t)sizeof(R));
+    C nu4 = zero, nu3 = zero, nu2 = zero;
+    if (nF >= 1) nu4 = horner<R>(p0.b, h4, hreal(L.lb[rd], p0.v, p0.b, pp1.b, pm1.b));
+    if (nF >= 2) nu3 = horner<R>(pm1.b, h3, hreal(L.lu4[rd], pm1.v, u4q[Q], nu4, u4q[P]));
+    if (writer_wave) nu2 = horner<R>(pm2.b, h2, hreal(L.lu3[rd], pm2.v, u3q[P], nu3, u3q[Q]));
+    // c M^H x = c i H x here
+    auto mh = [&](R cc, C hx_) { return mkc<R>(-cc * hx_.y, cc * hx_.x); };
+    const C nv2 = mh(h, hreal(L.ll[rd], p0.v, p0.l, pp1.l, pm1.l));
+    C nv3 = zero, nv4 = zero, ln = zero;
+    if (nA >= 2) nv3 = mh(h2, hreal(L.lv2[rd], pm1.v, v2q[Q], nv2, v2q[P]));
+    if (nA >= 3) nv4 = mh(h3, hreal(L.lv3[rd], pm2.v, v3q[P], nv3, v3q[Q]));
+    R gsum = R(0);
+    if (writer_wave) {
+      ln = cadd<R>(s3, mh(h4, hreal(L.lv4[rd], pm3.v, v4q[Q], nv4, v4q[P])));
+      const C s0 = cadd<R>(p0.l, nv2);
+      s1 = cadd<R>(s1, nv3);
+      s2 = cadd<R>(s2, nv4);
+      const R gnew = h2 * imcj<R>(v2q[Q], nu3) + h3 * imcj<R>(nv3, u4q[Q]);
+      gsum = gprev + h * imcj<R>(pm2.l, nu2) + h4 * imcj<R>(nv4, pm2.b);
+      s3 = s2;
+      s2 = s1;
+      s1 = s0;
+      gprev = gnew;
+    }
+    buf_st<C, CPOL>(rlo, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), ln);
+    buf_st<R, CPOL>(rvb, owr, (uint32_t)(m - 2) * plane * (uint32_t)sizeof(R), vb_here + gsum);
+    L.lb[wr][ss] = pp1.b;
+    L.ll[wr][ss] = pp1.l;
+    if (has_o) L.ll[wr][hs] = pp1.ol;
+    L.lu4[wr][ss] = nu4;
+    L.lv2[wr][ss] = nv2;
+    if (nF >= 2) L.lu3[wr][ss] = nu3;
+    if (nA >= 2) L.lv3[wr][ss] = nv3;
+    if (nA >= 3) L.lv4[wr][ss] = nv4;
+    u4q[P] = nu4;
+    u3q[Q] = nu3;
+    v2q[P] = nv2;
+    v3q[Q] = nv3;
+    v4q[P] = nv4;
+    __syncthreads();
+  };
+  // the steady-state steps (uniform), the range keeping the longer side of a source band
+  int mf_lo = max(k0 + 3, p.pz_lo + 4);
+  int mf_hi = min(k1 - 1, p.pz_hi - 1);
+  if (p.src_n > 0) {
+    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n + 2;  // steps touching the band
+    if (sb_lo <= mf_hi && sb_hi >= mf_lo) {
+      if (sb_lo - mf_lo >= mf_hi - sb_hi)
+        mf_hi = sb_lo - 1;
+      else
+        mf_lo = sb_hi + 1;
+    }
+  }
+  auto general6 = [&](int m0) __attribute__((always_inline)) {
+    step(std::integral_constant<int, 0>{}, m0);
+    if (m0 + 1 > m_end) return;
+    step(std::integral_constant<int, 1>{}, m0 + 1);
+    if (m0 + 2 > m_end) return;
+    step(std::integral_constant<int, 2>{}, m0 + 2);
+    if (m0 + 3 > m_end) return;
+    step(std::integral_constant<int, 3>{}, m0 + 3);
+    if (m0 + 4 > m_end) return;
+    step(std::integral_constant<int, 4>{}, m0 + 4);
+    if (m0 + 5 > m_end) return;
+    step(std::integral_constant<int, 5>{}, m0 + 5);
+  };
+  int m0 = m_start;
+  for (; m0 <= m_end && m0 < mf_lo; m0 += 6) general6(m0);
+  for (; m0 + 5 <= mf_hi; m0 += 6) {
+    step_fast(std::integral_constant<int, 0>{}, m0);
+    step_fast(std::integral_constant<int, 1>{}, m0 + 1);
+    step_fast(std::integral_constant<int, 2>{}, m0 + 2);
+    step_fast(std::integral_constant<int, 3>{}, m0 + 3);
+    step_fast(std::integral_constant<int, 4>{}, m0 + 4);
+    step_fast(std::integral_constant<int, 5>{}, m0 + 5);
+  }
+  for (; m0 <= m_end; m0 += 6) general6(m0);
 }
 
 // ------------------------------------- Klein-Gordon adjoint step in 1 fused pass
