@@ -142,6 +142,11 @@ def _group_of(p):
     return (p.equation, p.precision, p.resolved_schedule())
 
 
+# workgroup waves of the schedules whose shape is fixed (tile_y does not apply): the
+# one-pass Schrodinger step runs 14 row waves and a side wave
+FIXED_WAVES = {"schr_1pass": 15}
+
+
 def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, steps=4, warmup=2,
                verbose=False):
     """Launch shape for grids batched together: the z chunk per size class and the
@@ -154,7 +159,7 @@ def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, ste
     groups = sorted({_group_of(p) for p in param_list}, key=str)
 
     def label(choice, waves):
-        return {"/".join((c[0], "x".join(map(str, c[3])))): (waves[c[:3]], tz)
+        return {"/".join((c[0], "x".join(map(str, c[3])))): (FIXED_WAVES.get(c[2], waves[c[:3]]), tz)
                 for c, tz in choice.items()}
 
     def run(choice, waves):

@@ -1423,11 +1423,29 @@ struct Schr1Lane {
   int ij, ss, oo, hs, dyw;
   bool has_o, outer_wave, writer;
 };
+// The row (0..ROWS-1; ROWS: the side wave) that wave slot `slot` of a workgroup runs.  Wave
+// slots i and i + 4 of a workgroup issue on the same SIMD (the four classes i mod 4 on the
+// four SIMDs), and the rows differ in work: a row at distance dy from the interior runs
+// 4 - dy forward levels (the side wave 3) and the adjoint 7 - 2 dy stencils (the side wave
+// 5).  In row order the busiest SIMD holds 14 forward levels against 11.75 on average
+// (adjoint: 24 stencils against 19.75), and the barrier of every plane step waits for it;
+// with 14 rows the slots below give 12, 12, 12, 11 (adjoint 20, 20, 20, 19).
+template <int ROWS>
+__device__ __forceinline__ int schr1_wave_row(int slot) {
+  if constexpr (ROWS == 14) {
+    // slot 0..14 -> row 3, 5, 7, 9, 4, 6, 8, 10, 2, 11, 1, side, 0, 13, 12 (a nibble each)
+    constexpr uint64_t kRowOfSlot = 0xcd0e1b2a8649753ull;
+    return (int)((kRowOfSlot >> (4 * slot)) & 15);
+  }
+  return slot;
+}
+
 template <int ROWS>
 __device__ __forceinline__ Schr1Lane schr1_lane(int tx, int tyi, int nx, int ny) {
   constexpr int pw = kSchr1Pitch;
   const int lx = threadIdx.x;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
+  // one wavefront per row: uniform
+  const int wv = schr1_wave_row<ROWS>(__builtin_amdgcn_readfirstlane(threadIdx.y));
   const bool side = wv == ROWS;
   int r, c, dxo = 0;
   if (!side) {
@@ -1495,8 +1513,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   const int tyi = rest % p.nty;
   const int tz = rest / p.nty;
   const int lx = threadIdx.x;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wavefront per row: uniform
-  const bool side = wv == ROWS;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // wave slot (LDS clearing)
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const Schr1Lane g = schr1_lane<ROWS>(tx, tyi, nx, ny);
   const int ij = g.ij, ss = g.ss, oo = g.oo, hs = g.hs;
