@@ -183,3 +183,19 @@ def test_schr1_adjoint_equals_two_pass_adjoint(rng):
     rel = rel_l2(grads[0], grads[1])
     print(f"one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
     assert rel < 1e-4
+
+
+def test_schr1_adjoint_workspace_has_no_pass_buffers():
+    # the one-pass reverse step keeps no intermediate fields: the workspace is the two-pass
+    # adjoint's less the three fields its first pass writes
+    FDTD3D = _gpu()
+    from wideanglepropagation_amd.adjoint import _Adjoint
+    p1 = _params((22, 10, 12), schedule="schr_1pass")
+    nb = {}
+    for p in (p1, p1.with_(schedule="schr_fused")):
+        sim = FDTD3D(p)
+        nb[p.schedule] = (_Adjoint.workspace_bytes(sim, 1), _Adjoint.workspace_bytes(sim, 3))
+        sim.close()
+    field = -(-22 * 10 * 12 * 8 // 256) * 256
+    assert nb["schr_fused"][0] - nb["schr_1pass"][0] == 3 * field
+    assert nb["schr_fused"][1] - nb["schr_1pass"][1] == 3 * field

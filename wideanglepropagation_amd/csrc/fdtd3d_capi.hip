@@ -1947,7 +1947,8 @@ struct wap_adjoint {
   void *vacc = nullptr;  // fused path: dL/dV accumulated in the handle's real type,
                          // folded into vbar every kGradFlush reverse steps
   int since_flush = 0;
-  bool schr = false;     // fused Schrodinger (one state component; u3, v3, S in kba, kbb, acc)
+  bool schr = false;     // fused Schrodinger (one state component; with two-pass records u3,
+                         // v3, S in kba, kbb, acc)
   bool owns = true;      // false: the buffers are carved from a caller's workspace
   void *orig[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [comp]: the handle's
                                                                   // state, scratch at create
@@ -1973,7 +1974,7 @@ std::vector<std::pair<void **, size_t>> adjoint_buffers(wap_adjoint *a) {
       v.push_back({&a->lam2[0][0], fb});
       v.push_back({&a->lam2[1][0], fb});
     }
-    if (a->schr) {  // the pass-I outputs u3, v3 and S
+    if (a->schr && sched_of(h) == WAP_SCHED_SCHR_FUSED) {  // schr_adj_pass I's u3, v3 and S
       v.push_back({&a->kba[0], fb});
       v.push_back({&a->kbb[0], fb});
       v.push_back({&a->acc[0], fb});
