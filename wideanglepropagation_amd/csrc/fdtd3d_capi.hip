@@ -446,17 +446,21 @@ int kg1_nty(const wap_handle *h) { return kg1_nty(h, kg1_waves(h)); }
 // The fused passes hold per_cu workgroups per CU on 256 CUs.  Pick the z chunk
 // that minimises (full rounds of workgroups) x (planes per chunk + halo planes):
 // few, long chunks amortise the halo planes, but a last round that is mostly empty
-// costs a whole round.
-int chunk_tile_z(const wap_handle *h, long cols, int per_cu, int halo) {
+// costs a whole round.  A lone grid searches chunks from 2 planes: a small grid needs
+// short chunks to occupy the CUs, and a plane step costs its latency whatever the grid
+// (64^3: 2-plane chunks run 1.6x as fast as 8-plane ones).  A grid in a batch shares
+// the CUs with the other grids, where short chunks only add halo planes: from 8.
+int chunk_tile_z(const wap_handle *h, long cols, int per_cu, int halo, bool batch) {
   if (h->d.tile_z > 0) {
     const int tz = h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
     return tz < wap::kKG1MaxTileZ ? tz : wap::kKG1MaxTileZ;
   }
   const long slots = 256L * per_cu;
   const int tmax = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
+  const int tmin = batch ? 8 : 2;
   int best = tmax;
   double best_cost = 1e300;
-  for (int tz = 8 < tmax ? 8 : tmax; tz <= tmax; ++tz) {
+  for (int tz = tmin < tmax ? tmin : tmax; tz <= tmax; ++tz) {
     const long chunks = (h->d.nz + tz - 1) / tz;
     const double cost = (double)((cols * chunks + slots - 1) / slots) * (tz + halo);
     if (cost < best_cost) {  // ties keep the shorter chunk (more workgroups in flight)
@@ -466,24 +470,24 @@ int chunk_tile_z(const wap_handle *h, long cols, int per_cu, int halo) {
   }
   return best;
 }
-int kg1_tile_z(const wap_handle *h, int waves, int per_cu) {
-  return chunk_tile_z(h, (long)kg1_ntx(h) * kg1_nty(h, waves), per_cu, 2);
+int kg1_tile_z(const wap_handle *h, int waves, int per_cu, bool batch = false) {
+  return chunk_tile_z(h, (long)kg1_ntx(h) * kg1_nty(h, waves), per_cu, 2, batch);
 }
 int kg1_tile_z(const wap_handle *h) { return kg1_tile_z(h, kg1_waves(h), 16 / kg1_waves(h)); }
 
 // tiles of the fused passes over one grid with `waves` waves per workgroup
-int64_t fused_tiles(const wap_handle *h, int waves) {
-  const int tz = kg1_tile_z(h, waves, 16 / waves);
+int64_t fused_tiles(const wap_handle *h, int waves, bool batch = false) {
+  const int tz = kg1_tile_z(h, waves, 16 / waves, batch);
   return (int64_t)kg1_ntx(h) * kg1_nty(h, waves) * ((h->d.nz + tz - 1) / tz);
 }
 
 // the launch-invariant arguments of kg1_pass (buffers and source are set per launch)
 template <typename R>
-wap::KG1Args<R> kg1_args(const wap_handle *h, int waves) {
+wap::KG1Args<R> kg1_args(const wap_handle *h, int waves, bool batch = false) {
   using C = wap::cplx<R>;
   wap::KG1Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
-  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  const int tz = kg1_tile_z(h, waves, 16 / waves, batch);
   a.V = (const R *)h->V;
   a.gam = (const R *)h->gam;
   a.prof = (const C *)h->prof;
@@ -555,11 +559,11 @@ int run_steps_kg1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 
 // the launch-invariant arguments of schr2_pass; pass 0 (A) or 1 (B) sets the level factors
 template <typename R>
-wap::Schr2Args<R> schr2_args(const wap_handle *h, int waves, int pass) {
+wap::Schr2Args<R> schr2_args(const wap_handle *h, int waves, int pass, bool batch = false) {
   using C = wap::cplx<R>;
   wap::Schr2Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
-  const int tz = kg1_tile_z(h, waves, 16 / waves);
+  const int tz = kg1_tile_z(h, waves, 16 / waves, batch);
   a.V = (const R *)h->V;
   a.zc = (const C *)h->zc;
   a.prof = (const C *)h->prof;
@@ -631,20 +635,20 @@ int schr1_nty(const wap_handle *h) {
   constexpr int out_rows = wap::kSchr1Rows - 6;
   return (h->d.ny + out_rows - 1) / out_rows;
 }
-int schr1_tile_z(const wap_handle *h) {
-  return chunk_tile_z(h, (long)kg1_ntx(h) * schr1_nty(h), wap::kSchr1PerCu, 3);
+int schr1_tile_z(const wap_handle *h, bool batch = false) {
+  return chunk_tile_z(h, (long)kg1_ntx(h) * schr1_nty(h), wap::kSchr1PerCu, 3, batch);
 }
-int64_t schr1_tiles(const wap_handle *h) {
-  const int tz = schr1_tile_z(h);
+int64_t schr1_tiles(const wap_handle *h, bool batch = false) {
+  const int tz = schr1_tile_z(h, batch);
   return (int64_t)kg1_ntx(h) * schr1_nty(h) * ((h->d.nz + tz - 1) / tz);
 }
 
 template <typename R>
-wap::Schr1Args<R> schr1_args(const wap_handle *h) {
+wap::Schr1Args<R> schr1_args(const wap_handle *h, bool batch = false) {
   using C = wap::cplx<R>;
   wap::Schr1Args<R> a;
   std::memset((void *)&a, 0, sizeof(a));
-  const int tz = schr1_tile_z(h);
+  const int tz = schr1_tile_z(h, batch);
   a.V = (const R *)h->V;
   a.zc = (const C *)h->zc;
   a.prof = (const C *)h->prof;
@@ -1319,7 +1323,7 @@ int batch_upload_fused(wap_batch *b) {
       std::vector<wap::KG1Args<R>> a(n);
       for (int i = 0; i < n; ++i) {
         const wap_handle *h = b->hs[i];
-        a[i] = kg1_args<R>(h, b->waves);
+        a[i] = kg1_args<R>(h, b->waves, true);
         timeless_source(h, a[i]);
         a[i].in_psi = (const C *)(par ? h->ya_psi : h->psi);
         a[i].in_pi = (const C *)(par ? h->ya_pi : h->pi);
@@ -1332,7 +1336,7 @@ int batch_upload_fused(wap_batch *b) {
       std::vector<wap::Schr1Args<R>> a(n);
       for (int i = 0; i < n; ++i) {
         const wap_handle *h = b->hs[i];
-        a[i] = schr1_args<R>(h);
+        a[i] = schr1_args<R>(h, true);
         timeless_source(h, a[i]);
         a[i].in = (const C *)(par ? h->yb_psi : h->psi);
         a[i].out = (C *)(par ? h->psi : h->yb_psi);
@@ -1344,8 +1348,8 @@ int batch_upload_fused(wap_batch *b) {
       for (int i = 0; i < n; ++i) {
         const wap_handle *h = b->hs[i];
         const C *cur = (const C *)(par ? h->yb_psi : h->psi);
-        pa[i] = schr2_args<R>(h, b->waves, 0);
-        pb[i] = schr2_args<R>(h, b->waves, 1);
+        pa[i] = schr2_args<R>(h, b->waves, 0, true);
+        pb[i] = schr2_args<R>(h, b->waves, 1, true);
         timeless_source(h, pa[i]);
         timeless_source(h, pb[i]);
         pa[i].in = cur;
@@ -1413,8 +1417,8 @@ int batch_build(wap_batch *b) {
   for (int i = 0; i < n; ++i) {
     const wap_handle *h = b->hs[i];
     const int nty = (h->d.ny + b->tile_y - 1) / b->tile_y;
-    const int64_t t = b->sched == WAP_SCHED_SCHR_1PASS ? schr1_tiles(h)
-                      : batch_fused(b)                   ? fused_tiles(h, b->waves)
+    const int64_t t = b->sched == WAP_SCHED_SCHR_1PASS ? schr1_tiles(h, true)
+                      : batch_fused(b)                   ? fused_tiles(h, b->waves, true)
                                                          : (int64_t)h->ntx * nty * h->ntz;
     if (begin[i] + t > INT32_MAX) return fail(WAP_ERR_ARG, "too many tiles in one batch");
     begin[i + 1] = begin[i] + (int)t;
