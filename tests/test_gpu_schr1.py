@@ -72,10 +72,11 @@ def test_schr1_matches_oracle_ragged(shape, tz, rng):
 # the steady-state step (all levels outside the PML, no source plane, no clamped plane)
 # against the general one: a source band in the middle of the steady-state range (it splits
 # the range), a thin and a thick PML, a PML on the end planes only, chunk edges inside the
-# range
-@pytest.mark.parametrize("shape,tz,k_source,pml", [((60, 20, 64), 0, 30, 0.15),
+# range; whole-grid chunks (the cost model picks 2-plane chunks on these small grids, too
+# short for a steady-state step)
+@pytest.mark.parametrize("shape,tz,k_source,pml", [((60, 20, 64), 60, 30, 0.15),
                                                    ((60, 20, 64), 16, 29, 0.15),
-                                                   ((70, 12, 64), 0, 8, 0.6),
+                                                   ((70, 12, 64), 70, 8, 0.6),
                                                    ((50, 9, 40), 13, 3, 0.025)])
 def test_schr1_steady_state_range(shape, tz, k_source, pml, rng):
     print(f"case schr_1pass steady state {shape} tz={tz} k_source={k_source} pml={pml}",
@@ -199,3 +200,28 @@ def test_schr1_adjoint_workspace_has_no_pass_buffers():
     field = -(-22 * 10 * 12 * 8 // 256) * 256
     assert nb["schr_fused"][0] - nb["schr_1pass"][0] == 3 * field
     assert nb["schr_fused"][1] - nb["schr_1pass"][1] == 3 * field
+
+
+@pytest.mark.parametrize("shape,tz,k_source,pml", [((60, 20, 64), 60, 30, 0.15),
+                                                   ((60, 20, 64), 16, 29, 0.15),
+                                                   ((50, 9, 40), 13, 3, 0.025)])
+def test_schr1_adjoint_steady_state_range(shape, tz, k_source, pml, rng):
+    # the one-pass reverse step's steady-state variant (long chunks outside the PML, the
+    # source band splitting the range, chunk edges inside it) against the two-pass adjoint
+    _gpu()
+    from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
+    print(f"case schr_1pass adjoint steady state {shape} tz={tz} k_source={k_source} "
+          f"pml={pml}", flush=True)
+    p1 = _params(shape, tile_z=tz, pml_thick=pml, k_source=k_source, schedule="schr_1pass")
+    V0, psi0 = _case(p1, rng, vscale=300.0)
+    grads = []
+    for p in (p1, p1.with_(schedule="schr_fused", tile_z=0)):
+        V = torch.tensor(V0, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = exit_wave(V, p, 12, p.nz - 4, samples={12: 1.0, 9: 0.5}, checkpoint_every=4,
+                      psi0=psi0)
+        torch.sum(far_field_intensity(E) ** 2).backward()
+        grads.append(V.grad.detach().double().cpu().numpy())
+    assert np.abs(grads[1]).max() > 0
+    rel = rel_l2(grads[0], grads[1])
+    print(f"one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
+    assert rel < 1e-4
