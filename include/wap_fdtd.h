@@ -61,18 +61,20 @@ extern "C" {
 
 /* pass schedule of a step */
 #define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a field is under 2 GiB,
-                               * else KG_2PASS; Schrodinger with frozen end planes and fields
-                               * under 2 GiB: SCHR_1PASS in f32, SCHR_FUSED in f64 and
-                               * f32_kahan; else STAGES */
+                               * else KG_2PASS; Schrodinger with fields under 2 GiB:
+                               * SCHR_1PASS in f32 (and in f64 on 2D grids, ny = 1), else
+                               * SCHR_FUSED with frozen end planes; else STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
 #define WAP_SCHED_KG_FUSED 3  /* both stencil passes in one pass (LDS halo, ping-pong state;
                                * f32_kahan updates its compensation in place) */
-#define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger f32: the RK4 polynomial in Horner form, two passes
-                                 * of two stencil levels each (AUTO picks it for f32 with
-                                 * frozen end planes) */
-#define WAP_SCHED_SCHR_1PASS 5  /* Schrodinger f32 (not f32_kahan), frozen end planes: the four
-                                 * Horner levels in one pass (20 B per voxel-step) */
+#define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger, frozen end planes, every precision: the RK4
+                                 * polynomial in Horner form, two passes of two stencil levels
+                                 * each (AUTO picks it for f64 and f32_kahan on 3D grids) */
+#define WAP_SCHED_SCHR_1PASS 5  /* Schrodinger f32 (f32 or f64 on 2D grids; not f32_kahan),
+                                 * frozen or zero end planes: the four Horner levels in one
+                                 * pass (20 B per voxel-step; 2D grids: one wave per 56-column
+                                 * tile and z chunk) */
 
 /* planes k = 0 and nz-1 (never stepped) */
 #define WAP_ENDS_FROZEN 0  /* stage inputs there = the injected state (fdtd_3d.cu:184-190) */

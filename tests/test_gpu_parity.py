@@ -180,8 +180,10 @@ def test_schr_fused_is_the_default_in_every_precision():
     assert p.resolved_schedule() == "schr_1pass"
     assert p.with_(precision="f64").resolved_schedule() == "schr_fused"
     assert p.with_(precision="f32_kahan").resolved_schedule() == "schr_fused"
-    # the 2D rule (zero stage inputs on the end rows) runs one pass per stage
-    assert p.with_(end_planes="zero").resolved_schedule() == "stages"
+    # the 2D rule (zero stage inputs on the end rows): one pass in float32 (its end-plane
+    # factors), one pass per stage in float64 on 3D grids
+    assert p.with_(end_planes="zero").resolved_schedule() == "schr_1pass"
+    assert p.with_(end_planes="zero", precision="f64").resolved_schedule() == "stages"
 
 
 def test_schr_fused_kahan_compensates(rng):

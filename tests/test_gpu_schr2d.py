@@ -1,0 +1,158 @@
+"""The one-pass Schrodinger schedule on 2D grids (ny = 1: schr2d_pass, one wave per
+56-column x tile and z chunk, x neighbours exchanged between lanes) and with the end-plane
+rule of the reference's 2D program (stage inputs zero on the end rows), against the FP64 CPU
+oracle, which evaluates that rule stage by stage (``oracle/fdtd3d.py``, pinned by the
+sparse-matrix brute force in ``tests/test_oracle.py``).
+
+Tolerances: relative L2 1e-12 in float64 (rounding only); 2e-6 in float32 after up to 20
+steps (as test_gpu_parity); against the stage schedule in float32, 2e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle import fdtd3d
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from wideanglepropagation_amd.engine import FDTD3D
+    return FDTD3D
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _params(shape, **kw):
+    from wideanglepropagation_amd.physics import schrodinger_params
+    from wideanglepropagation_amd.workloads import centred_extent
+    ext = centred_extent(shape, 0.05)
+    return schrodinger_params(0.05, ext, shape, pml_thick=kw.pop("pml_thick", 0.15),
+                              k_source=kw.pop("k_source", 3),
+                              precision=kw.pop("precision", "f32"),
+                              end_planes=kw.pop("end_planes", "zero"), **kw)
+
+
+def _case(p, rng, vscale=2e3):
+    V = (vscale * rng.standard_normal(p.shape)).astype(np.float32)
+    psi = (rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)) * 0.5
+    return V, psi
+
+
+def _gpu_run(p, V, psi, nsteps, profile=None):
+    FDTD3D = _gpu()
+    sim = FDTD3D(p, V=V if p.precision != "f64" else V.astype(np.float64), source_profile=profile)
+    sim.set_state(psi)
+    sim.step(nsteps)
+    out = sim.get_state(np.complex128)
+    sim.close()
+    return out
+
+
+def _oracle(p, V, psi, nsteps, profile=None):
+    return fdtd3d.run(psi, V.astype(np.float64), nsteps, p.oracle_dict(src_profile=profile),
+                      f32_time=(p.time_mode == "f32"))
+
+
+def test_2d_grids_default_to_one_pass():
+    p = _params((30, 1, 64))
+    assert p.resolved_schedule() == "schr_1pass"
+    assert p.with_(precision="f64").resolved_schedule() == "schr_1pass"
+    assert p.with_(end_planes="frozen").resolved_schedule() == "schr_1pass"
+    FDTD3D = _gpu()
+    sim = FDTD3D(p)
+    ty, tz, groups = sim.launch_shape()
+    sim.close()
+    assert ty == 1 and groups == 2 * ((30 + tz - 1) // tz)   # 64 columns: two 56-column tiles
+
+
+# (nz, 1, nx) and z chunk: one tile narrower than the grid's 56 columns (32), exactly one
+# tile (56), a one-column partial tile (57), several tiles with a partial last one (130,
+# 200), one-plane and short chunks, chunks cut inside the source band, a whole-grid chunk
+@pytest.mark.parametrize("shape,tz", [((20, 1, 32), 1), ((20, 1, 56), 2), ((21, 1, 57), 5),
+                                      ((33, 1, 130), 0), ((40, 1, 200), 7), ((24, 1, 64), 24)])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_2d_one_pass_matches_oracle(shape, tz, prec, rng):
+    print(f"case schr2d {prec} {shape} tz={tz}", flush=True)
+    p = _params(shape, tile_z=tz, precision=prec, schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 10)
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < (1e-12 if prec == "f64" else 2e-6)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_2d_one_pass_frozen_ends(prec, rng):
+    # a 2D grid with the 3D rule (every stage input keeps b on the end planes)
+    print(f"case schr2d frozen {prec}", flush=True)
+    p = _params((26, 1, 90), precision=prec, end_planes="frozen", schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 12)
+    assert rel_l2(got, _oracle(p, V, psi, 12)) < (1e-12 if prec == "f64" else 2e-6)
+
+
+def test_2d_one_pass_source_profile_and_reference_clock(rng):
+    print("case schr2d source profile + f32 clock", flush=True)
+    p = _params((30, 1, 100), reference_quantize=True, k_source=6, schedule="schr_1pass")
+    assert p.time_mode == "f32"
+    prof = np.exp(-np.linspace(-1, 1, 100) ** 2)[None, :] * np.exp(0.4j)
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 20, profile=prof)
+    assert rel_l2(got, _oracle(p, V, psi, 20, profile=prof)) < 2e-6
+
+
+def test_2d_one_pass_equals_stages(rng):
+    print("case schr2d vs stages", flush=True)
+    p = _params((64, 1, 300), pml_thick=0.4, k_source=12, schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    a = _gpu_run(p, V, psi, 15)
+    b = _gpu_run(p.with_(schedule="stages"), V, psi, 15)
+    err = rel_l2(a, b)
+    print(f"rel L2 one pass vs stages: {err:.3e}", flush=True)
+    assert err < 2e-6
+
+
+def test_3d_zero_end_planes_one_pass(rng):
+    # the end-plane factors of the 3D one-pass kernel (zero end planes on a 3D grid)
+    print("case schr_1pass 3D zero end planes", flush=True)
+    p = _params((18, 12, 70), end_planes="zero", schedule="schr_1pass", tile_z=5)
+    assert p.with_(schedule="auto").resolved_schedule() == "schr_1pass"
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 10)
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < 2e-6
+
+
+def test_2d_batch_matches_oracle(rng):
+    # batched 2D grids run the 3D one-pass kernel with the 2D end-plane factors
+    _gpu()
+    from wideanglepropagation_amd.batched import BatchedFDTD
+    from wideanglepropagation_amd.engine import FDTD3D
+    print("case batched 2D one pass", flush=True)
+    ps = [_params((22, 1, 40), k_source=3), _params((30, 1, 90), k_source=5)]
+    cases = [_case(p, rng) for p in ps]
+    sims = []
+    for p, (V, psi) in zip(ps, cases):
+        s = FDTD3D(p, V=V)
+        s.set_state(psi)
+        sims.append(s)
+    b = BatchedFDTD(sims)
+    b.step(9)
+    for p, s, (V, psi) in zip(ps, sims, cases):
+        assert rel_l2(s.get_state(np.complex128), _oracle(p, V, psi, 9)) < 2e-6
+    b.close()
+    for s in sims:
+        s.close()
+
+
+def test_2d_float64_batch_rejected(rng):
+    FDTD3D = _gpu()
+    from wideanglepropagation_amd._lib import WapError
+    from wideanglepropagation_amd.batched import BatchedFDTD
+    p = _params((22, 1, 40), precision="f64")
+    s = FDTD3D(p, V=np.zeros(p.shape))
+    with pytest.raises(WapError, match="batched one-pass Schrodinger grids are float32"):
+        BatchedFDTD([s])
+    s.close()

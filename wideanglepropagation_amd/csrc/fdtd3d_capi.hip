@@ -407,8 +407,12 @@ bool schr_fused_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
          fused_plane_fits(d);
 }
-// the one-pass Schrodinger schedule: float32 without compensation
-bool schr_1pass_ok(const wap_desc &d) { return schr_fused_ok(d) && d.precision == WAP_PREC_F32; }
+// the one-pass Schrodinger schedule: float32 without compensation (2D grids: float64 too),
+// frozen or zero end planes (its levels' end-plane factors), fields < 2 GiB
+bool schr_1pass_ok(const wap_desc &d) {
+  return d.equation == WAP_EQ_SCHRODINGER && fused_plane_fits(d) &&
+         (d.precision == WAP_PREC_F32 || (d.precision == WAP_PREC_F64 && d.ny == 1));
+}
 
 int sched_of(const wap_handle *h) {
   if (!is_kg(h)) {
@@ -661,11 +665,50 @@ wap::Schr1Args<R> schr1_args(const wap_handle *h, bool batch = false) {
   a.f2 = (R)(dt / 3.0);
   a.f3 = (R)(dt / 2.0);
   a.f4 = (R)dt;
+  // end-plane outputs of levels 1..3: b (frozen planes) or b/2, b/3, b/6 (the 2D rule)
+  const bool zero_ends = h->d.end_planes == WAP_ENDS_ZERO;
+  a.g1 = zero_ends ? (R)0.5 : (R)1;
+  a.g2 = zero_ends ? (R)(1.0 / 3.0) : (R)1;
+  a.g3 = zero_ends ? (R)(1.0 / 6.0) : (R)1;
   a.pz_lo = h->pz_lo;
   a.pz_hi = h->pz_hi;
   a.hz = (R)h->pz_w;
   // the state ping-pongs between psi and yb: input, output and V per step
   a.stream_out = stream_stores(h->n, h->cs) ? 1 : 0;
+  return a;
+}
+
+// A single 2D grid (ny = 1) runs schr2d_pass: one wave per 56-column x tile and z chunk.
+// The waves are independent and up to 8 share a SIMD, which hides a step's latency once
+// there are a few per SIMD; beyond that a chunk's 6 halo steps are the cost.  Chunks of 8
+// planes were the fastest measured at 1398 x 1024 (17.7 us; 4: 20.2, 16: 21.2) and
+// 4096 x 2048 (46.9 us; 4: 75.2, 16: 49.1, 32: 56.4): 8 planes, halved while the grid
+// makes fewer than 2048 waves (two per SIMD), doubled while it still makes 65536.
+int schr2d_ntx(const wap_handle *h) { return (h->d.nx + wap::kSchr2dCols - 1) / wap::kSchr2dCols; }
+int schr2d_tile_z(const wap_handle *h) {
+  const int nz = h->d.nz;
+  if (h->d.tile_z > 0) return std::min(std::min(h->d.tile_z, nz), wap::kKG1MaxTileZ);
+  const long ntx = schr2d_ntx(h);
+  auto waves = [&](int tz) { return ntx * ((nz + tz - 1) / tz); };
+  int tz = 8;
+  while (tz > 2 && waves(tz) < 2048) tz /= 2;
+  while (2 * tz <= 64 && waves(2 * tz) >= 65536) tz *= 2;
+  return std::min(tz, nz);
+}
+int64_t schr2d_tiles(const wap_handle *h) {
+  const int tz = schr2d_tile_z(h);
+  return (int64_t)schr2d_ntx(h) * ((h->d.nz + tz - 1) / tz);
+}
+bool schr2d(const wap_handle *h) { return h->d.ny == 1; }
+template <typename R>
+wap::Schr1Args<R> schr2d_args(const wap_handle *h) {
+  wap::Schr1Args<R> a = schr1_args<R>(h);
+  const int tz = schr2d_tile_z(h);
+  a.tile_z = tz;
+  a.ntx = schr2d_ntx(h);
+  a.nty = 1;
+  a.ntz = (h->d.nz + tz - 1) / tz;
+  a.hy = (R)0;
   return a;
 }
 
@@ -675,16 +718,23 @@ template <typename R>
 int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                     double *pass_ms, bool record = false) {
   using C = wap::cplx<R>;
-  const dim3 block(wap::kWaveX, wap::kSchr1Rows + 1);
-  const dim3 grid((unsigned)schr1_tiles(h));
-  wap::Schr1Args<R> a = schr1_args<R>(h);
+  const bool two_d = schr2d(h);
+  const dim3 block = two_d ? dim3(wap::kWaveX) : dim3(wap::kWaveX, wap::kSchr1Rows + 1);
+  const dim3 grid((unsigned)(two_d ? schr2d_tiles(h) : schr1_tiles(h)));
+  wap::Schr1Args<R> a = two_d ? schr2d_args<R>(h) : schr1_args<R>(h);
   if (record) a.stream_out = 1;
   for (int64_t n = 0; n < nsteps; ++n) {
     fill_source(h, step_time(h, step0, n, t0), a);
     a.in = (const C *)h->psi;
     a.out = (C *)h->yb_psi;
     if (ev) WAP_HIP(hipEventRecord(ev[0], h->stream));
-    hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
+    if (two_d) {
+      hipLaunchKernelGGL((wap::schr2d_pass<R>), grid, block, 0, h->stream, a);
+    } else if constexpr (sizeof(R) == 4) {
+      hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
+    } else {
+      return fail(WAP_ERR_ARG, "the one-pass Schrodinger step is float32 on 3D grids");
+    }
     std::swap(h->psi, h->yb_psi);
     if (ev) {
       WAP_HIP(hipEventRecord(ev[1], h->stream));
@@ -767,7 +817,9 @@ int run_steps_kg2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEv
 int dispatch_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_t *ev,
                    double *pass_ms) {
   const bool kg = is_kg(h);
-  if (sched_of(h) == WAP_SCHED_SCHR_1PASS) return run_steps_schr1<float>(h, nsteps, step0, t0, ev, pass_ms);
+  if (sched_of(h) == WAP_SCHED_SCHR_1PASS)
+    return is_f64(h) ? run_steps_schr1<double>(h, nsteps, step0, t0, ev, pass_ms)
+                     : run_steps_schr1<float>(h, nsteps, step0, t0, ev, pass_ms);
   if (sched_of(h) == WAP_SCHED_SCHR_FUSED) {
     switch (h->d.precision) {
       case WAP_PREC_F64: return run_steps_schr2<double, false>(h, nsteps, step0, t0, ev, pass_ms);
@@ -855,7 +907,7 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "unknown schedule");
   if (d->schedule == WAP_SCHED_SCHR_1PASS && !schr_1pass_ok(*d))
     return fail(WAP_ERR_ARG, "the one-pass Schrodinger schedule needs the Schrodinger equation "
-                             "in f32, frozen end planes and fields under 2 GiB");
+                             "in f32 (f32 or f64 on 2D grids, ny = 1) and fields under 2 GiB");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
     return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
                              "frozen end planes and fields under 2 GiB");
@@ -1230,9 +1282,9 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (sched_of(h) == WAP_SCHED_SCHR_1PASS) {
-    if (tile_y) *tile_y = wap::kSchr1Rows + 1;
-    if (tile_z) *tile_z = schr1_tile_z(h);
-    if (groups) *groups = schr1_tiles(h);
+    if (tile_y) *tile_y = schr2d(h) ? 1 : wap::kSchr1Rows + 1;
+    if (tile_z) *tile_z = schr2d(h) ? schr2d_tile_z(h) : schr1_tile_z(h);
+    if (groups) *groups = schr2d(h) ? schr2d_tiles(h) : schr1_tiles(h);
     return WAP_OK;
   }
   if (sched_of(h) == WAP_SCHED_KG_FUSED || sched_of(h) == WAP_SCHED_SCHR_FUSED) {
@@ -1606,6 +1658,9 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
     if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set on a batched handle");
   }
   const int s0 = sched_of(h0);
+  if (s0 == WAP_SCHED_SCHR_1PASS && is_f64(h0))
+    return fail(WAP_ERR_ARG, "batched one-pass Schrodinger grids are float32 (float64 2D grids: "
+                             "create the handles with the stage schedule)");
   const bool fused = s0 == WAP_SCHED_KG_FUSED || s0 == WAP_SCHED_SCHR_FUSED ||
                      s0 == WAP_SCHED_SCHR_1PASS;
   if (fused ? (tile_y != 0 && tile_y < 4) : tile_y > 8)

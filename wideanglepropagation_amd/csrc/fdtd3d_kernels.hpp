@@ -1371,6 +1371,13 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 // whose plane lies in its range.  Planes outside the grid are loaded from the nearest end
 // plane; the LDS planes and register rings start at zero, so every value that a frozen
 // plane's zero factor multiplies is finite.
+//
+// End planes.  Level l returns g_l b on the end planes (g_4 = 1).  With every stage input
+// keeping b there (frozen planes, the 3D rule) the step is P4(hM) b: g = 1, 1, 1.  The
+// rule of the reference's 2D program (stage inputs y2..y4 zero on the end rows, k = 0
+// there, psi' = b) is the same chain with g = 1/2, 1/3, 1/6: with b = b' + e (e on the end
+// planes) its step is b + (P4(hM) - I) b' + h (1/6 + x/6 + x^2/12 + x^3/24) M e (x = hM),
+// which is what the end values b/2, b/3, b/6 of the inputs of levels 2, 3, 4 produce.
 template <typename R>
 struct Schr1Args {
   using C = cplx<R>;
@@ -1384,6 +1391,7 @@ struct Schr1Args {
   int ntx, nty, ntz;
   R hx, hy;
   R f1, f2, f3, f4;  // level factors h/4, h/3, h/2, h
+  R g1, g2, g3;      // end-plane outputs of levels 1..3, times b
   // planes pz_lo..pz_hi (outside the PML) have both z weights equal to the real hz there
   int pz_lo, pz_hi;
   R hz;
@@ -1588,8 +1596,8 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     if (has_o) w.o = inj<R>(w.o, sp, sp.c, prof, oo);
   };
   // one level at plane k: y its input there, zp / zm at k+1 / k-1, Lin the LDS plane of
-  // the input (lateral neighbours); the factor is 0 on the frozen planes and outside the grid
-  auto level = [&](const C *Lin, R fl, int k, C y, C zp, C zm, C base, R v) {
+  // the input (lateral neighbours); gl b on the end planes
+  auto level = [&](const C *Lin, R fl, R gl, int k, C y, C zp, C zm, C base, R v) {
     const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
     C hyv;
     if (k >= p.pz_lo && k <= p.pz_hi) {  // uniform: outside the PML
@@ -1598,7 +1606,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
       const int t = k - zbase;
       hyv = schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
     }
-    return horner<R>(base, (k > 0 && k < nz - 1) ? fl : R(0), hyv);
+    return (k > 0 && k < nz - 1) ? horner<R>(base, fl, hyv) : cscale<R>(base, gl);
   };
 
   // plane m_start + q lives in ring slot q mod 6; levels 1..3 keep their last two planes
@@ -1645,15 +1653,15 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     load_plane(m + 2, pp2);
     inject_plane(m + 1, pp1);
     C n1 = zero, n2 = zero, n3 = zero;
-    if (m >= lo1 && m <= hi1) n1 = level(L.lb[rd], p.f1, m, p0.b, pp1.b, pm1.b, p0.b, p0.v);
+    if (m >= lo1 && m <= hi1) n1 = level(L.lb[rd], p.f1, p.g1, m, p0.b, pp1.b, pm1.b, p0.b, p0.v);
     if (nlev >= 2 && m - 1 >= lo2 && m - 1 <= hi2)
-      n2 = level(L.l1[rd], p.f2, m - 1, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
+      n2 = level(L.l1[rd], p.f2, p.g2, m - 1, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
     if (nlev >= 3 && m - 2 >= lo3 && m - 2 <= hi3)
-      n3 = level(L.l2[rd], p.f3, m - 2, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
+      n3 = level(L.l2[rd], p.f3, p.g3, m - 2, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
     const int k4 = m - 3;
     const bool st = writer_wave && k4 >= k0 && k4 < k1;  // uniform
     C o = zero;
-    if (st) o = level(L.l3[rd], p.f4, k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+    if (st) o = level(L.l3[rd], p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
     buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
     finish(uc, pp1, n1, n2, n3);
   };
@@ -1761,6 +1769,239 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     schr1_body<R, ROWS, kStream>(a, t - tile_begin[inst], ph[inst]);
   else
     schr1_body<R, ROWS, kCached>(a, t - tile_begin[inst], ph[inst]);
+}
+
+// ------------------------------------------ Schrodinger step in one pass, 2D grids
+//
+// schr1_pass's four Horner levels on a grid of one row (ny = 1: the reference's 2D solver,
+// x periodic, z the propagation axis), one wavefront per (x tile, z chunk): lane l holds
+// column 56 tx - 4 + l (periodic in x) and the wave walks its chunk's planes as schr1_pass
+// does (step m: level l at plane m - l + 1, z neighbours in the lane's registers), taking
+// the x neighbours of each level's input from the adjacent lanes by whole-wave DPP
+// rotates, so it needs no LDS and no barrier.  Level l is exact on lanes l .. 63 - l and
+// lanes 4..59 write: 56 columns per wave.  A workgroup is one wave.  The end planes
+// follow Schr1Args's g (frozen rule or the 2D rule); the y term is absent.  Arguments:
+// Schr1Args with ntx the 56-column tiles, nty = 1.  HBM bytes per voxel: 20 (psi_n, V in;
+// psi_{n+1} out).
+constexpr int kSchr2dCols = kWaveX - 8;
+
+// y of lane i - 1 plus y of lane i + 1 (DPP wave_ror:1 gives lane i the value of lane
+// i - 1, wave_rol:1 that of lane i + 1, both rotating at the wave's ends).  Every lane of
+// the wave must be active.
+__device__ __forceinline__ uint32_t lane_prev_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13c, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane_next_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x134, 0xf, 0xf, false);
+}
+template <typename R>
+__device__ __forceinline__ R lane_nb_sum(R v) {
+  if constexpr (sizeof(R) == 4) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    return __builtin_bit_cast(R, lane_prev_u32(u)) + __builtin_bit_cast(R, lane_next_u32(u));
+  } else {
+    const u32x2 u = __builtin_bit_cast(u32x2, v);
+    u32x2 a, b;
+    a.x = lane_prev_u32(u.x);
+    a.y = lane_prev_u32(u.y);
+    b.x = lane_next_u32(u.x);
+    b.y = lane_next_u32(u.y);
+    return __builtin_bit_cast(R, a) + __builtin_bit_cast(R, b);
+  }
+}
+
+// PF: planes loaded ahead of the step that first needs them (ring of 6 or 8 planes)
+template <typename R, int CPOL, int PF>
+__device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int tile, const cplx<R> ph) {
+  using C = cplx<R>;
+  static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
+  constexpr int NR = PF == 1 ? 6 : 8;  // planes m-3 .. m+1+PF, an even count
+  const int tx = tile % p.ntx;
+  const int tz = tile / p.ntx;
+  const int lane = threadIdx.x;
+  const int nx = p.nx, nz = p.nz;
+  const int c = tx * kSchr2dCols - 4 + lane;
+  const int i = ((c % nx) + nx) % nx;
+  const bool writer = lane >= 4 && lane < kWaveX - 4 && c < nx;
+  const uint32_t plane = (uint32_t)nx;
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const uint32_t oc = (uint32_t)i * sizeof(C), ov = (uint32_t)i * sizeof(R);
+  const uint32_t ow = writer ? oc : kBufSkip;
+  const C zero = mkc<R>(R(0), R(0));
+  const C *prof = p.prof;
+  const int k0 = tz * p.tile_z;
+  const int k1 = min(nz, k0 + p.tile_z);
+  const int m_start = k0 - 3, m_end = k1 + 2;
+  const int lo1 = max(k0 - 3, 0), hi1 = min(k1 + 2, nz - 1);
+  const int lo2 = max(k0 - 2, 0), hi2 = min(k1 + 1, nz - 1);
+  const int lo3 = max(k0 - 1, 0), hi3 = min(k1, nz - 1);
+
+  struct Pl {  // one plane of this lane: b (raw until injected), V
+    C b;
+    R v;
+  };
+  auto load_plane = [&](int m, Pl &w) {
+    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    w.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
+    w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+  };
+  auto inject_plane = [&](int m, Pl &w) {
+    const int bnd = m - p.src_k0;
+    if (m < 0 || m >= nz || bnd < 0 || bnd >= p.src_n) return;
+    const SrcCoef<R> e = p.src_tab[bnd];
+    SrcPlane<R> sp;
+    sp.band = true;
+    sp.keep = e.keep;
+    sp.c = cmul<R>(e.c, ph);
+    sp.cq = sp.c;
+    w.b = inj<R>(w.b, sp, sp.c, prof, i);
+  };
+  auto xsum = [&](C y) { return mkc<R>(lane_nb_sum<R>(y.x), lane_nb_sum<R>(y.y)); };
+  // H y at a plane outside the PML (one real z weight hz)
+  auto hreal = [&](C y, C sx, C zp, C zm, R v) {
+    const R lx = sx.x - R(2) * y.x, mx = sx.y - R(2) * y.y;
+    const R lz = (zp.x + zm.x) - R(2) * y.x, mz = (zp.y + zm.y) - R(2) * y.y;
+    return mkc<R>(p.hx * lx + p.hz * lz + v * y.x, p.hx * mx + p.hz * mz + v * y.y);
+  };
+  // H y at plane k (0 <= k < nz)
+  auto hplane = [&](int k, C y, C sx, C zp, C zm, R v) {
+    if (k >= p.pz_lo && k <= p.pz_hi) return hreal(y, sx, zp, zm, v);  // uniform
+    const C wp = p.zc[2 * k], wm = p.zc[2 * k + 1];
+    const R lx = sx.x - R(2) * y.x, mx = sx.y - R(2) * y.y;
+    const C dp = csub<R>(zp, y), dm = csub<R>(y, zm);
+    return mkc<R>(p.hx * lx + (wp.x * dp.x - wp.y * dp.y) - (wm.x * dm.x - wm.y * dm.y) + v * y.x,
+                  p.hx * mx + (wp.x * dp.y + wp.y * dp.x) - (wm.x * dm.y + wm.y * dm.x) + v * y.y);
+  };
+  // one level at plane k (uniform control flow: the lane exchange needs every lane)
+  auto level = [&](R fl, R gl, int k, C y, C zp, C zm, C base, R v) {
+    const C hyv = hplane(k, y, xsum(y), zp, zm, v);
+    return (k > 0 && k < nz - 1) ? horner<R>(base, fl, hyv) : cscale<R>(base, gl);
+  };
+
+  // plane m_start + q in ring slot q mod NR; levels 1..3 keep their last two planes
+  // (plane k in slot (k - m_start) & 1)
+  Pl pq[NR];
+  C l1q[2], l2q[2], l3q[2];
+  for (int q = 0; q < NR; ++q) {
+    pq[q].b = zero;
+    pq[q].v = R(0);
+  }
+  l1q[0] = l1q[1] = l2q[0] = l2q[1] = l3q[0] = l3q[1] = zero;
+  load_plane(m_start - 1, pq[NR - 1]);
+  for (int q = 0; q <= PF; ++q) load_plane(m_start + q, pq[q]);
+  inject_plane(m_start - 1, pq[NR - 1]);
+  inject_plane(m_start, pq[0]);
+
+  auto rings = [&](auto uc, C n1, C n2, C n3) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    l1q[U & 1] = n1;
+    l2q[(U + 1) & 1] = n2;
+    l3q[U & 1] = n3;
+  };
+  auto step = [&](auto uc, const int m) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    Pl &pm3 = pq[(U + NR - 3) % NR], &pm2 = pq[(U + NR - 2) % NR], &pm1 = pq[(U + NR - 1) % NR];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % NR];
+    load_plane(m + 1 + PF, pq[(U + 1 + PF) % NR]);
+    inject_plane(m + 1, pp1);
+    C n1 = zero, n2 = zero, n3 = zero;
+    if (m >= lo1 && m <= hi1) n1 = level(p.f1, p.g1, m, p0.b, pp1.b, pm1.b, p0.b, p0.v);
+    if (m - 1 >= lo2 && m - 1 <= hi2)
+      n2 = level(p.f2, p.g2, m - 1, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
+    if (m - 2 >= lo3 && m - 2 <= hi3)
+      n3 = level(p.f3, p.g3, m - 2, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
+    const int k4 = m - 3;
+    const bool st = k4 >= k0 && k4 < k1;  // uniform
+    C o = zero;
+    if (st) o = level(p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
+    rings(uc, n1, n2, n3);
+  };
+  // A steady-state step: the four levels on planes m-3..m, all outside the PML (interior,
+  // one real z weight), no source plane at m+1 to inject, the plane loaded inside the grid,
+  // level 4's plane inside the chunk.  step's arithmetic without its range tests, clamps
+  // and weight loads.
+  auto fast_step = [&](auto uc, const int m) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    Pl &pm3 = pq[(U + NR - 3) % NR], &pm2 = pq[(U + NR - 2) % NR], &pm1 = pq[(U + NR - 1) % NR];
+    Pl &p0 = pq[U], &pp1 = pq[(U + 1) % NR], &pn = pq[(U + 1 + PF) % NR];
+    {
+      const uint32_t b = (uint32_t)(m + 1 + PF) * plane;
+      pn.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
+      pn.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+    }
+    const C n1 = horner<R>(p0.b, p.f1, hreal(p0.b, xsum(p0.b), pp1.b, pm1.b, p0.v));
+    const C y2 = l1q[(U + 1) & 1];
+    const C n2 = horner<R>(pm1.b, p.f2, hreal(y2, xsum(y2), n1, l1q[U & 1], pm1.v));
+    const C y3 = l2q[U & 1];
+    const C n3 = horner<R>(pm2.b, p.f3, hreal(y3, xsum(y3), n2, l2q[(U + 1) & 1], pm2.v));
+    const C y4 = l3q[(U + 1) & 1];
+    const C o = horner<R>(pm3.b, p.f4, hreal(y4, xsum(y4), n3, l3q[U & 1], pm3.v));
+    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), o);
+    rings(uc, n1, n2, n3);
+  };
+  // the steady-state steps (uniform), the range keeping the longer side of a source band
+  int mf_lo = max(k0 + 3, p.pz_lo + 3);
+  int mf_hi = min(min(k1 + 2, p.pz_hi), nz - 2 - PF);
+  if (p.src_n > 0) {
+    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2;  // steps injecting m + 1
+    if (sb_lo <= mf_hi && sb_hi >= mf_lo) {
+      if (sb_lo - mf_lo >= mf_hi - sb_hi)
+        mf_hi = sb_lo - 1;
+      else
+        mf_lo = sb_hi + 1;
+    }
+  }
+  auto general_block = [&](int m0) __attribute__((always_inline)) {
+    step(std::integral_constant<int, 0>{}, m0);
+    if (m0 + 1 > m_end) return;
+    step(std::integral_constant<int, 1>{}, m0 + 1);
+    if (m0 + 2 > m_end) return;
+    step(std::integral_constant<int, 2>{}, m0 + 2);
+    if (m0 + 3 > m_end) return;
+    step(std::integral_constant<int, 3>{}, m0 + 3);
+    if (m0 + 4 > m_end) return;
+    step(std::integral_constant<int, 4>{}, m0 + 4);
+    if (m0 + 5 > m_end) return;
+    step(std::integral_constant<int, 5>{}, m0 + 5);
+    if constexpr (NR == 8) {
+      if (m0 + 6 > m_end) return;
+      step(std::integral_constant<int, 6>{}, m0 + 6);
+      if (m0 + 7 > m_end) return;
+      step(std::integral_constant<int, 7>{}, m0 + 7);
+    }
+  };
+  auto fast_block = [&](int m0) __attribute__((always_inline)) {
+    fast_step(std::integral_constant<int, 0>{}, m0);
+    fast_step(std::integral_constant<int, 1>{}, m0 + 1);
+    fast_step(std::integral_constant<int, 2>{}, m0 + 2);
+    fast_step(std::integral_constant<int, 3>{}, m0 + 3);
+    fast_step(std::integral_constant<int, 4>{}, m0 + 4);
+    fast_step(std::integral_constant<int, 5>{}, m0 + 5);
+    if constexpr (NR == 8) {
+      fast_step(std::integral_constant<int, 6>{}, m0 + 6);
+      fast_step(std::integral_constant<int, 7>{}, m0 + 7);
+    }
+  };
+  // blocks of NR steps from m_start (the ring slot of plane m is (m - m_start) mod NR)
+  int m0 = m_start;
+  for (; m0 <= m_end && m0 < mf_lo; m0 += NR) general_block(m0);
+  for (; m0 + NR - 1 <= mf_hi; m0 += NR) fast_block(m0);
+  for (; m0 <= m_end; m0 += NR) general_block(m0);
+}
+
+// one plane of prefetch: float32 64 VGPRs, 8 waves per SIMD (two planes: 70 VGPRs, 7 waves,
+// 1398 x 1024: 18.3 against 17.0 us per step); float64 128 VGPRs
+constexpr int kSchr2dPf32 = 1, kSchr2dPf64 = 1;
+
+template <typename R>
+__global__ __launch_bounds__(kWaveX) void schr2d_pass(const Schr1Args<R> p) {
+  constexpr int PF = sizeof(R) == 4 ? kSchr2dPf32 : kSchr2dPf64;
+  const int tile = xcd_tile(blockIdx.x, p.ntx * p.ntz);
+  if (p.stream_out)
+    schr2d_body<R, kStream, PF>(p, tile, p.src_ph);
+  else
+    schr2d_body<R, kCached, PF>(p, tile, p.src_ph);
 }
 
 // ------------------------------------------------------------------ adjoint

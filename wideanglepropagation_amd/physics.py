@@ -140,9 +140,12 @@ class FDTDParams:
         fits = self.nx * self.ny * self.nz * (16 if self.precision == "f64" else 8) < 2 ** 31
         if self.equation != "klein_gordon":
             if self.schedule == "auto":
-                if self.end_planes != "frozen" or not fits:
+                if not fits:
                     return "stages"
-                return "schr_1pass" if self.precision == "f32" else "schr_fused"
+                # one pass: float32 (either end rule), float64 on 2D grids (ny = 1)
+                if self.precision == "f32" or (self.precision == "f64" and self.ny == 1):
+                    return "schr_1pass"
+                return "schr_fused" if self.end_planes == "frozen" else "stages"
             return self.schedule
         if self.schedule == "auto":
             return "kg_fused" if fits else "kg_2pass"
