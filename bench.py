@@ -61,8 +61,9 @@ PASS_BYTES = {
     ("schr_fused", "schrodinger", "f32"): (20, 28),
     ("schr_fused", "schrodinger", "f64"): (40, 56),
     ("schr_fused", "schrodinger", "f32_kahan"): (20, 44),  # pass B + the compensation, R/W
-    # the four Horner levels in one pass: read psi_n, V; write psi_{n+1}
+    # the four Horner levels in one pass: read psi_n, V; write psi_{n+1} (f64: 2D grids)
     ("schr_1pass", "schrodinger", "f32"): (20,),
+    ("schr_1pass", "schrodinger", "f64"): (40,),
 }
 
 
@@ -100,7 +101,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f32_kahan", "f64"])
     ap.add_argument("--equation", default="klein_gordon", choices=["klein_gordon", "schrodinger"])
-    ap.add_argument("--shape", default="512,256,256", help="nz,ny,nx")
+    ap.add_argument("--shape", default="512,256,256", help="nz,ny,nx (ny = 1: a 2D grid)")
+    ap.add_argument("--end-planes", default="frozen", choices=["frozen", "zero"],
+                    help="Schrodinger end-plane rule: frozen (3D reference) or zero stage "
+                         "inputs on the end rows (the 2D reference)")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-y", type=int, default=0)
@@ -225,8 +229,20 @@ def main():
                                precision=args.precision)
         from wideanglepropagation_amd.workloads import atom_potential, random_slab
         pos, Zs, scr = random_slab(400, ((ext[0], ext[1]), (ext[2], ext[3])), (-10, 10), 0)
-        V = atom_potential(shape, ext, pos, Zs, scr, 0.05, device="cuda")
+        if ny == 1:
+            # a 2D grid: the atoms in the x-z plane, as row 0 of a two-row grid whose other
+            # row (and the periodic images) lie 100 A away, beyond the cutoff
+            pos[:, 1] = 0.0
+            ext2 = (ext[0], ext[1], 0.0, 100.0, ext[4], ext[5])
+            V = atom_potential((nz, 2, nx), ext2, pos, Zs, scr, 0.05, device="cuda")
+            V = V[:, :1, :].contiguous()
+        else:
+            V = atom_potential(shape, ext, pos, Zs, scr, 0.05, device="cuda")
     p = p.with_(tile_y=args.tile_y, tile_z=args.tile_z, schedule=args.schedule)
+    if args.end_planes != "frozen":
+        if args.equation != "schrodinger":
+            sys.exit("--end-planes zero is the Schrodinger 2D rule")
+        p = p.with_(end_planes=args.end_planes)
     V = V.to(torch.float64 if args.precision == "f64" else torch.float32).contiguous()
     sim = FDTD3D(p, V=V)
     del V
@@ -260,7 +276,7 @@ def main():
     step_gbs = step_bytes / (sum(avg) * 1e-3) / 1e9
     norm = sim.norm2()
     adjoint = None
-    if args.adjoint_steps > 0 and p.precision != "f32_kahan":
+    if args.adjoint_steps > 0 and p.precision != "f32_kahan" and p.end_planes == "frozen":
         rec_s, rev_s = adjoint_cost(sim, args.adjoint_steps)
         step_s = elapsed / args.steps
         T = 1500
@@ -284,7 +300,8 @@ def main():
     from wideanglepropagation_amd._lib import stream_bandwidth
     copy_gbs, read_gbs = stream_bandwidth(2 << 30, 10)
 
-    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass", "schr_1pass": "schr1_pass",
+    one_pass = "schr2d_pass" if ny == 1 else "schr1_pass"
+    kernel = {"kg_2pass": f"kg2_pass_p{dom + 1}", "kg_fused": "kg1_pass", "schr_1pass": one_pass,
               "schr_fused": f"schr2_pass_p{dom + 1}"}.get(sched, f"rk4_pass_s{dom + 1}")
     # the Schrodinger summary covers both passes and quotes pass B's bytes
     traffic, traffic_src = (profiled_traffic("schr2_pass", p) if sched == "schr_fused" and dom == 1
@@ -308,14 +325,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{p.equation} RK4 FDTD {nx}x{ny}x{nz}, "
                                f"{'complex128' if p.precision == 'f64' else 'complex64'} fields "
-                               f"({p.precision}), 400-atom slab, PML 16 planes, 200 keV",
+                               f"({p.precision}), 400-atom slab, PML 16 planes, 200 keV"
+                               + (", zero stage inputs on the end rows (the 2D rule)"
+                                  if p.end_planes == "zero" else ""),
                    "grid": [nz, ny, nx], "precision": p.precision,
                    "equation": p.equation, "schedule": sched, "tile_y": ty, "tile_z": tz},
         "roofline": {"bound": "hbm", "achieved": round(comp_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": {"kg_2pass": f"kg2_pass pass {dom + 1}",
-                                "kg_fused": "kg1_pass", "schr_1pass": "schr1_pass",
+                                "kg_fused": "kg1_pass", "schr_1pass": one_pass,
                                 "schr_fused": f"schr2_pass pass {'AB'[dom]}"}.get(
                                     sched, f"rk4_pass stage {dom + 1}"),
                      "basis": f"algorithmic {pb[dom]} B per voxel of the executed {sched} "

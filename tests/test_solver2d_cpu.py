@@ -150,3 +150,9 @@ def test_2d_params_map_onto_the_3d_grid():
                         {"reference_quantize": False, "precision": "f64"})
     assert exact.extent[4] == -2.05 and exact.dt == 1e-6 and exact.precision == "f64"
     assert exact.k_beam == pytest.approx(math.sqrt(2 * exact.e_sim), rel=1e-15)
+    # one pass per step by default in both precisions; the schedule and chunk keywords
+    # reach the grid
+    assert p.resolved_schedule() == "schr_1pass" and exact.resolved_schedule() == "schr_1pass"
+    st = s._params2d((40, 16), (-1.0, 1.0, -2.05, 2.0468), 0.05, 0.4, 80.0, 7,
+                     {"schedule": "stages", "tile_z": 5})
+    assert st.resolved_schedule() == "stages" and st.tile_z == 5

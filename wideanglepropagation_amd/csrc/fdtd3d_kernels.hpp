@@ -1787,12 +1787,14 @@ constexpr int kSchr2dCols = kWaveX - 8;
 
 // y of lane i - 1 plus y of lane i + 1 (DPP wave_ror:1 gives lane i the value of lane
 // i - 1, wave_rol:1 that of lane i + 1, both rotating at the wave's ends).  Every lane of
-// the wave must be active.
+// the wave must be active.  A rotate has a source lane for every lane, so bound_ctrl
+// changes no value; set, it lets the compiler fold one rotate of each pair into the add
+// (v_add_f32_dpp): 435 instead of 483 VALU instructions per six float32 steady-state steps.
 __device__ __forceinline__ uint32_t lane_prev_u32(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13c, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13c, 0xf, 0xf, true);
 }
 __device__ __forceinline__ uint32_t lane_next_u32(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x134, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x134, 0xf, 0xf, true);
 }
 template <typename R>
 __device__ __forceinline__ R lane_nb_sum(R v) {

@@ -168,7 +168,8 @@ class RelativisticSolver:
                           k_beam=k_beam, equation="schrodinger",
                           precision=kw.get("precision", "f32"), end_planes="zero",
                           src_enabled=bool(kw.get("source", True)), src_k=int(j_source),
-                          **base)
+                          schedule=str(kw.get("schedule", "auto")),
+                          tile_z=int(kw.get("tile_z", 0)), **base)
 
     def run(self, V_map, wavelength_angstrom, extent,
             n_frames=600, steps_per_frame=100, snapshot_count=None,
