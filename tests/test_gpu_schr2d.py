@@ -84,6 +84,19 @@ def test_2d_one_pass_matches_oracle(shape, tz, prec, rng):
     assert rel_l2(got, _oracle(p, V, psi, 10)) < (1e-12 if prec == "f64" else 2e-6)
 
 
+# long grids: chunks whose first steps run the steady-state ramp (outside the PML and the
+# source band), blocks mixing steady-state and general steps, a chunk edge inside the band
+@pytest.mark.parametrize("tz,k_source", [(8, 60), (13, 57), (0, 33)])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_2d_one_pass_long_grid(tz, k_source, prec, rng):
+    print(f"case schr2d long {prec} tz={tz} k_source={k_source}", flush=True)
+    p = _params((120, 1, 100), tile_z=tz, precision=prec, pml_thick=0.3, k_source=k_source,
+                schedule="schr_1pass")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 10)
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < (1e-12 if prec == "f64" else 2e-6)
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_2d_one_pass_frozen_ends(prec, rng):
     # a 2D grid with the 3D rule (every stage input keeps b on the end planes)

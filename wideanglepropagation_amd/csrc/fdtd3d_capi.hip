@@ -680,10 +680,10 @@ wap::Schr1Args<R> schr1_args(const wap_handle *h, bool batch = false) {
 
 // A single 2D grid (ny = 1) runs schr2d_pass: one wave per 56-column x tile and z chunk.
 // The waves are independent and up to 8 share a SIMD, which hides a step's latency once
-// there are a few per SIMD; beyond that a chunk's 6 halo steps are the cost.  Chunks of 8
-// planes were the fastest measured at 1398 x 1024 (17.7 us; 4: 20.2, 16: 21.2) and
-// 4096 x 2048 (46.9 us; 4: 75.2, 16: 49.1, 32: 56.4): 8 planes, halved while the grid
-// makes fewer than 2048 waves (two per SIMD), doubled while it still makes 65536.
+// there are a few per SIMD; beyond that a chunk's 6 ramp steps are the cost.  Measured (us
+// per step, f32): 1398 x 1024: 4 planes 15.5, 8: 16.8, 16: 21.3; 4096 x 2048: 4: 46.6,
+// 8: 44.7, 16: 48.8, 32: 56.7.  8 planes, halved while the grid makes fewer than 4096
+// waves (four per SIMD), doubled while it still makes 65536.
 int schr2d_ntx(const wap_handle *h) { return (h->d.nx + wap::kSchr2dCols - 1) / wap::kSchr2dCols; }
 int schr2d_tile_z(const wap_handle *h) {
   const int nz = h->d.nz;
@@ -691,7 +691,7 @@ int schr2d_tile_z(const wap_handle *h) {
   const long ntx = schr2d_ntx(h);
   auto waves = [&](int tz) { return ntx * ((nz + tz - 1) / tz); };
   int tz = 8;
-  while (tz > 2 && waves(tz) < 2048) tz /= 2;
+  while (tz > 2 && waves(tz) < 4096) tz /= 2;
   while (2 * tz <= 64 && waves(2 * tz) >= 65536) tz *= 2;
   return std::min(tz, nz);
 }

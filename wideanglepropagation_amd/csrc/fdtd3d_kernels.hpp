@@ -1942,54 +1942,92 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
     buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), o);
     rings(uc, n1, n2, n3);
   };
-  // the steady-state steps (uniform), the range keeping the longer side of a source band
-  int mf_lo = max(k0 + 3, p.pz_lo + 3);
-  int mf_hi = min(min(k1 + 2, p.pz_hi), nz - 2 - PF);
-  if (p.src_n > 0) {
-    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2;  // steps injecting m + 1
-    if (sb_lo <= mf_hi && sb_hi >= mf_lo) {
-      if (sb_lo - mf_lo >= mf_hi - sb_hi)
-        mf_hi = sb_lo - 1;
+  // the steady-state steps (uniform): m in [mf_lo, mf_hi], m + 1 not a source plane
+  const int mf_lo = max(k0 + 3, p.pz_lo + 3);
+  const int mf_hi = min(min(k1 + 2, p.pz_hi), nz - 2 - PF);
+  const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2;  // steps injecting m + 1
+  auto fast_ok = [&](int m) {
+    return m >= mf_lo && m <= mf_hi && (p.src_n == 0 || m < sb_lo || m > sb_hi);
+  };
+  // The ramp: the first steps of a chunk (m = k0-3+U) run level 1 from U = 0, level 2 from
+  // U = 2, level 3 from U = 4 and level 4 from U = 6, a pattern known at compile time in the
+  // first block.  Steady-state form when the planes k0-3..k0+2 those levels touch are
+  // outside the PML, no plane k0-2..k0+3 is a source plane to inject and the loads stay
+  // inside the grid.
+  auto ramp_step = [&](auto uc, const int m) __attribute__((always_inline)) {
+    constexpr int U = decltype(uc)::value;
+    if constexpr (U >= 6) {  // level 4 from here: a full step (NR = 8)
+      if (fast_ok(m))
+        fast_step(uc, m);
       else
-        mf_lo = sb_hi + 1;
+        step(uc, m);
+    } else {
+      Pl &pm2 = pq[(U + NR - 2) % NR], &pm1 = pq[(U + NR - 1) % NR];
+      Pl &p0 = pq[U], &pp1 = pq[(U + 1) % NR], &pn = pq[(U + 1 + PF) % NR];
+      {
+        const uint32_t b = (uint32_t)(m + 1 + PF) * plane;
+        pn.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
+        pn.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
+      }
+      const C n1 = horner<R>(p0.b, p.f1, hreal(p0.b, xsum(p0.b), pp1.b, pm1.b, p0.v));
+      C n2 = zero, n3 = zero;
+      if constexpr (U >= 2) {
+        const C y2 = l1q[(U + 1) & 1];
+        n2 = horner<R>(pm1.b, p.f2, hreal(y2, xsum(y2), n1, l1q[U & 1], pm1.v));
+      }
+      if constexpr (U >= 4) {
+        const C y3 = l2q[U & 1];
+        n3 = horner<R>(pm2.b, p.f3, hreal(y3, xsum(y3), n2, l2q[(U + 1) & 1], pm2.v));
+      }
+      rings(uc, n1, n2, n3);
     }
-  }
-  auto general_block = [&](int m0) __attribute__((always_inline)) {
-    step(std::integral_constant<int, 0>{}, m0);
+  };
+  const bool ramp_ok = k0 - 3 >= p.pz_lo && k0 + 2 <= p.pz_hi && k0 + 3 + PF <= nz - 1 &&
+                       (p.src_n == 0 || p.src_k0 + p.src_n - 1 < k0 - 2 || p.src_k0 > k0 + 3) &&
+                       m_start + NR - 1 <= m_end;
+  // a block of NR steps from m0 (the ring slot of plane m is (m - m_start) mod NR), each step
+  // in steady-state or general form
+  auto block = [&](int m0) __attribute__((always_inline)) {
+    auto one = [&](auto uc) __attribute__((always_inline)) {
+      const int m = m0 + decltype(uc)::value;
+      if (fast_ok(m))
+        fast_step(uc, m);
+      else
+        step(uc, m);
+    };
+    one(std::integral_constant<int, 0>{});
     if (m0 + 1 > m_end) return;
-    step(std::integral_constant<int, 1>{}, m0 + 1);
+    one(std::integral_constant<int, 1>{});
     if (m0 + 2 > m_end) return;
-    step(std::integral_constant<int, 2>{}, m0 + 2);
+    one(std::integral_constant<int, 2>{});
     if (m0 + 3 > m_end) return;
-    step(std::integral_constant<int, 3>{}, m0 + 3);
+    one(std::integral_constant<int, 3>{});
     if (m0 + 4 > m_end) return;
-    step(std::integral_constant<int, 4>{}, m0 + 4);
+    one(std::integral_constant<int, 4>{});
     if (m0 + 5 > m_end) return;
-    step(std::integral_constant<int, 5>{}, m0 + 5);
+    one(std::integral_constant<int, 5>{});
     if constexpr (NR == 8) {
       if (m0 + 6 > m_end) return;
-      step(std::integral_constant<int, 6>{}, m0 + 6);
+      one(std::integral_constant<int, 6>{});
       if (m0 + 7 > m_end) return;
-      step(std::integral_constant<int, 7>{}, m0 + 7);
+      one(std::integral_constant<int, 7>{});
     }
   };
-  auto fast_block = [&](int m0) __attribute__((always_inline)) {
-    fast_step(std::integral_constant<int, 0>{}, m0);
-    fast_step(std::integral_constant<int, 1>{}, m0 + 1);
-    fast_step(std::integral_constant<int, 2>{}, m0 + 2);
-    fast_step(std::integral_constant<int, 3>{}, m0 + 3);
-    fast_step(std::integral_constant<int, 4>{}, m0 + 4);
-    fast_step(std::integral_constant<int, 5>{}, m0 + 5);
-    if constexpr (NR == 8) {
-      fast_step(std::integral_constant<int, 6>{}, m0 + 6);
-      fast_step(std::integral_constant<int, 7>{}, m0 + 7);
-    }
-  };
-  // blocks of NR steps from m_start (the ring slot of plane m is (m - m_start) mod NR)
   int m0 = m_start;
-  for (; m0 <= m_end && m0 < mf_lo; m0 += NR) general_block(m0);
-  for (; m0 + NR - 1 <= mf_hi; m0 += NR) fast_block(m0);
-  for (; m0 <= m_end; m0 += NR) general_block(m0);
+  if (ramp_ok) {
+    ramp_step(std::integral_constant<int, 0>{}, m0);
+    ramp_step(std::integral_constant<int, 1>{}, m0 + 1);
+    ramp_step(std::integral_constant<int, 2>{}, m0 + 2);
+    ramp_step(std::integral_constant<int, 3>{}, m0 + 3);
+    ramp_step(std::integral_constant<int, 4>{}, m0 + 4);
+    ramp_step(std::integral_constant<int, 5>{}, m0 + 5);
+    if constexpr (NR == 8) {
+      ramp_step(std::integral_constant<int, 6>{}, m0 + 6);
+      ramp_step(std::integral_constant<int, 7>{}, m0 + 7);
+    }
+    m0 += NR;
+  }
+  for (; m0 <= m_end; m0 += NR) block(m0);
 }
 
 // one plane of prefetch: float32 64 VGPRs, 8 waves per SIMD (two planes: 70 VGPRs, 7 waves,
