@@ -169,3 +169,24 @@ def test_2d_float64_batch_rejected(rng):
     with pytest.raises(WapError, match="batched one-pass Schrodinger grids are float32"):
         BatchedFDTD([s])
     s.close()
+
+
+def test_2d_adjoint_records_from_the_2d_step(rng):
+    # a 2D grid with frozen end planes (the adjoint's rule): records made by schr2d_pass,
+    # reverse steps by schr_adj1_pass on the one-row grid, against the two-pass pair
+    _gpu()
+    from wideanglepropagation_amd.adjoint import exit_wave, far_field_intensity
+    print("case schr2d adjoint", flush=True)
+    p1 = _params((40, 1, 64), end_planes="frozen", k_source=4, schedule="schr_1pass")
+    V0, psi0 = _case(p1, rng, vscale=300.0)
+    grads = []
+    for p in (p1, p1.with_(schedule="schr_fused")):
+        V = torch.tensor(V0, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = exit_wave(V, p, 20, p.nz - 5, samples={20: 1.0, 14: 0.5}, checkpoint_every=5,
+                      psi0=psi0)
+        torch.sum(far_field_intensity(E) ** 2).backward()
+        grads.append(V.grad.detach().double().cpu().numpy())
+    assert np.abs(grads[1]).max() > 0
+    rel = rel_l2(grads[0], grads[1])
+    print(f"2D one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
+    assert rel < 1e-4
