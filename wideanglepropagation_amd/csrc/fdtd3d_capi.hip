@@ -371,13 +371,18 @@ int run_steps(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hipEvent_
   return WAP_OK;
 }
 
-// the schedule a handle runs (WAP_SCHED_STAGES, _KG_2PASS or _KG_FUSED)
-// The fused passes address a field with a buffer descriptor and 32-bit offsets, and
-// skip lanes with an offset of 2^31: a field must stay below 2 GiB.
-bool fused_plane_fits(const wap_desc &d) {
-  const size_t cs = d.precision == WAP_PREC_F64 ? 16 : 8;
-  return (size_t)d.nx * d.ny * d.nz * cs < ((size_t)1 << 31);
+// The fused passes address each field through buffer descriptors based at the first plane
+// of their workgroup's window (its z chunk and the chunk's halo: at most tile_z + 9
+// planes) with 32-bit byte offsets, and skip lanes with an offset of 2^31: a window must
+// stay below 2 GiB.  The longest chunk whose window fits (capped at the LDS tables' 256
+// planes); the fused schedules need one of at least 1 plane, i.e. a plane under
+// 2^31 / 10 bytes (26.8 M voxels in complex64), whatever nz.
+int window_tz_max(const wap_desc &d) {
+  const int64_t cs = d.precision == WAP_PREC_F64 ? 16 : 8;
+  const int64_t planes = ((((int64_t)1 << 31) - 1) / ((int64_t)d.nx * d.ny * cs)) - 9;
+  return (int)std::min<int64_t>(planes, wap::kKG1MaxTileZ);
 }
+bool fused_plane_fits(const wap_desc &d) { return window_tz_max(d) >= 1; }
 
 // The fused passes store their results with non-temporal stores when the fields one
 // step touches (4 of the state's size: in and out of two components, or input, u3 and
@@ -455,12 +460,10 @@ int kg1_nty(const wap_handle *h) { return kg1_nty(h, kg1_waves(h)); }
 // (64^3: 2-plane chunks run 1.6x as fast as 8-plane ones).  A grid in a batch shares
 // the CUs with the other grids, where short chunks only add halo planes: from 8.
 int chunk_tile_z(const wap_handle *h, long cols, int per_cu, int halo, bool batch) {
-  if (h->d.tile_z > 0) {
-    const int tz = h->d.tile_z < h->d.nz ? h->d.tile_z : h->d.nz;
-    return tz < wap::kKG1MaxTileZ ? tz : wap::kKG1MaxTileZ;
-  }
+  const int cap = window_tz_max(h->d);  // <= kKG1MaxTileZ
+  if (h->d.tile_z > 0) return std::min(std::min(h->d.tile_z, h->d.nz), cap);
   const long slots = 256L * per_cu;
-  const int tmax = h->d.nz < wap::kKG1MaxTileZ ? h->d.nz : wap::kKG1MaxTileZ;
+  const int tmax = std::min(h->d.nz, cap);
   const int tmin = batch ? 8 : 2;
   int best = tmax;
   double best_cost = 1e300;
@@ -687,13 +690,14 @@ wap::Schr1Args<R> schr1_args(const wap_handle *h, bool batch = false) {
 int schr2d_ntx(const wap_handle *h) { return (h->d.nx + wap::kSchr2dCols - 1) / wap::kSchr2dCols; }
 int schr2d_tile_z(const wap_handle *h) {
   const int nz = h->d.nz;
-  if (h->d.tile_z > 0) return std::min(std::min(h->d.tile_z, nz), wap::kKG1MaxTileZ);
+  const int cap = window_tz_max(h->d);
+  if (h->d.tile_z > 0) return std::min(std::min(h->d.tile_z, nz), cap);
   const long ntx = schr2d_ntx(h);
   auto waves = [&](int tz) { return ntx * ((nz + tz - 1) / tz); };
   int tz = 8;
   while (tz > 2 && waves(tz) < 4096) tz /= 2;
   while (2 * tz <= 64 && waves(2 * tz) >= 65536) tz *= 2;
-  return std::min(tz, nz);
+  return std::min(std::min(tz, nz), cap);
 }
 int64_t schr2d_tiles(const wap_handle *h) {
   const int tz = schr2d_tile_z(h);

@@ -91,6 +91,14 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, kBufRange, kBufFlags);
 }
+// A plane-walking workgroup addresses each field from zb, the lowest plane it touches: a
+// descriptor based there keeps the 32-bit byte offsets of its planes (its chunk and the
+// chunk's z halo) under 2 GiB however large the field.  Plane offsets are taken from zb
+// and never below it (a skipped access keeps an in-window plane offset).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc_from(const T *base, int zb, uint32_t plane) {
+  return buf_rsrc(base ? base + (size_t)zb * plane : base);
+}
 // Pins a register value at this point of the program: an empty asm statement that reads
 // and rewrites it (no instruction is emitted), so nothing computed from the value can be
 // moved above it.  Placed where a prefetched value is consumed, it keeps the compiler's
@@ -801,10 +809,14 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;  // uniform
   const bool writer = wave_writes && gi < nx && gj < ny;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rsp = buf_rsrc(p.in_psi), rsq = buf_rsrc(p.in_pi);
-  const __amdgpu_buffer_rsrc_t rsv = buf_rsrc(p.V);
-  const __amdgpu_buffer_rsrc_t rop = buf_rsrc(p.out_psi), roq = buf_rsrc(p.out_pi);
-  const __amdgpu_buffer_rsrc_t rcs = buf_rsrc(p.cmp_psi), rcq = buf_rsrc(p.cmp_pi);
+  const int zb = max(tz * p.tile_z - 2, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rsp = buf_rsrc_from(p.in_psi, zb, plane);
+  const __amdgpu_buffer_rsrc_t rsq = buf_rsrc_from(p.in_pi, zb, plane);
+  const __amdgpu_buffer_rsrc_t rsv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rop = buf_rsrc_from(p.out_psi, zb, plane);
+  const __amdgpu_buffer_rsrc_t roq = buf_rsrc_from(p.out_pi, zb, plane);
+  const __amdgpu_buffer_rsrc_t rcs = buf_rsrc_from(p.cmp_psi, zb, plane);
+  const __amdgpu_buffer_rsrc_t rcq = buf_rsrc_from(p.cmp_pi, zb, plane);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;  // LDS slot
   // the outer neighbour this lane loads for the halo of S: x for side lanes,
@@ -843,7 +855,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   // Planes outside the grid are never used in arithmetic (the end planes are
   // frozen), so a valid plane is loaded in their place.
   auto load_plane = [&](int m, Raw &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     w.s = buf_ld<C>(rsp, oc, bc);
     w.q = buf_ld<C>(rsq, oc, bc);
@@ -893,7 +905,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
     C s, q;
   };
   auto load_cmp = [&](int m, Cmp &w) {
-    const uint32_t bc = (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(C);
+    const uint32_t bc = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane * (uint32_t)sizeof(C);
     const uint32_t o = (m >= k0 && m < k1) ? ow : kBufSkip;
     w.s = buf_ld<C>(rcs, o, bc);
     w.q = buf_ld<C>(rcq, o, bc);
@@ -986,7 +998,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
       }
     }
     {
-      const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
+      const uint32_t bc = (uint32_t)(max(k, zb) - zb) * plane * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
       buf_st<C, CPOL>(rop, o, bc, opsi);
       buf_st<C, CPOL>(roq, o, bc, opi);
@@ -1019,7 +1031,7 @@ __device__ __forceinline__ void kg1_body(const KG1Args<R> &p, const int tile, co
   if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, c0, c1);
   // the frozen top plane nz-1 of the last chunk: psi' = psi, pi' = pi (injected)
   if (k1 == nz) {
-    const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
+    const uint32_t bc = (uint32_t)(nz - 1 - zb) * plane * (uint32_t)sizeof(C);
     buf_st<C, CPOL>(rop, ow, bc, prev.pp);
     buf_st<C, CPOL>(roq, ow, bc, prev.qq);
   }
@@ -1160,9 +1172,12 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
   const bool writer = wave_writes && gi < nx && gj < ny;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rbs = buf_rsrc(PB ? p.base : p.in);
-  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
-  const __amdgpu_buffer_rsrc_t rcm = buf_rsrc(p.cmp);
+  const int zb = max(tz * p.tile_z - 2, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc_from(p.in, zb, plane);
+  const __amdgpu_buffer_rsrc_t rbs = buf_rsrc_from(PB ? p.base : p.in, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc_from(p.out, zb, plane);
+  const __amdgpu_buffer_rsrc_t rcm = buf_rsrc_from(p.cmp, zb, plane);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;
   int oo = -1, hs = ss;
@@ -1191,7 +1206,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
     R v;
   };
   auto load_plane = [&](int m, Raw &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     w.s = buf_ld<C>(rin, oc, bc);
     w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
@@ -1243,7 +1258,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   C uprev = zero, umm = zero;  // level-1 outputs at m-1 and m-2 (this column)
   // the compensation of plane m (own column; planes outside the chunk are not loaded)
   auto load_cmp = [&](int m, C &c) {
-    const uint32_t bc = (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(C);
+    const uint32_t bc = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane * (uint32_t)sizeof(C);
     c = buf_ld<C>(rcm, (m >= k0 && m < k1) ? ow : kBufSkip, bc);
   };
   auto keep_of = [&](int k) {
@@ -1275,7 +1290,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
       const C sx = cadd<R>(lu[ss + 1], lu[ss - 1]), sy = cadd<R>(lu[ss + pw], lu[ss - pw]);
       const int t = max(k, m_start) - m_start;
       const C hyv = schr_h<R>(p, ra.v, lzc[2 * t], lzc[2 * t + 1], uprev, sx, sy, ucur, umm);
-      const uint32_t bc = (uint32_t)max(k, 0) * plane * (uint32_t)sizeof(C);
+      const uint32_t bc = (uint32_t)(max(k, zb) - zb) * plane * (uint32_t)sizeof(C);
       const uint32_t o_at = (k >= k0) ? ow : kBufSkip;
       if constexpr (KAHAN) {
         const C b = base_of(ra);
@@ -1314,7 +1329,7 @@ __device__ __forceinline__ void schr2_body(const Schr2Args<R> &p, const int tile
   if (m + 2 <= m_end) plane_step(m + 2, r2, r3, r0, r1, c0, c1);
   // the frozen top plane nz-1 of the last chunk: its level-1 output is b exactly
   if (k1 == nz) {
-    const uint32_t bc = (uint32_t)(nz - 1) * plane * (uint32_t)sizeof(C);
+    const uint32_t bc = (uint32_t)(nz - 1 - zb) * plane * (uint32_t)sizeof(C);
     buf_st_pol<C>(stream, rout, ow, bc, uprev);
     if constexpr (KAHAN) {
       C c;
@@ -1530,7 +1545,10 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   const int nlev = 4 - g.dyw;
   const bool writer_wave = g.dyw == 0;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const int zb = max(tz * p.tile_z - 4, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc_from(p.in, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc_from(p.out, zb, plane);
   const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
@@ -1573,7 +1591,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   // the outer psi on the waves that have outer lanes or a store that writes nothing on
   // the others), so the in-order counter waits exactly for the plane a step needs
   auto load_plane = [&](int m, Pl &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     w.b = buf_ld<C>(rin, oc, bc);
     w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
@@ -1662,7 +1680,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     const bool st = writer_wave && k4 >= k0 && k4 < k1;  // uniform
     C o = zero;
     if (st) o = level(L.l3[rd], p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
-    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
+    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)(max(k4, zb) - zb) * plane * (uint32_t)sizeof(C), o);
     finish(uc, pp1, n1, n2, n3);
   };
   // A steady-state step: all four levels on planes m-3..m, which lie outside the PML (real,
@@ -1680,7 +1698,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
     Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
     {
-      const uint32_t b = (uint32_t)(m + 2) * plane;
+      const uint32_t b = (uint32_t)(m + 2 - zb) * plane;
       const uint32_t bc = b * (uint32_t)sizeof(C);
       pp2.b = buf_ld<C>(rin, oc, bc);
       pp2.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
@@ -1696,7 +1714,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     if (nlev >= 2) n2 = level_fast(L.l1[rd], p.f2, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
     if (nlev >= 3) n3 = level_fast(L.l2[rd], p.f3, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
     if (writer_wave) o = level_fast(L.l3[rd], p.f4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
-    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), o);
+    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3 - zb) * plane * (uint32_t)sizeof(C), o);
     finish(uc, pp1, n1, n2, n3);
   };
   // the steady-state steps (uniform): m in [mf_lo, mf_hi], with m + 1 outside the source
@@ -1826,7 +1844,10 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
   const int i = ((c % nx) + nx) % nx;
   const bool writer = lane >= 4 && lane < kWaveX - 4 && c < nx;
   const uint32_t plane = (uint32_t)nx;
-  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(p.in), rv = buf_rsrc(p.V), rout = buf_rsrc(p.out);
+  const int zb = max(tz * p.tile_z - 4, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc_from(p.in, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc_from(p.out, zb, plane);
   const uint32_t oc = (uint32_t)i * sizeof(C), ov = (uint32_t)i * sizeof(R);
   const uint32_t ow = writer ? oc : kBufSkip;
   const C zero = mkc<R>(R(0), R(0));
@@ -1843,7 +1864,7 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
     R v;
   };
   auto load_plane = [&](int m, Pl &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     w.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
     w.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
   };
@@ -1916,7 +1937,7 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
     const bool st = k4 >= k0 && k4 < k1;  // uniform
     C o = zero;
     if (st) o = level(p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
-    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)max(k4, 0) * plane * (uint32_t)sizeof(C), o);
+    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)(max(k4, zb) - zb) * plane * (uint32_t)sizeof(C), o);
     rings(uc, n1, n2, n3);
   };
   // A steady-state step: the four levels on planes m-3..m, all outside the PML (interior,
@@ -1928,7 +1949,7 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
     Pl &pm3 = pq[(U + NR - 3) % NR], &pm2 = pq[(U + NR - 2) % NR], &pm1 = pq[(U + NR - 1) % NR];
     Pl &p0 = pq[U], &pp1 = pq[(U + 1) % NR], &pn = pq[(U + 1 + PF) % NR];
     {
-      const uint32_t b = (uint32_t)(m + 1 + PF) * plane;
+      const uint32_t b = (uint32_t)(m + 1 + PF - zb) * plane;
       pn.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
       pn.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
     }
@@ -1939,7 +1960,7 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
     const C n3 = horner<R>(pm2.b, p.f3, hreal(y3, xsum(y3), n2, l2q[(U + 1) & 1], pm2.v));
     const C y4 = l3q[(U + 1) & 1];
     const C o = horner<R>(pm3.b, p.f4, hreal(y4, xsum(y4), n3, l3q[U & 1], pm3.v));
-    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), o);
+    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3 - zb) * plane * (uint32_t)sizeof(C), o);
     rings(uc, n1, n2, n3);
   };
   // the steady-state steps (uniform): m in [mf_lo, mf_hi], m + 1 not a source plane
@@ -1965,7 +1986,7 @@ __device__ __forceinline__ void schr2d_body(const Schr1Args<R> &p, const int til
       Pl &pm2 = pq[(U + NR - 2) % NR], &pm1 = pq[(U + NR - 1) % NR];
       Pl &p0 = pq[U], &pp1 = pq[(U + 1) % NR], &pn = pq[(U + 1 + PF) % NR];
       {
-        const uint32_t b = (uint32_t)(m + 1 + PF) * plane;
+        const uint32_t b = (uint32_t)(m + 1 + PF - zb) * plane;
         pn.b = buf_ld<C>(rin, oc, b * (uint32_t)sizeof(C));
         pn.v = buf_ld<R>(rv, ov, b * (uint32_t)sizeof(R));
       }
@@ -2333,9 +2354,15 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
   const bool writer = wave_writes && gi < nx && gj < ny;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc(p.rec), rlam = buf_rsrc(p.lam);
-  const __amdgpu_buffer_rsrc_t ru3 = buf_rsrc(p.u3), rv3 = buf_rsrc(p.v3), rsa = buf_rsrc(p.sacc);
-  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc(p.lam_out), rvb = buf_rsrc(p.vbar), rv = buf_rsrc(p.V);
+  const int zb = max(tz * p.tile_z - 2, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc_from(p.rec, zb, plane);
+  const __amdgpu_buffer_rsrc_t rlam = buf_rsrc_from(p.lam, zb, plane);
+  const __amdgpu_buffer_rsrc_t ru3 = buf_rsrc_from(p.u3, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv3 = buf_rsrc_from(p.v3, zb, plane);
+  const __amdgpu_buffer_rsrc_t rsa = buf_rsrc_from(p.sacc, zb, plane);
+  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc_from(p.lam_out, zb, plane);
+  const __amdgpu_buffer_rsrc_t rvb = buf_rsrc_from(p.vbar, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;
   int oo = -1, hs = ss;
@@ -2427,7 +2454,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     R vb;      // the dL/dV accumulator
   };
   auto load_plane = [&](int m, Raw &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     const __amdgpu_buffer_rsrc_t rf = PASS == 1 ? rrec : ru3, ra = PASS == 1 ? rlam : rv3;
     w.f = buf_ld<C>(rf, oc, bc);
@@ -2439,7 +2466,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   };
   // the accumulator (and pass II's S, lam) of plane m for the writer lanes
   auto load_pt = [&](int m, Pt &w) {
-    const uint32_t bcr = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t bcr = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = bcr * (uint32_t)sizeof(C);
     const bool in = m >= k0 && m < k1;
     w.vb = buf_ld<R>(rvb, in ? owr : kBufSkip, bcr * (uint32_t)sizeof(R));
@@ -2529,7 +2556,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       }
     }
     {
-      const uint32_t bcr = (uint32_t)max(k, 0) * plane;
+      const uint32_t bcr = (uint32_t)(max(k, zb) - zb) * plane;
       const uint32_t bc = bcr * (uint32_t)sizeof(C);
       const uint32_t o = (k >= k0) ? ow : kBufSkip;
       const uint32_t orr = (k >= k0) ? owr : kBufSkip;
@@ -2643,8 +2670,12 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int nA = 4 - g.dyw, nF = 3 - g.dyw;  // levels this wave runs (uniform)
   const bool writer_wave = g.dyw == 0;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc(p.rec), rlam = buf_rsrc(p.lam), rv = buf_rsrc(p.V);
-  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc(p.lam_out), rvb = buf_rsrc(p.vbar);
+  const int zb = max(tz * p.tile_z - 4, 0);  // the lowest plane touched (m_start - 1 below)
+  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc_from(p.rec, zb, plane);
+  const __amdgpu_buffer_rsrc_t rlam = buf_rsrc_from(p.lam, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rlo = buf_rsrc_from(p.lam_out, zb, plane);
+  const __amdgpu_buffer_rsrc_t rvb = buf_rsrc_from(p.vbar, zb, plane);
   const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = g.writer ? oc : kBufSkip;
@@ -2724,7 +2755,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   // every wave issues the same vector-memory operations per plane (loads at kBufSkip
   // fetch nothing), so the in-order counter waits exactly for the plane a step needs
   auto load_plane = [&](int m, Pl &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     w.b = buf_ld<C>(rrec, oc, bc);
     w.l = buf_ld<C>(rlam, oc, bc);
@@ -2746,7 +2777,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   auto ld_vbar = [&](int k) {
     const bool in = k >= k0 && k < k1;
     return buf_ld<R>(rvb, in ? owr : kBufSkip,
-                     (uint32_t)min(max(k, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+                     (uint32_t)(min(max(k, zb), nz - 1) - zb) * plane * (uint32_t)sizeof(R));
   };
 
   Pl pq[6];  // plane m_start + q in slot q mod 6
@@ -2824,8 +2855,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       s1 = s0;
       gprev = gnew;
     }
-    buf_st<C, CPOL>(rlo, st3 ? ow : kBufSkip, (uint32_t)max(k3, 0) * plane * (uint32_t)sizeof(C), ln);
-    buf_st<R, CPOL>(rvb, st2 ? owr : kBufSkip, (uint32_t)max(k2, 0) * plane * (uint32_t)sizeof(R),
+    buf_st<C, CPOL>(rlo, st3 ? ow : kBufSkip, (uint32_t)(max(k3, zb) - zb) * plane * (uint32_t)sizeof(C), ln);
+    buf_st<R, CPOL>(rvb, st2 ? owr : kBufSkip, (uint32_t)(max(k2, zb) - zb) * plane * (uint32_t)sizeof(R),
                     vb_here + gsum);
     // ---- LDS planes one higher, register rings
     L.lb[wr][ss] = pp1.b;
@@ -2858,7 +2889,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
     Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
     {
-      const uint32_t b = (uint32_t)(m + 2) * plane;
+      const uint32_t b = (uint32_t)(m + 2 - zb) * plane;
       const uint32_t bc = b * (uint32_t)sizeof(C);
       pp2.b = buf_ld<C>(rrec, oc, bc);
       pp2.l = buf_ld<C>(rlam, oc, bc);
@@ -2867,7 +2898,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     }
     const R vb_here = vbc;
     vbc = vbn;
-    vbn = buf_ld<R>(rvb, owr, (uint32_t)m * plane * (uint32_t)sizeof(R));
+    vbn = buf_ld<R>(rvb, owr, (uint32_t)(m - zb) * plane * (uint32_t)sizeof(R));
     C nu4 = zero, nu3 = zero, nu2 = zero;
     if (nF >= 1) nu4 = horner<R>(p0.b, h4, hreal(L.lb[rd], p0.v, p0.b, pp1.b, pm1.b));
     if (nF >= 2) nu3 = horner<R>(pm1.b, h3, hreal(L.lu4[rd], pm1.v, u4q[Q], nu4, u4q[P]));
@@ -2891,8 +2922,8 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       s1 = s0;
       gprev = gnew;
     }
-    buf_st<C, CPOL>(rlo, ow, (uint32_t)(m - 3) * plane * (uint32_t)sizeof(C), ln);
-    buf_st<R, CPOL>(rvb, owr, (uint32_t)(m - 2) * plane * (uint32_t)sizeof(R), vb_here + gsum);
+    buf_st<C, CPOL>(rlo, ow, (uint32_t)(m - 3 - zb) * plane * (uint32_t)sizeof(C), ln);
+    buf_st<R, CPOL>(rvb, owr, (uint32_t)(m - 2 - zb) * plane * (uint32_t)sizeof(R), vb_here + gsum);
     L.lb[wr][ss] = pp1.b;
     L.ll[wr][ss] = pp1.l;
     if (has_o) L.ll[wr][hs] = pp1.ol;
@@ -3080,9 +3111,13 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const bool wave_writes = !side && wv >= 1 && wv <= ROWS - 2;
   const bool writer = wave_writes && gi < nx && gj < ny;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-  const __amdgpu_buffer_rsrc_t rup = buf_rsrc(p.u_psi), ruq = buf_rsrc(p.u_pi);
-  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(p.V);
-  const __amdgpu_buffer_rsrc_t rvb = buf_rsrc(p.vbar);
+  // the lowest plane touched (m_start - 1 below); the lambda pairs take a descriptor per
+  // plane (ld_pair, st_pair)
+  const int zb = max(tz * p.tile_z - 2, 0);
+  const __amdgpu_buffer_rsrc_t rup = buf_rsrc_from(p.u_psi, zb, plane);
+  const __amdgpu_buffer_rsrc_t ruq = buf_rsrc_from(p.u_pi, zb, plane);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
+  const __amdgpu_buffer_rsrc_t rvb = buf_rsrc_from(p.vbar, zb, plane);
   const int ij = j * nx + i;
   const int ss = (r + 1) * pw + c;
   int oo = -1, hs = ss;
@@ -3137,7 +3172,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     R v;
   };
   auto load_plane = [&](int m, Raw &w) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     w.s = buf_ld<C>(rup, oc, bc);
     w.q = buf_ld<C>(ruq, oc, bc);
@@ -3147,7 +3182,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     w.lq = l.q;
   };
   auto load_halo = [&](int m, Halo &o) {
-    const uint32_t b = (uint32_t)min(max(m, 0), nz - 1) * plane;
+    const uint32_t b = (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane;
     const uint32_t bc = b * (uint32_t)sizeof(C);
     o.s = buf_ld<C>(rup, ooc, bc);
     o.q = buf_ld<C>(ruq, ooc, bc);
@@ -3281,7 +3316,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
     Halo hn;
     if (wave_writes) {
       vb_next = buf_ld<R>(rvb, (m >= k0 && m < k1) ? owd : kBufSkip,
-                          (uint32_t)min(max(m, 0), nz - 1) * plane * (uint32_t)sizeof(R));
+                          (uint32_t)(min(max(m, zb), nz - 1) - zb) * plane * (uint32_t)sizeof(R));
     } else {
       vb_next = R(0);
       load_halo(m + 1, hn);
@@ -3376,7 +3411,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
       store_halo(dgt, wr, m + 1, hn);
     }
     {
-      const uint32_t b = (uint32_t)max(k, 0) * plane;
+      const uint32_t b = (uint32_t)(max(k, zb) - zb) * plane;
       st_pair<R>(stream, p.lam_out, (uint32_t)max(k, 0), plane, (k >= k0) ? owl : kBufSkip,
                  {lns, lnq});
       buf_st_pol<R>(stream, rvb, (k >= k0) ? owd : kBufSkip, b * (uint32_t)sizeof(R), vb + g);

@@ -136,8 +136,9 @@ class FDTDParams:
 
     def resolved_schedule(self):
         """The pass schedule the library runs for these parameters."""
-        # the fused passes address a field with 32-bit offsets
-        fits = self.nx * self.ny * self.nz * (16 if self.precision == "f64" else 8) < 2 ** 31
+        # the fused passes address a window of their z chunk and its halo (at least 10
+        # planes) with 32-bit offsets
+        fits = self.nx * self.ny * (16 if self.precision == "f64" else 8) * 10 < 2 ** 31
         if self.equation != "klein_gordon":
             if self.schedule == "auto":
                 if not fits:
