@@ -60,8 +60,10 @@ extern "C" {
                              t0 must be 0 (WAP_ERR_ARG otherwise) */
 
 /* pass schedule of a step */
-#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a field is under 2 GiB,
-                               * else KG_2PASS; Schrodinger with fields under 2 GiB:
+#define WAP_SCHED_AUTO 0      /* KG: KG_FUSED (every precision) while a plane is under
+                               * 2^31 / 10 bytes (a chunk window of the fused passes, at least
+                               * 10 planes, under 2 GiB), else KG_2PASS; Schrodinger with such
+                               * planes:
                                * SCHR_1PASS in f32 (and in f64 on 2D grids, ny = 1), else
                                * SCHR_FUSED with frozen end planes; else STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */

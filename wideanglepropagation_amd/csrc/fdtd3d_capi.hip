@@ -405,15 +405,14 @@ void split_hi_lo(double x, R &hi, R &lo) {
   lo = (R)(x - (double)hi);
 }
 
-// the Schrodinger fused schedule (every precision): frozen end planes (the 2D rule zeroes
-// the stage inputs on the end rows, which the RK4 polynomial form does not express),
-// fields < 2 GiB
+// the Schrodinger two-pass schedule (every precision): frozen end planes (its levels have
+// no end-plane factors for the 2D rule), chunk windows under 2 GiB
 bool schr_fused_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
          fused_plane_fits(d);
 }
 // the one-pass Schrodinger schedule: float32 without compensation (2D grids: float64 too),
-// frozen or zero end planes (its levels' end-plane factors), fields < 2 GiB
+// frozen or zero end planes (its levels' end-plane factors), chunk windows under 2 GiB
 bool schr_1pass_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && fused_plane_fits(d) &&
          (d.precision == WAP_PREC_F32 || (d.precision == WAP_PREC_F64 && d.ny == 1));
@@ -911,15 +910,16 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "unknown schedule");
   if (d->schedule == WAP_SCHED_SCHR_1PASS && !schr_1pass_ok(*d))
     return fail(WAP_ERR_ARG, "the one-pass Schrodinger schedule needs the Schrodinger equation "
-                             "in f32 (f32 or f64 on 2D grids, ny = 1) and fields under 2 GiB");
+                             "in f32 (f32 or f64 on 2D grids, ny = 1) and planes under "
+                             "2^31 / 10 bytes");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
     return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
-                             "frozen end planes and fields under 2 GiB");
+                             "frozen end planes and planes under 2^31 / 10 bytes");
   if ((d->schedule == WAP_SCHED_KG_2PASS || d->schedule == WAP_SCHED_KG_FUSED) &&
       d->equation != WAP_EQ_KLEIN_GORDON)
     return fail(WAP_ERR_ARG, "the 2-pass and fused schedules are for the Klein-Gordon equation");
   if (d->schedule == WAP_SCHED_KG_FUSED && !fused_plane_fits(*d))
-    return fail(WAP_ERR_ARG, "the fused schedule needs each field under 2 GiB");
+    return fail(WAP_ERR_ARG, "the fused schedule needs planes under 2^31 / 10 bytes");
   return WAP_OK;
 }
 

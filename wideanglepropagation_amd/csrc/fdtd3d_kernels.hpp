@@ -76,12 +76,13 @@ struct SrcCoef {
   cplx<R> c, cq;
 };
 
-// Raw buffer access (MUBUF): a descriptor in scalar registers for one whole field,
-// a uniform byte offset (the plane, in a scalar register) and a 32-bit per-lane byte
-// offset.  A lane offset of kBufSkip lies beyond the descriptor's range, and the
-// hardware drops such an access (a load returns 0 and touches no memory, a store
-// writes nothing), which replaces per-lane branches around loads and stores.  The
-// host runs these kernels only for fields under 2 GiB, so every real offset is in range.
+// Raw buffer access (MUBUF): a descriptor in scalar registers for a field (from the first
+// plane of the workgroup's window, buf_rsrc_from), a uniform byte offset (the plane, in a
+// scalar register) and a 32-bit per-lane byte offset.  A lane offset of kBufSkip lies
+// beyond the descriptor's range, and the hardware drops such an access (a load returns 0
+// and touches no memory, a store writes nothing), which replaces per-lane branches around
+// loads and stores.  The host keeps every window under 2 GiB, so every real offset is in
+// range.
 constexpr uint32_t kBufSkip = 0x80000000u;
 constexpr int kBufRange = 0x7fffffff;
 constexpr int kBufFlags = 0x00020000;  // descriptor word 3 on gfx9-family parts: 32-bit data format
