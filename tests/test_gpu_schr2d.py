@@ -190,3 +190,33 @@ def test_2d_adjoint_records_from_the_2d_step(rng):
     rel = rel_l2(grads[0], grads[1])
     print(f"2D one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
     assert rel < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 3e-6), ("f64", 1e-12)])
+def test_2d_reference_grid_against_c_oracle(prec, tol, rng):
+    # the reference's fdtd.cu grid, NX 1024 x NY 1398 (here nx x nz), through the drop-in
+    # 2D parameters (atomic units, float32 clock and constants), 40 steps of the default
+    # schedule (the one-pass 2D step, 4-plane chunks) against the FP64 C + OpenMP oracle
+    from oracle import fdtd3d_c
+    from wideanglepropagation_amd.fdtd_solver import RelativisticSolver
+    FDTD3D = _gpu()
+    s = RelativisticSolver()
+    p = s._params2d((1398, 1024), (-40.0, 40.0, -60.0, 60.0), 0.05, 4.0, 80.0, 120,
+                    {"precision": prec})
+    assert p.resolved_schedule() == "schr_1pass" and (p.nz, p.ny, p.nx) == (1398, 1, 1024)
+    print(f"case schr2d reference grid {prec}", flush=True)
+    V = (0.05 * rng.standard_normal(p.shape)).astype(np.float32)
+    psi = ((rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)) * 0.5)
+    sim = FDTD3D(p, V=V if prec == "f32" else V.astype(np.float64))
+    sim.set_state(psi)
+    sim.step(40)
+    got = sim.get_state(np.complex128)
+    sim.close()
+    want = fdtd3d_c.run(psi.astype(np.complex128), V.astype(np.float64), 40, p.oracle_dict(),
+                        f32_time=(p.time_mode == "f32"), threads=16)
+    err = rel_l2(got, want)
+    # the state moves ~1 % in 40 steps: the increments are compared too (float32 rounds the
+    # state to complex64 every step, ~4e-7 of it over 40 steps, ~3e-5 of the increment)
+    inc = rel_l2(got - psi, want - psi)
+    print(f"rel L2 vs C oracle: state {err:.3e}, increment {inc:.3e}", flush=True)
+    assert err < tol and inc < (1e-4 if prec == "f32" else 1e-10)
