@@ -64,6 +64,7 @@ PASS_BYTES = {
     # the four Horner levels in one pass: read psi_n, V; write psi_{n+1} (f64: 2D grids)
     ("schr_1pass", "schrodinger", "f32"): (20,),
     ("schr_1pass", "schrodinger", "f64"): (40,),
+    ("schr_1pass", "schrodinger", "f32_kahan"): (36,),   # + the compensation, read and written
 }
 
 

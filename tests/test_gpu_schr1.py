@@ -153,11 +153,13 @@ def test_schr1_bitwise_deterministic(rng):
 
 
 def test_schr1_rejects_other_precisions():
-    # float64 runs one pass on 2D grids only; f32_kahan never
+    # float64 runs one pass on 2D grids only, f32_kahan on 3D grids only
     FDTD3D = _gpu()
     from wideanglepropagation_amd._lib import WapError
+    from wideanglepropagation_amd.workloads import centred_extent
     p = _params((12, 8, 8), schedule="schr_1pass")
-    for bad in (p.with_(precision="f64"), p.with_(precision="f32_kahan")):
+    flat = p.with_(ny=1, extent=centred_extent((12, 1, 8), 0.05), precision="f32_kahan")
+    for bad in (p.with_(precision="f64"), flat):
         with pytest.raises(WapError, match="one-pass Schrodinger schedule"):
             FDTD3D(bad)
 
@@ -225,3 +227,52 @@ def test_schr1_adjoint_steady_state_range(shape, tz, k_source, pml, rng):
     rel = rel_l2(grads[0], grads[1])
     print(f"one-pass vs two-pass gradient rel L2 {rel:.3e}", flush=True)
     assert rel < 1e-4
+
+
+# the compensated one-pass step (f32_kahan): the oracle on ragged shapes and chunk edges,
+# a steady-state range split by the source band (whose level-4 planes take the keep factor
+# on the compensation), a source profile with the float32 clock
+@pytest.mark.parametrize("shape,tz,k_source", [((9, 5, 70), 1, 2), ((11, 13, 130), 100, 2),
+                                               ((12, 40, 125), 5, 2), ((60, 20, 64), 60, 30),
+                                               ((60, 20, 64), 16, 29)])
+def test_schr1_kahan_matches_oracle(shape, tz, k_source, rng):
+    print(f"case schr_1pass f32_kahan {shape} tz={tz} k_source={k_source}", flush=True)
+    p = _params(shape, tile_z=tz, pml_thick=0.15, k_source=k_source, schedule="schr_1pass")
+    p = p.with_(precision="f32_kahan")
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 10)
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < 2e-6
+
+
+def test_schr1_kahan_source_profile_and_reference_clock(rng):
+    print("case schr_1pass f32_kahan source profile + f32 clock", flush=True)
+    p = _params((20, 12, 16), reference_quantize=True, k_source=5, schedule="schr_1pass")
+    p = p.with_(precision="f32_kahan")
+    prof = np.exp(-np.linspace(-1, 1, 16)[None, :] ** 2 - np.linspace(-1, 1, 12)[:, None] ** 2)
+    prof = (prof * np.exp(0.3j)).astype(np.complex128)
+    V, psi = _case(p, rng)
+    got = _gpu_run(p, V, psi, 12, profile=prof)
+    assert rel_l2(got, _oracle(p, V, psi, 12, profile=prof)) < 2e-6
+
+
+def test_schr1_kahan_compensates(rng):
+    # over many small steps the compensated one-pass step stays closer to float64 than the
+    # plain float32 one, and as close as the compensated two-pass schedule (the same inputs;
+    # errors against the float64 two-pass schedule)
+    FDTD3D = _gpu()
+    print("case schr_1pass kahan vs plain f32, 1000 small steps", flush=True)
+    p = _params((48, 32, 64), pml_thick=0.1, k_source=4)
+    p = p.with_(precision="f64", schedule="schr_fused", dt=p.dt / 50)
+    V, psi = _case(p, rng)
+    out = {}
+    for name, q in (("f64", p), ("f32", p.with_(precision="f32", schedule="schr_1pass")),
+                    ("kahan1", p.with_(precision="f32_kahan", schedule="schr_1pass")),
+                    ("kahan2", p.with_(precision="f32_kahan", schedule="schr_fused"))):
+        sim = FDTD3D(q, V=V.astype(np.float64) if name == "f64" else V)
+        sim.set_state(psi)
+        sim.step(1000)
+        out[name] = sim.get_state(np.complex128)
+        sim.close()
+    err = {k: rel_l2(out[k], out["f64"]) for k in ("f32", "kahan1", "kahan2")}
+    print(f"rel L2 vs f64 after 1000 steps: {err}", flush=True)
+    assert err["kahan1"] < err["f32"] and err["kahan1"] < 2 * err["kahan2"] + 1e-9

@@ -411,11 +411,13 @@ bool schr_fused_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
          fused_plane_fits(d);
 }
-// the one-pass Schrodinger schedule: float32 without compensation (2D grids: float64 too),
-// frozen or zero end planes (its levels' end-plane factors), chunk windows under 2 GiB
+// the one-pass Schrodinger schedule: float32 (3D grids: with or without compensation; 2D
+// grids: float64 too), frozen or zero end planes (its levels' end-plane factors), chunk
+// windows under 2 GiB
 bool schr_1pass_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && fused_plane_fits(d) &&
-         (d.precision == WAP_PREC_F32 || (d.precision == WAP_PREC_F64 && d.ny == 1));
+         (d.precision == WAP_PREC_F32 || (d.precision == WAP_PREC_F32_KAHAN && d.ny > 1) ||
+          (d.precision == WAP_PREC_F64 && d.ny == 1));
 }
 
 int sched_of(const wap_handle *h) {
@@ -668,6 +670,7 @@ wap::Schr1Args<R> schr1_args(const wap_handle *h, bool batch = false) {
   a.f3 = (R)(dt / 2.0);
   a.f4 = (R)dt;
   // end-plane outputs of levels 1..3: b (frozen planes) or b/2, b/3, b/6 (the 2D rule)
+  a.cmp = is_kahan(h) ? (C *)h->cmp_psi : nullptr;  // compensation, updated in place
   const bool zero_ends = h->d.end_planes == WAP_ENDS_ZERO;
   a.g1 = zero_ends ? (R)0.5 : (R)1;
   a.g2 = zero_ends ? (R)(1.0 / 3.0) : (R)1;
@@ -734,7 +737,10 @@ int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
     if (two_d) {
       hipLaunchKernelGGL((wap::schr2d_pass<R>), grid, block, 0, h->stream, a);
     } else if constexpr (sizeof(R) == 4) {
-      hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
+      if (a.cmp)
+        hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows, true>), grid, block, 0, h->stream, a);
+      else
+        hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
     } else {
       return fail(WAP_ERR_ARG, "the one-pass Schrodinger step is float32 on 3D grids");
     }
@@ -910,8 +916,8 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "unknown schedule");
   if (d->schedule == WAP_SCHED_SCHR_1PASS && !schr_1pass_ok(*d))
     return fail(WAP_ERR_ARG, "the one-pass Schrodinger schedule needs the Schrodinger equation "
-                             "in f32 (f32 or f64 on 2D grids, ny = 1) and planes under "
-                             "2^31 / 10 bytes");
+                             "in f32 (with or without compensation on 3D grids; f32 or f64 on "
+                             "2D grids, ny = 1) and planes under 2^31 / 10 bytes");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
     return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
                              "frozen end planes and planes under 2^31 / 10 bytes");
@@ -1563,9 +1569,13 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
   if (b->sched == WAP_SCHED_SCHR_1PASS) {
     if constexpr (sizeof(R) == 4) {
       const wap::Schr1Args<R> *a = (const wap::Schr1Args<R> *)b->d_args[par];
-      hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows>), grid,
-                         dim3(wap::kWaveX, wap::kSchr1Rows + 1), 0, b->stream, a,
-                         b->d_tile_begin, n, ph);
+      const dim3 blk(wap::kWaveX, wap::kSchr1Rows + 1);
+      if (b->precision == WAP_PREC_F32_KAHAN)
+        hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows, true>), grid, blk, 0,
+                           b->stream, a, b->d_tile_begin, n, ph);
+      else
+        hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows>), grid, blk, 0, b->stream,
+                           a, b->d_tile_begin, n, ph);
     }
     return;
   }

@@ -1408,6 +1408,7 @@ struct Schr1Args {
   R hx, hy;
   R f1, f2, f3, f4;  // level factors h/4, h/3, h/2, h
   R g1, g2, g3;      // end-plane outputs of levels 1..3, times b
+  C *cmp;            // f32_kahan: the compensation of the state, updated in place
   // planes pz_lo..pz_hi (outside the PML) have both z weights equal to the real hz there
   int pz_lo, pz_hi;
   R hz;
@@ -1525,7 +1526,12 @@ __device__ __forceinline__ Schr1Lane schr1_lane(int tx, int tyi, int nx, int ny)
   return g;
 }
 
-template <typename R, int ROWS, int CPOL>
+// KAHAN (float32): the state is (hi, comp) with value hi - comp; the levels run on
+// b = inject(hi) and level 4 adds the step's increment h M u2 with the compensated update
+// of the stage passes (y = inc - keep comp, t = b + y, comp' = (t - b) - y; on the frozen
+// planes hi' = b and comp' = keep comp), the compensation of plane m-1 loaded at step m and
+// rewritten in place at step m+2 (+16 B: 36 B per voxel-step).
+template <typename R, int ROWS, int CPOL, bool KAHAN = false>
 __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile, const cplx<R> ph) {
   using C = cplx<R>;
   static_assert(ROWS >= 7 && 3 * ROWS - 10 <= kWaveX / 2, "rows");
@@ -1550,6 +1556,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc_from(p.in, zb, plane);
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc_from(p.V, zb, plane);
   const __amdgpu_buffer_rsrc_t rout = buf_rsrc_from(p.out, zb, plane);
+  const __amdgpu_buffer_rsrc_t rcm = buf_rsrc_from(p.cmp, zb, plane);
   const uint32_t oc = (uint32_t)ij * sizeof(C), ov = (uint32_t)ij * sizeof(R);
   const uint32_t ooc = has_o ? (uint32_t)oo * sizeof(C) : kBufSkip;
   const uint32_t ow = writer ? oc : kBufSkip;
@@ -1614,24 +1621,46 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     w.b = inj<R>(w.b, sp, sp.c, prof, ij);
     if (has_o) w.o = inj<R>(w.o, sp, sp.c, prof, oo);
   };
-  // one level at plane k: y its input there, zp / zm at k+1 / k-1, Lin the LDS plane of
-  // the input (lateral neighbours); gl b on the end planes
-  auto level = [&](const C *Lin, R fl, R gl, int k, C y, C zp, C zm, C base, R v) {
+  // H y at plane k: y the level's input there, zp / zm at k+1 / k-1, Lin the LDS plane of
+  // the input (lateral neighbours)
+  auto hlevel = [&](const C *Lin, int k, C y, C zp, C zm, R v) {
     const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
-    C hyv;
-    if (k >= p.pz_lo && k <= p.pz_hi) {  // uniform: outside the PML
-      hyv = schr_h_real<R>(p, v, p.hz, y, sx, sy, zp, zm);
-    } else {
-      const int t = k - zbase;
-      hyv = schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
-    }
+    if (k >= p.pz_lo && k <= p.pz_hi)  // uniform: outside the PML
+      return schr_h_real<R>(p, v, p.hz, y, sx, sy, zp, zm);
+    const int t = k - zbase;
+    return schr_h<R>(p, v, L.lzc[2 * t], L.lzc[2 * t + 1], y, sx, sy, zp, zm);
+  };
+  // one level at plane k: b + f M y, gl b on the end planes
+  auto level = [&](const C *Lin, R fl, R gl, int k, C y, C zp, C zm, C base, R v) {
+    const C hyv = hlevel(Lin, k, y, zp, zm, v);
     return (k > 0 && k < nz - 1) ? horner<R>(base, fl, hyv) : cscale<R>(base, gl);
+  };
+  auto keep_of = [&](int k) {
+    const int bnd = k - p.src_k0;
+    return (bnd >= 0 && bnd < p.src_n) ? L.lsrc[bnd].keep : R(1);
+  };
+  // the compensated last level (KAHAN): o = hi', cn = comp' from b, c = keep comp and H u2
+  auto kahan4 = [&](C b, C c, C hyv, bool in, C &o, C &cn) {
+    o = b;
+    cn = c;
+    if (in) {
+      const C y = csub<R>(mkc<R>(p.f4 * hyv.y, -p.f4 * hyv.x), c);  // inc - keep comp
+      o = cadd<R>(b, y);
+      cn = csub<R>(csub<R>(o, b), y);
+    }
+  };
+  // the compensation of plane k (writer lanes, planes of the chunk)
+  auto ld_cmp = [&](int k) {
+    return buf_ld<C>(rcm, (k >= k0 && k < k1) ? ow : kBufSkip,
+                     (uint32_t)(min(max(k, zb), nz - 1) - zb) * plane * (uint32_t)sizeof(C));
   };
 
   // plane m_start + q lives in ring slot q mod 6; levels 1..3 keep their last two planes
   // (plane k in slot (k - m_start) & 1)
   Pl pq[6];
   C l1q[2], l2q[2], l3q[2];
+  C cq[3];  // KAHAN: the compensation of plane k in slot (k - m_start) mod 3
+  cq[0] = cq[1] = cq[2] = zero;
   for (int q = 0; q < 6; ++q) {
     pq[q].b = pq[q].o = zero;
     pq[q].v = R(0);
@@ -1670,6 +1699,7 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
     Pl &pm3 = pq[(U + 3) % 6], &pm2 = pq[(U + 4) % 6], &pm1 = pq[(U + 5) % 6];
     Pl &p0 = pq[U], &pp1 = pq[(U + 1) % 6], &pp2 = pq[(U + 2) % 6];
     load_plane(m + 2, pp2);
+    if constexpr (KAHAN) cq[(U + 2) % 3] = ld_cmp(m - 1);
     inject_plane(m + 1, pp1);
     C n1 = zero, n2 = zero, n3 = zero;
     if (m >= lo1 && m <= hi1) n1 = level(L.lb[rd], p.f1, p.g1, m, p0.b, pp1.b, pm1.b, p0.b, p0.v);
@@ -1679,9 +1709,18 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
       n3 = level(L.l2[rd], p.f3, p.g3, m - 2, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
     const int k4 = m - 3;
     const bool st = writer_wave && k4 >= k0 && k4 < k1;  // uniform
-    C o = zero;
-    if (st) o = level(L.l3[rd], p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
-    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, (uint32_t)(max(k4, zb) - zb) * plane * (uint32_t)sizeof(C), o);
+    C o = zero, cn = zero;
+    if (st) {
+      if constexpr (KAHAN) {
+        const C hyv = hlevel(L.l3[rd], k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.v);
+        kahan4(pm3.b, cscale<R>(cq[U % 3], keep_of(k4)), hyv, k4 > 0 && k4 < nz - 1, o, cn);
+      } else {
+        o = level(L.l3[rd], p.f4, R(1), k4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+      }
+    }
+    const uint32_t b4 = (uint32_t)(max(k4, zb) - zb) * plane * (uint32_t)sizeof(C);
+    buf_st<C, CPOL>(rout, st ? ow : kBufSkip, b4, o);
+    if constexpr (KAHAN) buf_st<C, CPOL>(rcm, st ? ow : kBufSkip, b4, cn);
     finish(uc, pp1, n1, n2, n3);
   };
   // A steady-state step: all four levels on planes m-3..m, which lie outside the PML (real,
@@ -1710,12 +1749,25 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
         pp2.o = zero;
       }
     }
+    if constexpr (KAHAN) cq[(U + 2) % 3] = ld_cmp(m - 1);
     const C n1 = level_fast(L.lb[rd], p.f1, p0.b, pp1.b, pm1.b, p0.b, p0.v);
-    C n2 = zero, n3 = zero, o = zero;
+    C n2 = zero, n3 = zero, o = zero, cn = zero;
     if (nlev >= 2) n2 = level_fast(L.l1[rd], p.f2, l1q[(U + 1) & 1], n1, l1q[U & 1], pm1.b, pm1.v);
     if (nlev >= 3) n3 = level_fast(L.l2[rd], p.f3, l2q[U & 1], n2, l2q[(U + 1) & 1], pm2.b, pm2.v);
-    if (writer_wave) o = level_fast(L.l3[rd], p.f4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
-    buf_st<C, CPOL>(rout, ow, (uint32_t)(m - 3 - zb) * plane * (uint32_t)sizeof(C), o);
+    if (writer_wave) {
+      if constexpr (KAHAN) {  // no source plane among the steady state's level-4 planes
+        const C *Lin = L.l3[rd];
+        const C y4 = l3q[(U + 1) & 1];
+        const C sx = cadd<R>(Lin[ss + 1], Lin[ss - 1]), sy = cadd<R>(Lin[ss + pw], Lin[ss - pw]);
+        kahan4(pm3.b, cq[U % 3], schr_h_real<R>(p, pm3.v, p.hz, y4, sx, sy, n3, l3q[U & 1]),
+               true, o, cn);
+      } else {
+        o = level_fast(L.l3[rd], p.f4, l3q[(U + 1) & 1], n3, l3q[U & 1], pm3.b, pm3.v);
+      }
+    }
+    const uint32_t b4 = (uint32_t)(m - 3 - zb) * plane * (uint32_t)sizeof(C);
+    buf_st<C, CPOL>(rout, ow, b4, o);
+    if constexpr (KAHAN) buf_st<C, CPOL>(rcm, ow, b4, cn);
     finish(uc, pp1, n1, n2, n3);
   };
   // the steady-state steps (uniform): m in [mf_lo, mf_hi], with m + 1 outside the source
@@ -1723,7 +1775,9 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
   int mf_lo = max(k0 + 3, p.pz_lo + 3);
   int mf_hi = min(min(k1 + 2, p.pz_hi), nz - 3);
   if (p.src_n > 0) {
-    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2;  // steps injecting m + 1
+    // steps injecting m + 1 (KAHAN: also the steps whose level-4 plane m - 3 is a source
+    // plane, whose compensation takes the keep factor)
+    const int sb_lo = p.src_k0 - 1, sb_hi = p.src_k0 + p.src_n - 2 + (KAHAN ? 4 : 0);
     if (sb_lo <= mf_hi && sb_hi >= mf_lo) {
       if (sb_lo - mf_lo >= mf_hi - sb_hi)
         mf_hi = sb_lo - 1;
@@ -1766,18 +1820,18 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
 constexpr int kSchr1WavesPerEU = 4;
 constexpr int kSchr1PerCu = kSchr1WavesPerEU >= 8 ? 2 : 1;
 
-template <typename R, int ROWS>
+template <typename R, int ROWS, bool KAHAN = false>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass(
     const Schr1Args<R> p) {
   const int tile = zgroup_tile<kSchr1PerCu * kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz),
                                                       p.ntx * p.nty, p.ntz);
   if (p.stream_out)
-    schr1_body<R, ROWS, kStream>(p, tile, p.src_ph);
+    schr1_body<R, ROWS, kStream, KAHAN>(p, tile, p.src_ph);
   else
-    schr1_body<R, ROWS, kCached>(p, tile, p.src_ph);
+    schr1_body<R, ROWS, kCached, KAHAN>(p, tile, p.src_ph);
 }
 
-template <typename R, int ROWS>
+template <typename R, int ROWS, bool KAHAN = false>
 __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass_batched(
     const Schr1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
     const cplx<R> *__restrict__ ph) {
@@ -1785,9 +1839,9 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
   const int inst = find_instance(tile_begin, ninst, t);
   const Schr1Args<R> a = args[inst];
   if (a.stream_out)
-    schr1_body<R, ROWS, kStream>(a, t - tile_begin[inst], ph[inst]);
+    schr1_body<R, ROWS, kStream, KAHAN>(a, t - tile_begin[inst], ph[inst]);
   else
-    schr1_body<R, ROWS, kCached>(a, t - tile_begin[inst], ph[inst]);
+    schr1_body<R, ROWS, kCached, KAHAN>(a, t - tile_begin[inst], ph[inst]);
 }
 
 // ------------------------------------------ Schrodinger step in one pass, 2D grids
