@@ -39,7 +39,8 @@ def _make(cases):
 
 
 @pytest.mark.parametrize("eq,prec", [("schrodinger", "f32"), ("klein_gordon", "f32"),
-                                     ("klein_gordon", "f32_kahan"), ("schrodinger", "f64")])
+                                     ("klein_gordon", "f32_kahan"), ("schrodinger", "f64"),
+                                     ("schrodinger", "f32_kahan")])
 def test_batched_step_equals_single_grids(eq, prec, rng):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")

@@ -138,6 +138,15 @@ def test_3d_zero_end_planes_one_pass(rng):
     assert rel_l2(got, _oracle(p, V, psi, 10)) < 2e-6
 
 
+def test_zero_end_planes_reject_compensation():
+    # the 2D rule is the reference's 2D program's, which has no compensated precision
+    FDTD3D = _gpu()
+    from wideanglepropagation_amd._lib import WapError
+    p = _params((18, 12, 70), end_planes="zero", precision="f32_kahan")
+    with pytest.raises(WapError, match="zero end planes are for the f32/f64 Schrodinger"):
+        FDTD3D(p)
+
+
 def test_2d_batch_matches_oracle(rng):
     # batched 2D grids run the 3D one-pass kernel with the 2D end-plane factors
     _gpu()
