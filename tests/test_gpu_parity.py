@@ -175,15 +175,14 @@ def test_kg_fused_is_the_default_in_every_precision():
 
 
 def test_schr_fused_is_the_default_in_every_precision():
-    # float32 and f32_kahan: the four Horner levels in one pass; float64: two passes
+    # every precision: the four Horner levels in one pass (two passes on request)
     p = _params((24, 16, 16), "schrodinger", precision="f32")
     assert p.resolved_schedule() == "schr_1pass"
-    assert p.with_(precision="f64").resolved_schedule() == "schr_fused"
+    assert p.with_(precision="f64").resolved_schedule() == "schr_1pass"
     assert p.with_(precision="f32_kahan").resolved_schedule() == "schr_1pass"
-    # the 2D rule (zero stage inputs on the end rows): one pass in float32 (its end-plane
-    # factors), one pass per stage in float64 on 3D grids
+    # the 2D rule (zero stage inputs on the end rows): one pass too (its end-plane factors)
     assert p.with_(end_planes="zero").resolved_schedule() == "schr_1pass"
-    assert p.with_(end_planes="zero", precision="f64").resolved_schedule() == "stages"
+    assert p.with_(end_planes="zero", precision="f64").resolved_schedule() == "schr_1pass"
 
 
 def test_schr_fused_kahan_compensates(rng):

@@ -128,14 +128,15 @@ def test_2d_one_pass_equals_stages(rng):
     assert err < 2e-6
 
 
-def test_3d_zero_end_planes_one_pass(rng):
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_3d_zero_end_planes_one_pass(prec, rng):
     # the end-plane factors of the 3D one-pass kernel (zero end planes on a 3D grid)
-    print("case schr_1pass 3D zero end planes", flush=True)
-    p = _params((18, 12, 70), end_planes="zero", schedule="schr_1pass", tile_z=5)
+    print(f"case schr_1pass 3D zero end planes {prec}", flush=True)
+    p = _params((18, 12, 70), end_planes="zero", schedule="schr_1pass", tile_z=5, precision=prec)
     assert p.with_(schedule="auto").resolved_schedule() == "schr_1pass"
     V, psi = _case(p, rng)
     got = _gpu_run(p, V, psi, 10)
-    assert rel_l2(got, _oracle(p, V, psi, 10)) < 2e-6
+    assert rel_l2(got, _oracle(p, V, psi, 10)) < (1e-12 if prec == "f64" else 2e-6)
 
 
 def test_zero_end_planes_reject_compensation():
@@ -175,7 +176,7 @@ def test_2d_float64_batch_rejected(rng):
     from wideanglepropagation_amd.batched import BatchedFDTD
     p = _params((22, 1, 40), precision="f64")
     s = FDTD3D(p, V=np.zeros(p.shape))
-    with pytest.raises(WapError, match="batched one-pass Schrodinger grids are float32"):
+    with pytest.raises(WapError, match="batched float64 one-pass Schrodinger grids are 3D"):
         BatchedFDTD([s])
     s.close()
 

@@ -411,13 +411,12 @@ bool schr_fused_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && d.end_planes == WAP_ENDS_FROZEN &&
          fused_plane_fits(d);
 }
-// the one-pass Schrodinger schedule: float32 (3D grids: with or without compensation; 2D
-// grids: float64 too), frozen or zero end planes (its levels' end-plane factors), chunk
+// the one-pass Schrodinger schedule: float32 and float64 (3D grids: float32 with
+// compensation too), frozen or zero end planes (its levels' end-plane factors), chunk
 // windows under 2 GiB
 bool schr_1pass_ok(const wap_desc &d) {
   return d.equation == WAP_EQ_SCHRODINGER && fused_plane_fits(d) &&
-         (d.precision == WAP_PREC_F32 || (d.precision == WAP_PREC_F32_KAHAN && d.ny > 1) ||
-          (d.precision == WAP_PREC_F64 && d.ny == 1));
+         (d.precision != WAP_PREC_F32_KAHAN || d.ny > 1);
 }
 
 int sched_of(const wap_handle *h) {
@@ -636,11 +635,14 @@ int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
 }
 
 // One-pass Schrodinger step (schr1_pass): kSchr1Rows row waves (8 interior rows of 64
-// columns) and a side wave per workgroup, kSchr1PerCu workgroups per CU.  A chunk of tz
-// planes evaluates 4 tz + 12 levels (the z halo of levels 1..3), which the chunk choice
-// counts as 3 extra planes.
+// columns; float64: 11 rows, 5 interior) and a side wave per workgroup, kSchr1PerCu
+// workgroups per CU.  A chunk of tz planes evaluates 4 tz + 12 levels (the z halo of levels
+// 1..3), which the chunk choice counts as 3 extra planes.
+int schr1_rows(const wap_handle *h) {
+  return is_f64(h) ? wap::kSchr1RowsOf<double> : wap::kSchr1RowsOf<float>;
+}
 int schr1_nty(const wap_handle *h) {
-  constexpr int out_rows = wap::kSchr1Rows - 6;
+  const int out_rows = schr1_rows(h) - 6;
   return (h->d.ny + out_rows - 1) / out_rows;
 }
 int schr1_tile_z(const wap_handle *h, bool batch = false) {
@@ -725,7 +727,8 @@ int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
                     double *pass_ms, bool record = false) {
   using C = wap::cplx<R>;
   const bool two_d = schr2d(h);
-  const dim3 block = two_d ? dim3(wap::kWaveX) : dim3(wap::kWaveX, wap::kSchr1Rows + 1);
+  constexpr int kRows = wap::kSchr1RowsOf<R>;
+  const dim3 block = two_d ? dim3(wap::kWaveX) : dim3(wap::kWaveX, kRows + 1);
   const dim3 grid((unsigned)(two_d ? schr2d_tiles(h) : schr1_tiles(h)));
   wap::Schr1Args<R> a = two_d ? schr2d_args<R>(h) : schr1_args<R>(h);
   if (record) a.stream_out = 1;
@@ -738,11 +741,11 @@ int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
       hipLaunchKernelGGL((wap::schr2d_pass<R>), grid, block, 0, h->stream, a);
     } else if constexpr (sizeof(R) == 4) {
       if (a.cmp)
-        hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows, true>), grid, block, 0, h->stream, a);
+        hipLaunchKernelGGL((wap::schr1_pass<R, kRows, true>), grid, block, 0, h->stream, a);
       else
-        hipLaunchKernelGGL((wap::schr1_pass<R, wap::kSchr1Rows>), grid, block, 0, h->stream, a);
+        hipLaunchKernelGGL((wap::schr1_pass<R, kRows>), grid, block, 0, h->stream, a);
     } else {
-      return fail(WAP_ERR_ARG, "the one-pass Schrodinger step is float32 on 3D grids");
+      hipLaunchKernelGGL((wap::schr1_pass<R, kRows>), grid, block, 0, h->stream, a);
     }
     std::swap(h->psi, h->yb_psi);
     if (ev) {
@@ -916,8 +919,8 @@ int validate(const wap_desc *d) {
     return fail(WAP_ERR_ARG, "unknown schedule");
   if (d->schedule == WAP_SCHED_SCHR_1PASS && !schr_1pass_ok(*d))
     return fail(WAP_ERR_ARG, "the one-pass Schrodinger schedule needs the Schrodinger equation "
-                             "in f32 (with or without compensation on 3D grids; f32 or f64 on "
-                             "2D grids, ny = 1) and planes under 2^31 / 10 bytes");
+                             "in f32 or f64 (f32_kahan on 3D grids only) and planes under "
+                             "2^31 / 10 bytes");
   if (d->schedule == WAP_SCHED_SCHR_FUSED && !schr_fused_ok(*d))
     return fail(WAP_ERR_ARG, "the Schrodinger fused schedule needs the Schrodinger equation, "
                              "frozen end planes and planes under 2^31 / 10 bytes");
@@ -1292,7 +1295,7 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (sched_of(h) == WAP_SCHED_SCHR_1PASS) {
-    if (tile_y) *tile_y = schr2d(h) ? 1 : wap::kSchr1Rows + 1;
+    if (tile_y) *tile_y = schr2d(h) ? 1 : schr1_rows(h) + 1;
     if (tile_z) *tile_z = schr2d(h) ? schr2d_tile_z(h) : schr1_tile_z(h);
     if (groups) *groups = schr2d(h) ? schr2d_tiles(h) : schr1_tiles(h);
     return WAP_OK;
@@ -1567,15 +1570,16 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
     return;
   }
   if (b->sched == WAP_SCHED_SCHR_1PASS) {
-    if constexpr (sizeof(R) == 4) {
-      const wap::Schr1Args<R> *a = (const wap::Schr1Args<R> *)b->d_args[par];
-      const dim3 blk(wap::kWaveX, wap::kSchr1Rows + 1);
-      if (b->precision == WAP_PREC_F32_KAHAN)
-        hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows, true>), grid, blk, 0,
-                           b->stream, a, b->d_tile_begin, n, ph);
-      else
-        hipLaunchKernelGGL((wap::schr1_pass_batched<R, wap::kSchr1Rows>), grid, blk, 0, b->stream,
-                           a, b->d_tile_begin, n, ph);
+    constexpr int kRows = wap::kSchr1RowsOf<R>;
+    const wap::Schr1Args<R> *a = (const wap::Schr1Args<R> *)b->d_args[par];
+    const dim3 blk(wap::kWaveX, kRows + 1);
+    if (b->precision == WAP_PREC_F32_KAHAN) {
+      if constexpr (sizeof(R) == 4)
+        hipLaunchKernelGGL((wap::schr1_pass_batched<R, kRows, true>), grid, blk, 0, b->stream, a,
+                           b->d_tile_begin, n, ph);
+    } else {
+      hipLaunchKernelGGL((wap::schr1_pass_batched<R, kRows>), grid, blk, 0, b->stream, a,
+                         b->d_tile_begin, n, ph);
     }
     return;
   }
@@ -1672,9 +1676,10 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
     if (!h->v_set) return fail(WAP_ERR_STATE, "potential not set on a batched handle");
   }
   const int s0 = sched_of(h0);
-  if (s0 == WAP_SCHED_SCHR_1PASS && is_f64(h0))
-    return fail(WAP_ERR_ARG, "batched one-pass Schrodinger grids are float32 (float64 2D grids: "
-                             "create the handles with the stage schedule)");
+  for (int i = 0; i < n; ++i)
+    if (s0 == WAP_SCHED_SCHR_1PASS && is_f64(h0) && handles[i]->d.ny == 1)
+      return fail(WAP_ERR_ARG, "batched float64 one-pass Schrodinger grids are 3D (float64 2D "
+                               "grids: create the handles with the stage schedule)");
   const bool fused = s0 == WAP_SCHED_KG_FUSED || s0 == WAP_SCHED_SCHR_FUSED ||
                      s0 == WAP_SCHED_SCHR_1PASS;
   if (fused ? (tile_y != 0 && tile_y < 4) : tile_y > 8)
@@ -1687,7 +1692,7 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
   b->kg2 = kg_two_pass(h0) && !fused;
   b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
   if (is_f64(h0) && is_kg(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
-  if (s0 == WAP_SCHED_SCHR_1PASS) b->waves = wap::kSchr1Rows + 1;  // one fixed shape
+  if (s0 == WAP_SCHED_SCHR_1PASS) b->waves = schr1_rows(h0) + 1;  // one fixed shape
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;

@@ -143,10 +143,9 @@ class FDTDParams:
             if self.schedule == "auto":
                 if not fits:
                     return "stages"
-                # one pass: float32 (either end rule), float32 + Kahan on 3D grids,
-                # float64 on 2D grids (ny = 1)
-                if (self.precision == "f32" or (self.precision == "f32_kahan" and self.ny > 1)
-                        or (self.precision == "f64" and self.ny == 1)):
+                # one pass: float32 and float64 (either end rule), float32 + Kahan on 3D
+                # grids
+                if self.precision in ("f32", "f64") or self.ny > 1:
                     return "schr_1pass"
                 return "schr_fused" if self.end_planes == "frozen" else "stages"
             return self.schedule
