@@ -64,8 +64,7 @@ extern "C" {
                                * 2^31 / 10 bytes (a chunk window of the fused passes, at least
                                * 10 planes, under 2 GiB), else KG_2PASS; Schrodinger with such
                                * planes:
-                               * SCHR_1PASS in f32 (f32_kahan on 3D grids, f64 on 2D grids,
-                               * ny = 1), else
+                               * SCHR_1PASS (f32_kahan on 3D grids only), else
                                * SCHR_FUSED with frozen end planes; else STAGES */
 #define WAP_SCHED_STAGES 1    /* one streaming pass per RK4 stage (4 per step) */
 #define WAP_SCHED_KG_2PASS 2  /* the same RK4 polynomial in 2 stencil passes */
@@ -73,11 +72,11 @@ extern "C" {
                                * f32_kahan updates its compensation in place) */
 #define WAP_SCHED_SCHR_FUSED 4  /* Schrodinger, frozen end planes, every precision: the RK4
                                  * polynomial in Horner form, two passes of two stencil levels
-                                 * each (AUTO picks it for f64 on 3D grids) */
-#define WAP_SCHED_SCHR_1PASS 5  /* Schrodinger f32 (f32_kahan on 3D grids, updating its
-                                 * compensation in place; f64 on 2D grids), frozen or zero
-                                 * end planes: the four Horner levels in one pass (20 B per
-                                 * voxel-step, f32_kahan 36; 2D grids: one wave per 56-column
+                                 * each */
+#define WAP_SCHED_SCHR_1PASS 5  /* Schrodinger f32 and f64 (f32_kahan on 3D grids, updating
+                                 * its compensation in place), frozen or zero end planes: the
+                                 * four Horner levels in one pass (20 B per voxel-step,
+                                 * f32_kahan 36, f64 40; 2D grids: one wave per 56-column
                                  * tile and z chunk) */
 
 /* planes k = 0 and nz-1 (never stepped) */
