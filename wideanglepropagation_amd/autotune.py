@@ -143,14 +143,8 @@ def _group_of(p):
 
 
 # workgroup waves of the schedules whose shape is fixed (tile_y does not apply): the
-# one-pass Schrodinger step runs 14 row waves and a side wave (float64: 11 and a side wave)
-FIXED_WAVES = {("schr_1pass", "f32"): 15, ("schr_1pass", "f32_kahan"): 15,
-               ("schr_1pass", "f64"): 12}
-
-
-def fixed_waves(schedule, precision, default=None):
-    """The workgroup waves of a fixed-shape schedule in this precision, else default."""
-    return FIXED_WAVES.get((schedule, precision), default)
+# one-pass Schrodinger step runs 14 row waves and a side wave
+FIXED_WAVES = {"schr_1pass": 15}
 
 
 def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, steps=4, warmup=2,
@@ -165,7 +159,7 @@ def tune_batch(param_list, tile_zs=(16, 32, 64), tile_ys=(8, 16), tile_y=16, ste
     groups = sorted({_group_of(p) for p in param_list}, key=str)
 
     def label(choice, waves):
-        return {"/".join((c[0], "x".join(map(str, c[3])))): (fixed_waves(c[2], c[1], waves[c[:3]]), tz)
+        return {"/".join((c[0], "x".join(map(str, c[3])))): (FIXED_WAVES.get(c[2], waves[c[:3]]), tz)
                 for c, tz in choice.items()}
 
     def run(choice, waves):

@@ -635,14 +635,11 @@ int run_steps_schr2(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
 }
 
 // One-pass Schrodinger step (schr1_pass): kSchr1Rows row waves (8 interior rows of 64
-// columns; float64: 11 rows, 5 interior) and a side wave per workgroup, kSchr1PerCu
-// workgroups per CU.  A chunk of tz planes evaluates 4 tz + 12 levels (the z halo of levels
-// 1..3), which the chunk choice counts as 3 extra planes.
-int schr1_rows(const wap_handle *h) {
-  return is_f64(h) ? wap::kSchr1RowsOf<double> : wap::kSchr1RowsOf<float>;
-}
+// columns) and a side wave per workgroup, kSchr1PerCu workgroups per CU.  A chunk of tz
+// planes evaluates 4 tz + 12 levels (the z halo of levels 1..3), which the chunk choice
+// counts as 3 extra planes.
 int schr1_nty(const wap_handle *h) {
-  const int out_rows = schr1_rows(h) - 6;
+  constexpr int out_rows = wap::kSchr1Rows - 6;
   return (h->d.ny + out_rows - 1) / out_rows;
 }
 int schr1_tile_z(const wap_handle *h, bool batch = false) {
@@ -727,7 +724,7 @@ int run_steps_schr1(wap_handle *h, int64_t nsteps, int64_t step0, double t0, hip
                     double *pass_ms, bool record = false) {
   using C = wap::cplx<R>;
   const bool two_d = schr2d(h);
-  constexpr int kRows = wap::kSchr1RowsOf<R>;
+  constexpr int kRows = wap::kSchr1Rows;
   const dim3 block = two_d ? dim3(wap::kWaveX) : dim3(wap::kWaveX, kRows + 1);
   const dim3 grid((unsigned)(two_d ? schr2d_tiles(h) : schr1_tiles(h)));
   wap::Schr1Args<R> a = two_d ? schr2d_args<R>(h) : schr1_args<R>(h);
@@ -1295,7 +1292,7 @@ int wap_device_bytes(wap_handle *h, int64_t *out) {
 int wap_launch_shape(wap_handle *h, int32_t *tile_y, int32_t *tile_z, int64_t *groups) {
   if (!h) return fail(WAP_ERR_ARG, "handle is NULL");
   if (sched_of(h) == WAP_SCHED_SCHR_1PASS) {
-    if (tile_y) *tile_y = schr2d(h) ? 1 : schr1_rows(h) + 1;
+    if (tile_y) *tile_y = schr2d(h) ? 1 : wap::kSchr1Rows + 1;
     if (tile_z) *tile_z = schr2d(h) ? schr2d_tile_z(h) : schr1_tile_z(h);
     if (groups) *groups = schr2d(h) ? schr2d_tiles(h) : schr1_tiles(h);
     return WAP_OK;
@@ -1570,7 +1567,7 @@ void batch_launch_fused(wap_batch *b, int64_t t, const wap::cplx<R> *ph) {
     return;
   }
   if (b->sched == WAP_SCHED_SCHR_1PASS) {
-    constexpr int kRows = wap::kSchr1RowsOf<R>;
+    constexpr int kRows = wap::kSchr1Rows;
     const wap::Schr1Args<R> *a = (const wap::Schr1Args<R> *)b->d_args[par];
     const dim3 blk(wap::kWaveX, kRows + 1);
     if (b->precision == WAP_PREC_F32_KAHAN) {
@@ -1692,7 +1689,7 @@ int wap_batch_create(wap_handle *const *handles, int n, int tile_y, wap_batch **
   b->kg2 = kg_two_pass(h0) && !fused;
   b->waves = tile_y > 0 ? tile_y : kg1_waves(h0);
   if (is_f64(h0) && is_kg(h0) && b->waves > 8) b->waves = 8;  // as kg1_waves
-  if (s0 == WAP_SCHED_SCHR_1PASS) b->waves = schr1_rows(h0) + 1;  // one fixed shape
+  if (s0 == WAP_SCHED_SCHR_1PASS) b->waves = wap::kSchr1Rows + 1;  // one fixed shape
   b->precision = h0->d.precision;
   b->equation = h0->d.equation;
   b->tile_y = tile_y > 0 ? tile_y : h0->tile_y;

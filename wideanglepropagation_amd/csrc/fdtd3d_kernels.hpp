@@ -1419,10 +1419,6 @@ struct Schr1Args {
 };
 
 constexpr int kSchr1Rows = 14;           // row waves (8 interior rows) plus one side wave
-// the row waves per precision: float64 holds its rings in ~135 VGPRs, so its workgroup is
-// 11 row waves (5 interior rows) and the side wave at three waves per SIMD
-template <typename R>
-constexpr int kSchr1RowsOf = sizeof(R) == 4 ? kSchr1Rows : 11;
 constexpr int kSchr1Pitch = kWaveX + 9;  // LDS row pitch: columns -4 .. 67 and one unused slot
 constexpr int kSchr1Zc = kKG1MaxTileZ + 16;  // z weights of planes k0-7 .. k1+8
 
@@ -1458,18 +1454,12 @@ struct Schr1Lane {
 // 4 - dy forward levels (the side wave 3) and the adjoint 7 - 2 dy stencils (the side wave
 // 5).  In row order the busiest SIMD holds 14 forward levels against 11.75 on average
 // (adjoint: 24 stencils against 19.75), and the barrier of every plane step waits for it;
-// with 14 rows the slots below give 12, 12, 12, 11 (adjoint 20, 20, 20, 19); with 11 rows
-// (float64) 9, 9, 9, 8 forward levels against 11 in row order.
+// with 14 rows the slots below give 12, 12, 12, 11 (adjoint 20, 20, 20, 19).
 template <int ROWS>
 __device__ __forceinline__ int schr1_wave_row(int slot) {
   if constexpr (ROWS == 14) {
     // slot 0..14 -> row 3, 5, 7, 9, 4, 6, 8, 10, 2, 11, 1, side, 0, 13, 12 (a nibble each)
     constexpr uint64_t kRowOfSlot = 0xcd0e1b2a8649753ull;
-    return (int)((kRowOfSlot >> (4 * slot)) & 15);
-  }
-  if constexpr (ROWS == 11) {
-    // slot 0..11 -> row 3, 5, 7, side, 4, 6, 2, 8, 0, 10, 1, 9
-    constexpr uint64_t kRowOfSlot = 0x91a08264b753ull;
     return (int)((kRowOfSlot >> (4 * slot)) & 15);
   }
   return slot;
@@ -1827,14 +1817,15 @@ __device__ __forceinline__ void schr1_body(const Schr1Args<R> &p, const int tile
 // with it, held to 64 VGPRs, the body spills ~80 registers and runs 0.66 against 0.42 ms
 // at 512 x 512 x 256 (one workgroup per CU, steady-state step: 161 Gvoxel-steps/s at
 // 256 x 256 x 512, against 132 for two workgroups per CU without it).
-// float64: one 12-wave workgroup per CU (135 VGPRs, 128 KB of LDS), three waves per SIMD.
+// float64 runs the same 15-wave workgroup (155 KB of LDS): its rings want ~135 VGPRs, and
+// held to 128 it spills 20 of them, yet at 256 x 256 x 512 it runs 109-110 against 76
+// Gvoxel-steps/s for 11-row workgroups at three waves per SIMD without spills (the wider
+// tile writes 8 of 14 rows instead of 5 of 11, and four waves per SIMD hide more latency).
 constexpr int kSchr1WavesPerEU = 4;
 constexpr int kSchr1PerCu = kSchr1WavesPerEU >= 8 ? 2 : 1;
-template <typename R>
-constexpr int kSchr1WavesPerEUOf = sizeof(R) == 4 ? kSchr1WavesPerEU : 3;
 
 template <typename R, int ROWS, bool KAHAN = false>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEUOf<R>))) void schr1_pass(
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass(
     const Schr1Args<R> p) {
   const int tile = zgroup_tile<kSchr1PerCu * kXcdCus>(xcd_tile(blockIdx.x, p.ntx * p.nty * p.ntz),
                                                       p.ntx * p.nty, p.ntz);
@@ -1845,7 +1836,7 @@ __global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per
 }
 
 template <typename R, int ROWS, bool KAHAN = false>
-__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEUOf<R>))) void schr1_pass_batched(
+__global__ __launch_bounds__(kWaveX *(ROWS + 1)) __attribute__((amdgpu_waves_per_eu(kSchr1WavesPerEU))) void schr1_pass_batched(
     const Schr1Args<R> *__restrict__ args, const int *__restrict__ tile_begin, int ninst,
     const cplx<R> *__restrict__ ph) {
   const int t = xcd_tile(blockIdx.x, tile_begin[ninst]);

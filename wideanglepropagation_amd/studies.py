@@ -145,7 +145,7 @@ def probe_profile(shape, extent, x0, y0, lam, width=1.0, focus=-5.0):
 
 def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
     from .batched import BatchedFDTD
-    from .autotune import fixed_waves, tune_batch
+    from .autotune import FIXED_WAVES, tune_batch
     rng = np.random.default_rng(1)
     classes = [(96, 96, 96), (128, 128, 128), (256, 160, 160), (192, 192, 192)]
     if quick:
@@ -214,7 +214,7 @@ def batched_study(quick=False, n_inst=48, nsteps=1000, autotune=True):
            "gvoxel_steps_per_s_device": vox * done / (dev_ms * 1e-3) / 1e9,
            "graph_steps_per_launch": chunk, "launch_groups": b.tiles(),
            "autotuned": {f"{k[0]}/{'x'.join(map(str, k[3]))}":
-                         {"waves": fixed_waves(k[2], k[1], v[0]), "tile_z": v[1]}
+                         {"waves": FIXED_WAVES.get(k[2], v[0]), "tile_z": v[1]}
                          for k, v in tuned.items()},
            "autotuned_ms_per_batched_step": tuned_ms,
            "all_finite": bool(np.all(np.isfinite(norms))),
